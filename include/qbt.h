@@ -1,0 +1,197 @@
+/*
+ * qbt.h - C ABI of libqbt.so, the MI355X (gfx950) HIP layer of
+ * qubatron-mi355x.
+ *
+ * It replaces the OpenGL connector of the reference (src/qubatron/octree_glc.c)
+ * for the per-pixel sparse-octree ray traversal, shadow test and shading:
+ *
+ *   reference (octree_glc.c)                 this library
+ *   octree_glc_init            :93-247       qbt_create
+ *   octree_glc_upload_texbuffer_data :361-498 qbt_upload (same arguments)
+ *   octree_glc_update          :249-352      qbt_update (same arguments) /
+ *                                            qbt_render (explicit views+lights)
+ *   (GL context teardown)                    qbt_destroy
+ *   octree_trace_line (octree.c:341-537)     qbt_trace_rays (batched, on GPU)
+ *   octree_insert_point loop (modelutil.c:203-217)
+ *                                            qbt_build_octree (on GPU)
+ *
+ * Plain pointers and sizes only. All entry points return QBT_OK (0) or a
+ * negative QBT_E* code (qbt_host.h); qbt_strerror names it. A context owns one
+ * HIP stream and its device buffers; it is not thread-safe, calls on one
+ * context must be serialised by the caller (the reference is single-threaded
+ * on its GL thread, qubatron.c:678-679).
+ */
+#ifndef QBT_H
+#define QBT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "qbt_host.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct qbt_ctx qbt_ctx;
+
+/* Buffer ids, in the order of octree_glc_buffer_t (octree_glc.c:16-24);
+ * the two POINTS buffers are an extension feeding qbt_build_octree. */
+enum {
+    QBT_BUFFER_STATIC_COLOR   = 0,
+    QBT_BUFFER_STATIC_NORMAL  = 1,
+    QBT_BUFFER_STATIC_OCTREE  = 2,
+    QBT_BUFFER_DYNAMIC_COLOR  = 3,
+    QBT_BUFFER_DYNAMIC_NORMAL = 4,
+    QBT_BUFFER_DYNAMIC_OCTREE = 5,
+    QBT_BUFFER_STATIC_POINTS  = 6,
+    QBT_BUFFER_DYNAMIC_POINTS = 7
+};
+
+/* Element types: the GL enum values the reference passes (GL_INT, GL_FLOAT). */
+#define QBT_TYPE_INT 0x1404
+#define QBT_TYPE_FLOAT 0x1406
+
+typedef struct qbt_opts {
+    int    device;           /* HIP device ordinal */
+    size_t max_device_bytes; /* cap on the library's device allocations, 0 = none */
+} qbt_opts;
+
+/* octree_glc_init: picks the device, creates the stream. opts may be NULL. */
+int         qbt_create(const qbt_opts* opts, qbt_ctx** out);
+void        qbt_destroy(qbt_ctx* ctx);
+const char* qbt_strerror(int code);
+/* Device memory currently held by the context, bytes. */
+size_t      qbt_device_bytes(const qbt_ctx* ctx);
+
+/* octree_glc_upload_texbuffer_data (octree_glc.c:361-498), same arguments.
+ * data: host array of `size` bytes; bytes [start, end) are copied into the
+ * device buffer `buftype`. itemsize is 16 for an octree (one int4 texel, three
+ * per 48-byte node) and 12 for float3 colours/normals/points; start, end and
+ * size must be multiples of itemsize, start <= end <= size. When size exceeds
+ * the buffer's capacity the buffer is reallocated with slack and the whole
+ * array [0, size) is copied, as the reference does on texture regrowth
+ * (octree_glc.c:412-428). size also sets the buffer's logical length. Copies
+ * are complete when the call returns (data may be freed). */
+int qbt_upload(qbt_ctx* ctx, const void* data, int type, size_t size, size_t itemsize, size_t start, size_t end,
+               int buftype);
+
+/* Copy bytes [start, end) of a buffer's host-layout image back to host
+ * (octree buffers in the 48-byte node layout, float3 buffers as float3). */
+int qbt_download(qbt_ctx* ctx, int buftype, void* host, size_t start, size_t end);
+
+/* One camera view: uniforms of octree_fsh.c:9-15. */
+typedef struct qbt_view {
+    float camfp[3];    /* camera position */
+    float angle[2];    /* yaw (angle_in.x), pitch (angle_in.y) */
+    float width;       /* render dimensions (uniform `dimensions`) */
+    float height;
+    int   shoot;       /* gunshot light term */
+    float basecube[4]; /* x_min, y_max, z_max, edge */
+    int   maxlevel;    /* leaf depth */
+} qbt_view;
+
+/* Outputs (device frame of each view). */
+#define QBT_OUT_HIT 0x001u    /* int32 static model index of the hit leaf, -1 none */
+#define QBT_OUT_LEAF 0x002u   /* int32 static leaf node index, -1 none */
+#define QBT_OUT_SHADOW 0x004u /* uint8 bit l = visible from light l */
+#define QBT_OUT_RGBA8 0x008u  /* uint32 RGBA8, r in the low byte */
+#define QBT_OUT_RGBAF 0x010u  /* float4 colour before clamping */
+#define QBT_OUT_ISP 0x020u    /* float4 primary entry point (x, y, z, t) */
+#define QBT_OUT_FLAGS 0x040u  /* uint8 QBT_PX_* */
+#define QBT_OUT_DYN 0x080u    /* int32 dynamic model index (slot 8), -1 none */
+#define QBT_OUT_EXP 0x100u    /* int32 node expansions (primary + shadow) */
+#define QBT_OUT_DEFAULT (QBT_OUT_HIT | QBT_OUT_LEAF | QBT_OUT_SHADOW | QBT_OUT_RGBA8)
+
+/* Render flags (high bits of the same word). */
+#define QBT_RENDER_COUNT 0x10000u     /* count node expansions (stats + QBT_OUT_EXP) */
+#define QBT_RENDER_SURROGATE 0x20000u /* shadow bits from the MLP outside its band */
+
+/* Pixel flags. */
+#define QBT_PX_DISCARD 1u /* fragment discarded: colour (0,0,0,0) */
+#define QBT_PX_LEAF 2u    /* primary trace reached a leaf */
+#define QBT_PX_SHADED 4u  /* isp.t > 0: shadow traces and shading ran */
+#define QBT_PX_SPRITE 8u  /* light sprite painted white */
+#define QBT_PX_PREDICTED 16u /* surrogate decided the shadow bits */
+
+#define QBT_MAX_LIGHTS_RENDER 8
+#define QBT_MAX_VIEWS 256
+
+/* Render nviews views with the same nlights point lights (lights: nlights x 3
+ * floats) into the context's device frames (one per view index). Pixel
+ * (px, py), py = 0 the bottom row, is stored at py * (int)width + px.
+ * out_mask selects outputs (QBT_OUT_*) and flags (QBT_RENDER_*). Asynchronous
+ * on the context stream; qbt_read_frame / qbt_get_stats synchronise. */
+int qbt_render(qbt_ctx* ctx, const qbt_view* views, int nviews, const float* lights, int nlights,
+               unsigned out_mask);
+
+/* octree_glc_update (octree_glc.c:249-352), same arguments: derives the
+ * light (420, 200 - sin(lighta)*20, 680 - sin(lighta)*200), basecube
+ * {0, basesize, basesize, basesize} and dimensions width/(6 - quality/2),
+ * then renders view 0 with QBT_OUT_RGBA8 (+ hit, leaf, shadow). */
+int qbt_update(qbt_ctx* ctx, float width, float height, const float position[3], const float angle[3],
+               float lighta, uint8_t quality, int maxlevel, float basesize, int shoot);
+
+/* Host copies of one view's frame; NULL members are skipped. Sizes are
+ * w*h elements (w = (int)width, h = (int)height of that view). */
+typedef struct qbt_frame {
+    int       w, h;
+    int32_t*  hit;
+    int32_t*  leaf;
+    uint8_t*  shadow;
+    uint32_t* rgba8;
+    float*    rgbaf; /* 4 per pixel */
+    float*    isp;   /* 4 per pixel */
+    uint8_t*  flags;
+    int32_t*  dyn;
+    int32_t*  exp;
+} qbt_frame;
+
+int qbt_read_frame(qbt_ctx* ctx, int view, qbt_frame* out);
+/* Device pointers of one view's frame (valid until the next render with a
+ * larger frame); members not rendered are NULL. */
+int qbt_frame_device(qbt_ctx* ctx, int view, qbt_frame* out);
+
+typedef struct qbt_stats {
+    double   kernel_ms;      /* device time of the last qbt_render / qbt_trace_rays */
+    double   trace_ms;       /* time of the traversal kernel(s) only */
+    uint64_t primary_rays;   /* pixels */
+    uint64_t shadow_rays;    /* shadow traces performed */
+    uint64_t hits;           /* pixels with isp.t > 0 */
+    uint64_t expansions;     /* node expansions (QBT_RENDER_COUNT only) */
+    uint64_t predicted;      /* shadow queries answered by the surrogate */
+    uint32_t launches;       /* kernel launches of the last call */
+    uint32_t grid_exact;     /* the basecube grid was exact in fp32 */
+    uint32_t dual;           /* the dynamic tree took part */
+    uint32_t reserved;
+} qbt_stats;
+
+int qbt_get_stats(qbt_ctx* ctx, qbt_stats* out);
+
+/* Batched octree_trace_line: n rays (origins, dirs: n x 3 floats, host).
+ * semantics 0 = shader (octree_fsh.c cube_trace_line, dual tree), 1 = CPU
+ * twin (octree.c:341-537: static tree only, its plane-test returns). Outputs
+ * (host, any may be NULL): status (1 hit, 0 miss, -1 no entry),
+ * model (slot 8 of the static leaf, -1 none), leaf node, isp (4 floats),
+ * tlf (leaf cube, 4 floats). basecube/maxlevel as in qbt_view. */
+int qbt_trace_rays(qbt_ctx* ctx, const float* origins, const float* dirs, size_t n, int semantics,
+                   const float basecube[4], int maxlevel, int32_t* status, int32_t* model, int32_t* leaf,
+                   float* isp, float* tlf);
+
+/* Build the static or dynamic octree (tree = QBT_BUFFER_STATIC_OCTREE or
+ * QBT_BUFFER_DYNAMIC_OCTREE) on the GPU from the first npoints points of the
+ * matching POINTS buffer, giving the node array that inserting the points in
+ * index order with octree_insert_point (octree.c:95-147) produces, node for
+ * node. *nodes_out receives the node count (the reference's len). */
+int qbt_build_octree(qbt_ctx* ctx, int tree, const float basecube[4], int levels, size_t npoints,
+                     size_t* nodes_out);
+
+/* Library version string. */
+const char* qbt_version(void);
+/* Text of the last HIP runtime error seen by the library (QBT_EDEVICE). */
+const char* qbt_last_hip_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1083 @@
+/*
+ * qbt_core.hip - context, buffer uploads, validation, rendering and batched
+ * ray queries of libqbt.so (gfx950).
+ *
+ * Device layout (per tree): int4 child[2*n] (the 8 child indices of a node in
+ * one 32-byte aligned block) and int model[n] (slot 8), split from the
+ * reference's 48-byte nodes (octree.c:11-14) while uploading. Per point
+ * model: float4 ca[2*n] = {colour rgb 1, normal xyz 1}, so the leaf fetch of
+ * a primary hit reads one 32-byte block.
+ */
+#include <hip/hip_runtime.h>
+
+#include <float.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <new>
+
+#include "qbt.h"
+#include "qbt_internal.hpp"
+#include "trace.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace qbt;
+
+/* ------------------------------------------------------------------------ */
+/* kernels: upload scatter, validation, counters                            */
+/* ------------------------------------------------------------------------ */
+
+/* octree texels [t0, t0+count) from staging into child/model arrays */
+__global__ void k_scatter_octree(const int4* __restrict__ stage, size_t t0, size_t count, int4* __restrict__ child,
+                                 int* __restrict__ model)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    size_t t    = t0 + i;
+    size_t node = t / 3;
+    int    k    = (int) (t - node * 3);
+    int4   v    = stage[i];
+    if (k < 2) child[2 * node + k] = v;
+    else model[node] = v.x;
+}
+
+/* float3 items [i0, i0+count) into the .xyz of ca[2*i + which] */
+__global__ void k_scatter_attr(const float* __restrict__ stage, size_t i0, size_t count, float4* __restrict__ ca,
+                               int which)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    ca[2 * (i0 + i) + which] = make_float4(stage[3 * i], stage[3 * i + 1], stage[3 * i + 2], 1.0f);
+}
+
+__global__ void k_scatter_points(const float* __restrict__ stage, size_t i0, size_t count, float* __restrict__ xyz)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * count) return;
+    xyz[3 * i0 + i] = stage[i];
+}
+
+/* device -> host image of a tree: texels [t0, t0+count) */
+__global__ void k_gather_octree(int4* __restrict__ stage, size_t t0, size_t count, const int4* __restrict__ child,
+                                const int* __restrict__ model)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    size_t t    = t0 + i;
+    size_t node = t / 3;
+    int    k    = (int) (t - node * 3);
+    stage[i]    = k < 2 ? child[2 * node + k] : make_int4(model[node], 0, 0, 0);
+}
+
+__global__ void k_gather_attr(float* __restrict__ stage, size_t i0, size_t count, const float4* __restrict__ ca,
+                              int which)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    float4 v          = ca[2 * (i0 + i) + which];
+    stage[3 * i]      = v.x;
+    stage[3 * i + 1]  = v.y;
+    stage[3 * i + 2]  = v.z;
+}
+
+/* every child index of nodes [lo, hi) must lie in [0, n) */
+__global__ void k_validate(const int4* __restrict__ child, size_t lo, size_t hi, int n,
+                           unsigned int* __restrict__ bad)
+{
+    size_t i = lo + (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hi) return;
+    int4 a = child[2 * i], b = child[2 * i + 1];
+    bool ok = (unsigned) a.x < (unsigned) n && (unsigned) a.y < (unsigned) n && (unsigned) a.z < (unsigned) n &&
+              (unsigned) a.w < (unsigned) n && (unsigned) b.x < (unsigned) n && (unsigned) b.y < (unsigned) n &&
+              (unsigned) b.z < (unsigned) n && (unsigned) b.w < (unsigned) n;
+    if (!ok) atomicAdd(bad, 1u);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+    int i = threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+/* ------------------------------------------------------------------------ */
+/* pixel program (octree_fsh.c:381-464)                                     */
+/* ------------------------------------------------------------------------ */
+
+__device__ __forceinline__ void cross3(const float a[3], const float b[3], float r[3])
+{
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* v + 2 * cross(q.xyz, cross(q.xyz, v) + q.w * v) (octree_fsh.c:392-395) */
+__device__ __forceinline__ void quat_rotate(const float q[4], const float v[3], float r[3])
+{
+    float c1[3], s[3], c2[3];
+    cross3(q, v, c1);
+    s[0] = c1[0] + q[3] * v[0];
+    s[1] = c1[1] + q[3] * v[1];
+    s[2] = c1[2] + q[3] * v[2];
+    cross3(q, s, c2);
+    r[0] = v[0] + 2.0f * c2[0];
+    r[1] = v[1] + 2.0f * c2[1];
+    r[2] = v[2] + 2.0f * c2[2];
+}
+
+__device__ __forceinline__ float dot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+__device__ __forceinline__ void normalize3(const float v[3], float r[3])
+{
+    float s = 1.0f / sqrtf(dot3(v, v));
+    r[0]    = v[0] * s;
+    r[1]    = v[1] * s;
+    r[2]    = v[2] * s;
+}
+
+__device__ __forceinline__ float gmax(float x, float y) { return x < y ? y : x; } /* GLSL max */
+
+__device__ __forceinline__ uint32_t pack_rgba8(float c0, float c1, float c2, float c3)
+{
+    float    c[4] = {c0, c1, c2, c3};
+    uint32_t r    = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        float x = c[k];
+        if (!(x > 0.0f)) x = 0.0f;
+        if (x > 1.0f) x = 1.0f;
+        r |= ((uint32_t) (x * 255.0f + 0.5f)) << (8 * k);
+    }
+    return r;
+}
+
+__device__ __forceinline__ void fetch_attr(const AttrDev& a, int m, float col[4], float nrm[4])
+{
+    if (m >= 0 && m < a.n) {
+        float4 c = a.ca[2 * (size_t) m], v = a.ca[2 * (size_t) m + 1];
+        col[0] = c.x, col[1] = c.y, col[2] = c.z, col[3] = 1.0f;
+        nrm[0] = v.x, nrm[1] = v.y, nrm[2] = v.z, nrm[3] = 1.0f;
+    } else {
+        col[0] = col[1] = col[2] = col[3] = 0.0f;
+        nrm[0] = nrm[1] = nrm[2] = nrm[3] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ void pixel_dir(const ViewDev& v, int px, int py, float csv[3])
+{
+    float c0[3] = {((float) px + 0.5f) - v.W / 2.0f, ((float) py + 0.5f) - v.H / 2.0f, 0.0f - (v.W / 2.0f) / 1.0f};
+    float c1[3];
+    quat_rotate(v.qz, c0, c1);
+    quat_rotate(v.qx, c1, csv);
+}
+
+__device__ __forceinline__ unsigned wave_sum(unsigned x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+    return x;
+}
+
+__device__ __forceinline__ Stack make_stack(uint32_t* lds, int L, int dual)
+{
+    Stack st;
+    int   nt  = blockDim.x;
+    int   tid = threadIdx.x;
+    st.stride = nt;
+    st.pw     = lds + tid;
+    st.ns     = reinterpret_cast<int*>(lds) + L * nt + tid;
+    st.nd     = dual ? reinterpret_cast<int*>(lds) + 2 * L * nt + tid : st.ns;
+    return st;
+}
+
+/* per-view, per-light sprite trace (octree_fsh.c:452-459): the ray from the
+ * camera to the light is the same for every pixel of a view. */
+template <bool DUAL, bool GRID>
+__global__ void k_sprite(SceneDev s, ViewDev v, int* __restrict__ out)
+{
+    extern __shared__ uint32_t lds[];
+    int                        l = threadIdx.x;
+    Stack                      st = make_stack(lds, s.maxlevel, DUAL);
+    if (l >= v.nlights) return;
+    Ray r = {v.cam[0], v.cam[1], v.cam[2], v.light[l][0] - v.cam[0], v.light[l][1] - v.cam[1],
+             v.light[l][2] - v.cam[2]};
+    Hit h;
+    int stt   = trace<DUAL, false, GRID>(s, r, st, h);
+    float ix  = stt == ST_HIT ? h.isp.x : 0.0f;
+    float resv = ix - v.cam[0];
+    int   f   = 0;
+    if (stt == ST_DISCARD) f |= 1;
+    if (resv / r.dx > 1.0f) f |= 2;
+    if (stt == ST_ABORT) f |= 4;
+    out[l] = f;
+}
+
+template <bool DUAL, bool GRID, bool COUNT>
+__global__ void __launch_bounds__(256) k_render(SceneDev s, ViewDev v, const int* __restrict__ sprite, FrameDev f,
+                                                unsigned mask, unsigned long long* __restrict__ counters)
+{
+    extern __shared__ uint32_t lds[];
+    const Stack st = make_stack(lds, s.maxlevel, DUAL);
+
+    /* tiles of 16 x (blockDim/16) pixels */
+    const int tw = 16, th = blockDim.x / 16;
+    const int tiles_x = (v.w + tw - 1) / tw;
+    const int tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+    const int px = tx * tw + (threadIdx.x % tw), py = ty * th + (threadIdx.x / tw);
+    const bool valid = px < v.w && py < v.h;
+
+    unsigned n_shadow = 0, n_hit = 0, n_exp = 0, n_abort = 0;
+    if (valid) {
+        const size_t o = (size_t) py * v.w + px;
+        float        csv[3];
+        pixel_dir(v, px, py, csv);
+        Ray r = {v.cam[0], v.cam[1], v.cam[2], csv[0], csv[1], csv[2]};
+        Hit h;
+        int stt = trace<DUAL, false, GRID>(s, r, st, h);
+        if (COUNT) n_exp += h.exps;
+
+        int32_t  hit = -1, leaf = -1, dyn = -1;
+        uint32_t flg = 0, sbits = 0;
+        float    col[4] = {0.f, 0.f, 0.f, 0.f};
+        float4   isp    = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (stt == ST_ABORT) n_abort++;
+        if (stt == ST_DISCARD) {
+            flg |= QBT_PX_DISCARD;
+        } else {
+            float nrm[4] = {0.f, 0.f, 0.f, 0.f};
+            if (stt == ST_HIT) {
+                const int L  = s.maxlevel;
+                int       ms = (h.ns == 0 && L > 0) ? 0 : s.stat.model[h.ns];
+                int       md = 0;
+                if (DUAL) md = (h.nd == 0 && L > 0) ? 0 : s.dyn.model[h.nd];
+                fetch_attr(s.astat, ms, col, nrm);
+                if (DUAL && md > 0) fetch_attr(s.adyn, md, col, nrm);
+                flg |= QBT_PX_LEAF;
+                hit  = ms;
+                leaf = h.ns;
+                dyn  = md;
+                isp  = h.isp;
+            }
+            if (stt == ST_HIT && h.isp.w > 0.0f) {
+                flg |= QBT_PX_SHADED;
+                n_hit++;
+                float nn[3], mc[3] = {-csv[0], -csv[1], -csv[2]}, nmc[3];
+                normalize3(nrm, nn);
+                normalize3(mc, nmc);
+                float camv = gmax(dot3(nmc, nn), 0.0f);
+                float fac  = 0.1f + 0.2f * camv;
+                for (int l = 0; l < v.nlights; l++) {
+                    Ray lr = {v.light[l][0], v.light[l][1], v.light[l][2], h.isp.x - v.light[l][0],
+                              h.isp.y - v.light[l][1], h.isp.z - v.light[l][2]};
+                    Hit lh;
+                    int ls = trace<DUAL, false, GRID>(s, lr, st, lh);
+                    n_shadow++;
+                    if (COUNT) n_exp += lh.exps;
+                    if (ls == ST_ABORT) n_abort++;
+                    if (ls == ST_DISCARD) {
+                        flg |= QBT_PX_DISCARD;
+                        break;
+                    }
+                    float4 li   = ls == ST_HIT ? lh.isp : make_float4(0.f, 0.f, 0.f, 0.f);
+                    float  ix   = li.x - h.isp.x, iy = li.y - h.isp.y, iz = li.z - h.isp.z;
+                    float  sqr  = ix * ix + iy * iy + iz * iz;
+                    float  bit  = (15.0f < sqr) ? 0.0f : 1.0f; /* step(sqr, 15.0) */
+                    if (bit != 0.0f) sbits |= 1u << l;
+                    float ml[3] = {-lr.dx, -lr.dy, -lr.dz}, nml[3];
+                    normalize3(ml, nml);
+                    float lam = gmax(dot3(nml, nn), 0.0f);
+                    fac       = fac + lam * bit * 0.7f;
+                }
+                if (!(flg & QBT_PX_DISCARD)) {
+                    col[0] = col[0] * fac;
+                    col[1] = col[1] * fac;
+                    col[2] = col[2] * fac;
+                    col[2] *= 0.7f;
+                    float g = (float) v.shoot * camv * 0.1f;
+                    col[0] += g;
+                    col[1] += g;
+                    col[2] += g;
+                }
+            }
+            if (!(flg & QBT_PX_DISCARD) && v.nlights > 0) {
+                float ncsv[3];
+                normalize3(csv, ncsv);
+                for (int l = 0; l < v.nlights; l++) {
+                    if (dot3(v.cl[l], ncsv) > v.cos_thr) {
+                        int sf = sprite[l];
+                        if (sf & 1) {
+                            flg |= QBT_PX_DISCARD;
+                            break;
+                        }
+                        if (sf & 2) {
+                            col[0] = col[1] = col[2] = col[3] = 1.0f;
+                            flg |= QBT_PX_SPRITE;
+                        }
+                    }
+                }
+            }
+        }
+        if (flg & QBT_PX_DISCARD) {
+            col[0] = col[1] = col[2] = col[3] = 0.0f;
+            sbits                             = 0;
+        }
+        if (mask & QBT_OUT_HIT) f.hit[o] = hit;
+        if (mask & QBT_OUT_LEAF) f.leaf[o] = leaf;
+        if (mask & QBT_OUT_DYN) f.dyn[o] = dyn;
+        if (mask & QBT_OUT_SHADOW) f.shadow[o] = (uint8_t) sbits;
+        if (mask & QBT_OUT_FLAGS) f.flags[o] = (uint8_t) flg;
+        if (mask & QBT_OUT_RGBA8) f.rgba8[o] = pack_rgba8(col[0], col[1], col[2], col[3]);
+        if (mask & QBT_OUT_RGBAF) f.rgbaf[o] = make_float4(col[0], col[1], col[2], col[3]);
+        if (mask & QBT_OUT_ISP) f.isp[o] = isp;
+        if (COUNT && (mask & QBT_OUT_EXP)) f.exp[o] = (int) n_exp;
+    }
+    /* per-wave totals: shadow traces, shaded pixels, expansions, aborts */
+    n_shadow = wave_sum(n_shadow);
+    n_hit    = wave_sum(n_hit);
+    n_abort  = wave_sum(n_abort);
+    if (COUNT) n_exp = wave_sum(n_exp);
+    if ((threadIdx.x & 63) == 0) {
+        if (n_shadow) atomicAdd(&counters[CNT_SHADOW], (unsigned long long) n_shadow);
+        if (n_hit) atomicAdd(&counters[CNT_HITS], (unsigned long long) n_hit);
+        if (n_abort) atomicAdd(&counters[CNT_ABORT], (unsigned long long) n_abort);
+        if (COUNT && n_exp) atomicAdd(&counters[CNT_EXP], (unsigned long long) n_exp);
+    }
+}
+
+/* batched octree_trace_line */
+template <bool DUAL, bool CPU, bool GRID>
+__global__ void __launch_bounds__(256) k_trace_rays(SceneDev s, const float* __restrict__ org,
+                                                     const float* __restrict__ dir, size_t n, int* __restrict__ status,
+                                                     int* __restrict__ model, int* __restrict__ leaf,
+                                                     float4* __restrict__ isp, float4* __restrict__ tlf,
+                                                     unsigned long long* __restrict__ counters)
+{
+    extern __shared__ uint32_t lds[];
+    const Stack st = make_stack(lds, s.maxlevel, DUAL);
+    size_t      i  = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned    ab = 0;
+    if (i < n) {
+        Ray r = {org[3 * i], org[3 * i + 1], org[3 * i + 2], dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
+        Hit h;
+        int stt = trace<DUAL, CPU, GRID>(s, r, st, h);
+        if (stt == ST_ABORT) ab = 1;
+        int    m = -1, lf = -1;
+        float4 ip = make_float4(0.f, 0.f, 0.f, 0.f), tc = ip;
+        if (stt == ST_HIT) {
+            const int L = s.maxlevel;
+            m           = (h.ns == 0 && L > 0) ? 0 : s.stat.model[h.ns];
+            lf          = h.ns;
+            ip          = h.isp;
+            float x, y, z, w;
+            cube_at<GRID>(s, L, h.ix, h.iy, h.iz, x, y, z, w);
+            tc = make_float4(x, y, z, w);
+        }
+        if (status) status[i] = stt == ST_ABORT ? -2 : stt;
+        if (model) model[i] = m;
+        if (leaf) leaf[i] = lf;
+        if (isp) isp[i] = ip;
+        if (tlf) tlf[i] = tc;
+    }
+    ab = wave_sum(ab);
+    if ((threadIdx.x & 63) == 0 && ab) atomicAdd(&counters[CNT_ABORT], (unsigned long long) ab);
+}
+
+/* ------------------------------------------------------------------------ */
+/* host side                                                                */
+/* ------------------------------------------------------------------------ */
+
+#define HIPCHK(x) QBT_HIPCHK(x)
+
+static char g_last_hip[256];
+
+void qbt_set_hip_error(hipError_t e, const char* what, const char* file, int line)
+{
+    snprintf(g_last_hip, sizeof(g_last_hip), "%s: %s (%s:%d)", hipGetErrorString(e), what, file, line);
+}
+
+extern "C" const char* qbt_last_hip_error(void) { return g_last_hip; }
+
+int qbt_ctx::dev_alloc(DevBuf& b, size_t bytes)
+{
+    if (bytes <= b.cap) return QBT_OK;
+    size_t grow = bytes - b.cap;
+    if (max_bytes && held + grow > max_bytes) return QBT_ENOMEM;
+    if (b.p) {
+        HIPCHK(hipFree(b.p));
+        held -= b.cap;
+        b.p   = nullptr;
+        b.cap = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        (void) hipGetLastError();
+        b.p = nullptr;
+        return QBT_ENOMEM;
+    }
+    b.cap = bytes;
+    held += bytes;
+    return QBT_OK;
+}
+
+void qbt_ctx::dev_free(DevBuf& b)
+{
+    if (b.p) {
+        (void) hipFree(b.p);
+        held -= b.cap;
+    }
+    b.p   = nullptr;
+    b.cap = 0;
+}
+
+static const size_t kStageBytes = 64u << 20;
+
+extern "C" int qbt_create(const qbt_opts* opts, qbt_ctx** out)
+{
+    if (!out) return QBT_EINVAL;
+    *out    = nullptr;
+    int dev = opts ? opts->device : 0;
+    int cnt = 0;
+    if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0) {
+        (void) hipGetLastError();
+        return QBT_EDEVICE;
+    }
+    if (dev < 0 || dev >= cnt) return QBT_EINVAL;
+    qbt_ctx* c = new (std::nothrow) qbt_ctx();
+    if (!c) return QBT_ENOMEM;
+    c->device    = dev;
+    c->max_bytes = opts ? opts->max_device_bytes : 0;
+    if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess) {
+        (void) hipGetLastError();
+        delete c;
+        return QBT_EDEVICE;
+    }
+    if (c->dev_alloc(c->stage, kStageBytes) || c->dev_alloc(c->counters, sizeof(unsigned long long) * CNT_N) ||
+        c->dev_alloc(c->sprite, sizeof(int) * QBT_MAX_LIGHTS_RENDER * QBT_MAX_VIEWS)) {
+        qbt_destroy(c);
+        return QBT_ENOMEM;
+    }
+    *out = c;
+    return QBT_OK;
+}
+
+extern "C" void qbt_destroy(qbt_ctx* c)
+{
+    if (!c) return;
+    (void) hipSetDevice(c->device);
+    if (c->stream) (void) hipStreamSynchronize(c->stream);
+    for (int t = 0; t < 2; t++) {
+        c->dev_free(c->tree[t].child);
+        c->dev_free(c->tree[t].model);
+        c->dev_free(c->attr[t].ca);
+        c->dev_free(c->pts[t].xyz);
+    }
+    for (auto& fr : c->frames)
+        for (int k = 0; k < FR_N; k++) c->dev_free(fr.buf[k]);
+    c->dev_free(c->stage);
+    c->dev_free(c->counters);
+    c->dev_free(c->sprite);
+    c->dev_free(c->rays);
+    c->mlp_free();
+    c->builder_free();
+    if (c->ev0) (void) hipEventDestroy(c->ev0);
+    if (c->ev1) (void) hipEventDestroy(c->ev1);
+    if (c->ev2) (void) hipEventDestroy(c->ev2);
+    if (c->stream) (void) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" size_t qbt_device_bytes(const qbt_ctx* c) { return c ? c->held : 0; }
+
+extern "C" const char* qbt_strerror(int code)
+{
+    switch (code) {
+    case QBT_OK: return "ok";
+    case QBT_EINVAL: return "invalid argument";
+    case QBT_ENOMEM: return "out of memory (or device allocation cap reached)";
+    case QBT_ERANGE: return "value out of range";
+    case QBT_EIO: return "file missing, short or unreadable";
+    case QBT_EDEVICE: return "no usable GPU or HIP runtime error";
+    case QBT_ETREE: return "octree references a node out of range";
+    case QBT_EABORT: return "a ray exceeded the traversal iteration cap";
+    case QBT_ENOTSUP: return "not supported";
+    default: return "unknown error";
+    }
+}
+
+extern "C" const char* qbt_version(void) { return "qubatron-mi355x 0.1 (gfx950)"; }
+
+static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned) ((n + bs - 1) / bs); }
+
+/* logical/capacity bookkeeping of one upload target */
+extern "C" int qbt_upload(qbt_ctx* c, const void* data, int type, size_t size, size_t itemsize, size_t start,
+                          size_t end, int buftype)
+{
+    if (!c || buftype < 0 || buftype > QBT_BUFFER_DYNAMIC_POINTS) return QBT_EINVAL;
+    const bool octree = buftype == QBT_BUFFER_STATIC_OCTREE || buftype == QBT_BUFFER_DYNAMIC_OCTREE;
+    if (octree ? (type != QBT_TYPE_INT || itemsize != 16) : (type != QBT_TYPE_FLOAT || itemsize != 12))
+        return QBT_EINVAL;
+    if (start > end || end > size || start % itemsize || end % itemsize || size % itemsize) return QBT_EINVAL;
+    if (octree && size % 48) return QBT_EINVAL; /* whole 48-byte nodes */
+    if (!data && end > start) return QBT_EINVAL;
+    if (size / itemsize > (size_t) INT32_MAX * 3) return QBT_ENOTSUP;
+    HIPCHK(hipSetDevice(c->device));
+
+    const int t     = (buftype == QBT_BUFFER_STATIC_COLOR || buftype == QBT_BUFFER_STATIC_NORMAL ||
+                   buftype == QBT_BUFFER_STATIC_OCTREE || buftype == QBT_BUFFER_STATIC_POINTS)
+                          ? 0
+                          : 1;
+    int       rc    = QBT_OK;
+    size_t    items = size / itemsize;
+    if (octree) {
+        TreeBuf& tb    = c->tree[t];
+        size_t   nodes = items / 3;
+        if (nodes > INT32_MAX) return QBT_ENOTSUP;
+        if (nodes > tb.cap_nodes) { /* regrowth: reallocate and copy everything (octree_glc.c:412-428) */
+            size_t cap = nodes + nodes / 4 + 1024;
+            c->dev_free(tb.child);
+            c->dev_free(tb.model);
+            if ((rc = c->dev_alloc(tb.child, cap * 32)) || (rc = c->dev_alloc(tb.model, cap * 4))) {
+                tb.cap_nodes = 0;
+                tb.n         = 0;
+                return rc;
+            }
+            tb.cap_nodes = cap;
+            start        = 0;
+            end          = size;
+            tb.full_check = true;
+        }
+        if (nodes < tb.n) tb.full_check = true;
+        tb.n = nodes;
+        /* copy texels [start/16, end/16) through the staging buffer */
+        size_t t0 = start / 16, t1 = end / 16;
+        if (t1 > t0) {
+            size_t lo = t0 / 3, hi = (t1 + 2) / 3;
+            if (tb.dirty_hi <= tb.dirty_lo) tb.dirty_lo = lo, tb.dirty_hi = hi;
+            else tb.dirty_lo = lo < tb.dirty_lo ? lo : tb.dirty_lo, tb.dirty_hi = hi > tb.dirty_hi ? hi : tb.dirty_hi;
+        }
+        tb.validated = false;
+        const size_t per = kStageBytes / 16;
+        for (size_t a = t0; a < t1; a += per) {
+            size_t cnt = t1 - a < per ? t1 - a : per;
+            HIPCHK(hipMemcpyAsync(c->stage.p, (const char*) data + a * 16, cnt * 16, hipMemcpyHostToDevice, c->stream));
+            k_scatter_octree<<<blocks_for(cnt, 256), 256, 0, c->stream>>>((const int4*) c->stage.p, a, cnt,
+                                                                          (int4*) tb.child.p, (int*) tb.model.p);
+            HIPCHK(hipGetLastError());
+        }
+    } else if (buftype == QBT_BUFFER_STATIC_POINTS || buftype == QBT_BUFFER_DYNAMIC_POINTS) {
+        PointsBuf& pb = c->pts[t];
+        if (items > pb.cap) {
+            size_t cap = items + items / 4 + 1024;
+            c->dev_free(pb.xyz);
+            if ((rc = c->dev_alloc(pb.xyz, cap * 12))) {
+                pb.cap = pb.n = 0;
+                return rc;
+            }
+            pb.cap = cap;
+            start  = 0;
+            end    = size;
+        }
+        pb.n             = items;
+        size_t       i0  = start / 12, i1 = end / 12;
+        const size_t per = kStageBytes / 12;
+        for (size_t a = i0; a < i1; a += per) {
+            size_t cnt = i1 - a < per ? i1 - a : per;
+            HIPCHK(hipMemcpyAsync(c->stage.p, (const char*) data + a * 12, cnt * 12, hipMemcpyHostToDevice, c->stream));
+            k_scatter_points<<<blocks_for(3 * cnt, 256), 256, 0, c->stream>>>((const float*) c->stage.p, a, cnt,
+                                                                             (float*) pb.xyz.p);
+            HIPCHK(hipGetLastError());
+        }
+    } else {
+        AttrBuf& ab    = c->attr[t];
+        int      which = (buftype == QBT_BUFFER_STATIC_NORMAL || buftype == QBT_BUFFER_DYNAMIC_NORMAL) ? 1 : 0;
+        if (items > INT32_MAX) return QBT_ENOTSUP;
+        if (items > ab.cap) {
+            size_t cap = items + items / 4 + 1024;
+            /* the other attribute of the pair lives in the same array: keep it */
+            DevBuf nb;
+            if ((rc = c->dev_alloc(nb, cap * 32))) return rc;
+            k_fill_u32<<<blocks_for(cap * 8, 256), 256, 0, c->stream>>>((uint32_t*) nb.p, cap * 8, 0u);
+            HIPCHK(hipGetLastError());
+            if (ab.ca.p && ab.n) HIPCHK(hipMemcpyAsync(nb.p, ab.ca.p, ab.n * 32, hipMemcpyDeviceToDevice, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            c->dev_free(ab.ca);
+            ab.ca  = nb;
+            ab.cap = cap;
+            start  = 0;
+            end    = size;
+        }
+        size_t& cnt_which = which ? ab.n_nrm : ab.n_col;
+        cnt_which         = items;
+        ab.n              = ab.n_col > ab.n_nrm ? ab.n_col : ab.n_nrm;
+        size_t       i0   = start / 12, i1 = end / 12;
+        const size_t per  = kStageBytes / 12;
+        for (size_t a = i0; a < i1; a += per) {
+            size_t cnt = i1 - a < per ? i1 - a : per;
+            HIPCHK(hipMemcpyAsync(c->stage.p, (const char*) data + a * 12, cnt * 12, hipMemcpyHostToDevice, c->stream));
+            k_scatter_attr<<<blocks_for(cnt, 256), 256, 0, c->stream>>>((const float*) c->stage.p, a, cnt,
+                                                                        (float4*) ab.ca.p, which);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return QBT_OK;
+}
+
+extern "C" int qbt_download(qbt_ctx* c, int buftype, void* host, size_t start, size_t end)
+{
+    if (!c || !host || start > end || buftype < 0 || buftype > QBT_BUFFER_DYNAMIC_POINTS) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    const bool octree = buftype == QBT_BUFFER_STATIC_OCTREE || buftype == QBT_BUFFER_DYNAMIC_OCTREE;
+    const int  t      = (buftype == QBT_BUFFER_STATIC_COLOR || buftype == QBT_BUFFER_STATIC_NORMAL ||
+                   buftype == QBT_BUFFER_STATIC_OCTREE || buftype == QBT_BUFFER_STATIC_POINTS)
+                            ? 0
+                            : 1;
+    if (octree) {
+        const TreeBuf& tb = c->tree[t];
+        if (start % 16 || end % 16 || end > tb.n * 48) return QBT_EINVAL;
+        size_t       t0 = start / 16, t1 = end / 16;
+        const size_t per = kStageBytes / 16;
+        for (size_t a = t0; a < t1; a += per) {
+            size_t cnt = t1 - a < per ? t1 - a : per;
+            k_gather_octree<<<blocks_for(cnt, 256), 256, 0, c->stream>>>((int4*) c->stage.p, a, cnt,
+                                                                         (const int4*) tb.child.p, (const int*) tb.model.p);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync((char*) host + (a - t0) * 16, c->stage.p, cnt * 16, hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+        }
+        return QBT_OK;
+    }
+    if (start % 12 || end % 12) return QBT_EINVAL;
+    size_t i0 = start / 12, i1 = end / 12;
+    if (buftype == QBT_BUFFER_STATIC_POINTS || buftype == QBT_BUFFER_DYNAMIC_POINTS) {
+        if (i1 > c->pts[t].n) return QBT_EINVAL;
+        if (i1 > i0)
+            HIPCHK(hipMemcpy(host, (const char*) c->pts[t].xyz.p + i0 * 12, (i1 - i0) * 12, hipMemcpyDeviceToHost));
+        return QBT_OK;
+    }
+    const AttrBuf& ab    = c->attr[t];
+    int            which = (buftype == QBT_BUFFER_STATIC_NORMAL || buftype == QBT_BUFFER_DYNAMIC_NORMAL) ? 1 : 0;
+    if (i1 > (which ? ab.n_nrm : ab.n_col)) return QBT_EINVAL;
+    const size_t per = kStageBytes / 12;
+    for (size_t a = i0; a < i1; a += per) {
+        size_t cnt = i1 - a < per ? i1 - a : per;
+        k_gather_attr<<<blocks_for(cnt, 256), 256, 0, c->stream>>>((float*) c->stage.p, a, cnt,
+                                                                   (const float4*) ab.ca.p, which);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync((char*) host + (a - i0) * 12, c->stage.p, cnt * 12, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return QBT_OK;
+}
+
+/* Check that the child indices of dirty node ranges stay inside the tree. */
+int qbt_ctx::validate_tree(int t)
+{
+    TreeBuf& tb = tree[t];
+    if (tb.validated) return QBT_OK;
+    if (tb.n == 0) {
+        tb.validated = true;
+        return QBT_OK;
+    }
+    size_t lo = tb.full_check ? 0 : tb.dirty_lo, hi = tb.full_check ? tb.n : (tb.dirty_hi < tb.n ? tb.dirty_hi : tb.n);
+    DevBuf& flag = valflag;
+    if (!flag.p) {
+        int rc = dev_alloc(flag, 256);
+        if (rc) return rc;
+    }
+    k_fill_u32<<<1, 64, 0, stream>>>((uint32_t*) flag.p, 1, 0u);
+    if (hi > lo)
+        k_validate<<<blocks_for(hi - lo, 256), 256, 0, stream>>>((const int4*) tb.child.p, lo, hi, (int) tb.n,
+                                                                 (unsigned int*) flag.p);
+    HIPCHK(hipGetLastError());
+    unsigned int nbad = 0;
+    HIPCHK(hipMemcpyAsync(&nbad, flag.p, 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    if (nbad) return QBT_ETREE;
+    tb.validated  = true;
+    tb.full_check = false;
+    tb.dirty_lo = tb.dirty_hi = 0;
+    return QBT_OK;
+}
+
+/* every subdivision coordinate is an exact fp32 multiple of the leaf edge */
+bool qbt_grid_exact(const float b[4], int L)
+{
+    float u = ldexpf(b[3], -L);
+    if (!(u >= FLT_MIN) || !(b[3] > 0.0f) || !isfinite(b[3])) return false;
+    const double lim = 16777216.0;
+    const double sgn[3] = {1.0, -1.0, -1.0};
+    for (int k = 0; k < 3; k++) {
+        double q  = (double) b[k] / (double) u;
+        double qi = nearbyint(q);
+        if (qi != q || qi * (double) u != (double) b[k]) return false;
+        double far = qi + sgn[k] * ldexp(1.0, L);
+        if (fabs(qi) > lim || fabs(far) > lim) return false;
+    }
+    return true;
+}
+
+int qbt_ctx::scene_for(const float basecube[4], int maxlevel, SceneDev& s, bool& dual, bool& grid)
+{
+    if (maxlevel < 0 || maxlevel > kMaxLevel) return QBT_EINVAL;
+    if (!(basecube[3] > 0.0f) || !isfinite(basecube[0]) || !isfinite(basecube[1]) || !isfinite(basecube[2]) ||
+        !isfinite(basecube[3]))
+        return QBT_EINVAL;
+    if (!(ldexpf(basecube[3], -maxlevel) >= FLT_MIN)) return QBT_EINVAL;
+    if (tree[0].n == 0) return QBT_EINVAL; /* no static octree uploaded */
+    int rc;
+    if ((rc = validate_tree(0))) return rc;
+    if ((rc = validate_tree(1))) return rc;
+    memset(&s, 0, sizeof(s));
+    s.stat  = {(const int4*) tree[0].child.p, (const int*) tree[0].model.p, (int) tree[0].n};
+    s.dyn   = {(const int4*) tree[1].child.p, (const int*) tree[1].model.p, (int) tree[1].n};
+    s.astat = {(const float4*) attr[0].ca.p, (int) attr[0].n};
+    s.adyn  = {(const float4*) attr[1].ca.p, (int) attr[1].n};
+    s.bx = basecube[0], s.by = basecube[1], s.bz = basecube[2], s.bw = basecube[3];
+    s.maxlevel = maxlevel;
+    s.iter_cap = iter_cap;
+    dual       = tree[1].n > 1 || (maxlevel == 0 && tree[1].n >= 1);
+    grid       = qbt_grid_exact(basecube, maxlevel);
+    return QBT_OK;
+}
+
+size_t qbt_stack_bytes(int L, bool dual, int threads) { return (size_t) threads * (size_t) (L > 0 ? L : 1) * (dual ? 12 : 8); }
+
+/* view uniforms: the camera quaternions are computed once per view with the
+ * host's sinf/cosf (octree_fsh.c:406-408) */
+void qbt_make_view(const qbt_view& q, const float* lights, int nlights, ViewDev& v)
+{
+    memset(&v, 0, sizeof(v));
+    v.cam[0] = q.camfp[0], v.cam[1] = q.camfp[1], v.cam[2] = q.camfp[2];
+    const float ay[3] = {0.0f, 1.0f, 0.0f}, mx[3] = {-1.0f, 0.0f, 0.0f};
+    auto axis = [](const float a[3], float angle, float out[4]) {
+        float h = angle * 0.5f;
+        out[0]  = a[0] * sinf(h);
+        out[1]  = a[1] * sinf(h);
+        out[2]  = a[2] * sinf(h);
+        out[3]  = cosf(h);
+    };
+    auto rot = [](const float qq[4], const float vv[3], float r[3]) {
+        float c1[3] = {qq[1] * vv[2] - qq[2] * vv[1], qq[2] * vv[0] - qq[0] * vv[2], qq[0] * vv[1] - qq[1] * vv[0]};
+        float s[3]  = {c1[0] + qq[3] * vv[0], c1[1] + qq[3] * vv[1], c1[2] + qq[3] * vv[2]};
+        float c2[3] = {qq[1] * s[2] - qq[2] * s[1], qq[2] * s[0] - qq[0] * s[2], qq[0] * s[1] - qq[1] * s[0]};
+        r[0]        = vv[0] + 2.0f * c2[0];
+        r[1]        = vv[1] + 2.0f * c2[1];
+        r[2]        = vv[2] + 2.0f * c2[2];
+    };
+    float vx[3];
+    axis(ay, -q.angle[0], v.qz);
+    rot(v.qz, mx, vx);
+    axis(vx, -q.angle[1], v.qx);
+    v.W       = q.width;
+    v.H       = q.height;
+    v.w       = (int) q.width;
+    v.h       = (int) q.height;
+    v.shoot   = q.shoot;
+    v.nlights = nlights;
+    v.cos_thr = cosf(0.02f);
+    for (int l = 0; l < nlights; l++) {
+        float lv[3] = {lights[3 * l] - q.camfp[0], lights[3 * l + 1] - q.camfp[1], lights[3 * l + 2] - q.camfp[2]};
+        float d     = lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2];
+        float sc    = 1.0f / sqrtf(d);
+        for (int k = 0; k < 3; k++) {
+            v.light[l][k] = lights[3 * l + k];
+            v.cl[l][k]    = lv[k] * sc;
+        }
+    }
+}
+
+int qbt_ctx::ensure_frame(int view, int w, int h, unsigned mask)
+{
+    if (view >= (int) frames.size()) frames.resize(view + 1);
+    Frame&       fr  = frames[view];
+    const size_t n   = (size_t) w * (size_t) h;
+    const size_t esz[FR_N] = {4, 4, 1, 4, 16, 16, 1, 4, 4};
+    for (int k = 0; k < FR_N; k++) {
+        if (!(mask & (1u << k))) continue;
+        int rc = dev_alloc(fr.buf[k], n * esz[k] > 0 ? n * esz[k] : 16);
+        if (rc) return rc;
+    }
+    fr.w    = w;
+    fr.h    = h;
+    fr.mask = mask & QBT_OUT_ALL;
+    return QBT_OK;
+}
+
+FrameDev qbt_ctx::frame_dev(int view)
+{
+    Frame&   fr = frames[view];
+    FrameDev f;
+    f.hit    = (int*) fr.buf[FR_HIT].p;
+    f.leaf   = (int*) fr.buf[FR_LEAF].p;
+    f.shadow = (uint8_t*) fr.buf[FR_SHADOW].p;
+    f.rgba8  = (uint32_t*) fr.buf[FR_RGBA8].p;
+    f.rgbaf  = (float4*) fr.buf[FR_RGBAF].p;
+    f.isp    = (float4*) fr.buf[FR_ISP].p;
+    f.flags  = (uint8_t*) fr.buf[FR_FLAGS].p;
+    f.dyn    = (int*) fr.buf[FR_DYN].p;
+    f.exp    = (int*) fr.buf[FR_EXP].p;
+    return f;
+}
+
+template <bool DUAL, bool GRID, bool COUNT>
+static void launch_render_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const int* spr, FrameDev f, unsigned mask)
+{
+    const int    bs    = c->block_threads(s.maxlevel, DUAL);
+    const int    th    = bs / 16;
+    const int    tiles = ((v.w + 15) / 16) * ((v.h + th - 1) / th);
+    const size_t lds   = qbt_stack_bytes(s.maxlevel, DUAL, bs);
+    k_render<DUAL, GRID, COUNT><<<tiles, bs, lds, c->stream>>>(s, v, spr, f, mask, (unsigned long long*) c->counters.p);
+}
+
+int qbt_ctx::block_threads(int L, bool dual) const
+{
+    int bs = tune_block > 0 ? tune_block : 256;
+    while (bs > 64 && qbt_stack_bytes(L, dual, bs) > 64 * 1024) bs /= 2;
+    return bs;
+}
+
+int qbt_ctx::launch_render(const SceneDev& s, bool dual, bool grid, bool count, const ViewDev& v, const int* spr,
+                           FrameDev f, unsigned mask)
+{
+    if (v.w <= 0 || v.h <= 0) return QBT_OK;
+    if (dual) {
+        if (grid) count ? launch_render_t<true, true, true>(this, s, v, spr, f, mask) : launch_render_t<true, true, false>(this, s, v, spr, f, mask);
+        else count ? launch_render_t<true, false, true>(this, s, v, spr, f, mask) : launch_render_t<true, false, false>(this, s, v, spr, f, mask);
+    } else {
+        if (grid) count ? launch_render_t<false, true, true>(this, s, v, spr, f, mask) : launch_render_t<false, true, false>(this, s, v, spr, f, mask);
+        else count ? launch_render_t<false, false, true>(this, s, v, spr, f, mask) : launch_render_t<false, false, false>(this, s, v, spr, f, mask);
+    }
+    HIPCHK(hipGetLastError());
+    return QBT_OK;
+}
+
+int qbt_ctx::launch_sprite(const SceneDev& s, bool dual, bool grid, const ViewDev& v, int* spr)
+{
+    if (v.nlights <= 0) return QBT_OK;
+    const size_t lds = qbt_stack_bytes(s.maxlevel, dual, 64);
+    if (dual) {
+        if (grid) k_sprite<true, true><<<1, 64, lds, stream>>>(s, v, spr);
+        else k_sprite<true, false><<<1, 64, lds, stream>>>(s, v, spr);
+    } else {
+        if (grid) k_sprite<false, true><<<1, 64, lds, stream>>>(s, v, spr);
+        else k_sprite<false, false><<<1, 64, lds, stream>>>(s, v, spr);
+    }
+    HIPCHK(hipGetLastError());
+    return QBT_OK;
+}
+
+extern "C" int qbt_render(qbt_ctx* c, const qbt_view* views, int nviews, const float* lights, int nlights,
+                          unsigned out_mask)
+{
+    if (!c || !views || nviews <= 0 || nviews > QBT_MAX_VIEWS) return QBT_EINVAL;
+    if (nlights < 0 || nlights > QBT_MAX_LIGHTS_RENDER || (nlights > 0 && !lights)) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    const bool count = (out_mask & QBT_RENDER_COUNT) != 0;
+    unsigned   mask  = out_mask & QBT_OUT_ALL;
+    if (!count) mask &= ~QBT_OUT_EXP;
+    for (int i = 0; i < nviews; i++) {
+        const qbt_view& q = views[i];
+        if (!(q.width >= 1.0f) || !(q.height >= 1.0f) || q.width > 65536.0f || q.height > 65536.0f) return QBT_EINVAL;
+        if ((double) (int) q.width * (double) (int) q.height > 2147483647.0) return QBT_EINVAL;
+    }
+    if (out_mask & QBT_RENDER_SURROGATE) return c->render_surrogate(views, nviews, lights, nlights, out_mask);
+
+    c->last = qbt_stats();
+    k_zero_u64<<<1, 64, 0, c->stream>>>((unsigned long long*) c->counters.p, CNT_N);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    uint64_t prim = 0;
+    for (int i = 0; i < nviews; i++) {
+        const qbt_view& q = views[i];
+        SceneDev        s;
+        bool            dual, grid;
+        int             rc = c->scene_for(q.basecube, q.maxlevel, s, dual, grid);
+        if (rc) return rc;
+        ViewDev v;
+        qbt_make_view(q, lights, nlights, v);
+        if ((rc = c->ensure_frame(i, v.w, v.h, mask))) return rc;
+        int* spr = (int*) c->sprite.p + i * QBT_MAX_LIGHTS_RENDER;
+        if ((rc = c->launch_sprite(s, dual, grid, v, spr))) return rc;
+        if ((rc = c->launch_render(s, dual, grid, count, v, spr, c->frame_dev(i), mask))) return rc;
+        prim += (uint64_t) v.w * (uint64_t) v.h;
+        c->last.grid_exact = grid;
+        c->last.dual       = dual;
+        c->last.launches += 2;
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    c->last.primary_rays = prim;
+    c->pending_stats     = true;
+    c->stats_has_trace   = false;
+    return QBT_OK;
+}
+
+extern "C" int qbt_update(qbt_ctx* c, float width, float height, const float position[3], const float angle[3],
+                          float lighta, uint8_t quality, int maxlevel, float basesize, int shoot)
+{
+    if (!c || !position || !angle) return QBT_EINVAL;
+    /* octree_glc.c:263-270: the arithmetic of the reference, double where it is double */
+    float    light[3] = {420.0f, (float) (200.0f - (double) sinf(lighta) * 20.0),
+                      (float) (680.0f - (double) sinf(lighta) * 200.0)};
+    qbt_view v;
+    memset(&v, 0, sizeof(v));
+    v.camfp[0]    = position[0];
+    v.camfp[1]    = position[1];
+    v.camfp[2]    = position[2];
+    v.angle[0]    = angle[0];
+    v.angle[1]    = angle[1];
+    v.width       = (float) ((double) width / (6.0 - (double) ((float) quality / 2.0)));
+    v.height      = (float) ((double) height / (6.0 - (double) ((float) quality / 2.0)));
+    v.shoot       = shoot;
+    v.basecube[0] = 0.0f;
+    v.basecube[1] = basesize;
+    v.basecube[2] = basesize;
+    v.basecube[3] = basesize;
+    v.maxlevel    = maxlevel;
+    return qbt_render(c, &v, 1, light, 1, QBT_OUT_DEFAULT);
+}
+
+extern "C" int qbt_get_stats(qbt_ctx* c, qbt_stats* out)
+{
+    if (!c || !out) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    if (c->pending_stats) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        c->last.kernel_ms = ms;
+        c->last.trace_ms  = ms;
+        if (c->stats_has_trace) {
+            float tms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&tms, c->ev0, c->ev2));
+            c->last.trace_ms = tms;
+        }
+        unsigned long long cnt[CNT_N];
+        HIPCHK(hipMemcpy(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+        c->last.shadow_rays = cnt[CNT_SHADOW];
+        c->last.hits        = cnt[CNT_HITS];
+        c->last.expansions  = cnt[CNT_EXP];
+        c->last.predicted   = cnt[CNT_PRED];
+        c->aborted          = cnt[CNT_ABORT];
+        c->pending_stats    = false;
+    }
+    *out = c->last;
+    return c->aborted ? QBT_EABORT : QBT_OK;
+}
+
+static int frame_fields(qbt_ctx* c, int view, qbt_frame* out, bool device)
+{
+    if (!c || !out || view < 0 || view >= (int) c->frames.size()) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    Frame& fr = c->frames[view];
+    if (device) {
+        FrameDev f = c->frame_dev(view);
+        out->w     = fr.w;
+        out->h     = fr.h;
+        out->hit   = (fr.mask & QBT_OUT_HIT) ? f.hit : nullptr;
+        out->leaf  = (fr.mask & QBT_OUT_LEAF) ? f.leaf : nullptr;
+        out->shadow = (fr.mask & QBT_OUT_SHADOW) ? f.shadow : nullptr;
+        out->rgba8  = (fr.mask & QBT_OUT_RGBA8) ? f.rgba8 : nullptr;
+        out->rgbaf  = (fr.mask & QBT_OUT_RGBAF) ? (float*) f.rgbaf : nullptr;
+        out->isp    = (fr.mask & QBT_OUT_ISP) ? (float*) f.isp : nullptr;
+        out->flags  = (fr.mask & QBT_OUT_FLAGS) ? f.flags : nullptr;
+        out->dyn    = (fr.mask & QBT_OUT_DYN) ? f.dyn : nullptr;
+        out->exp    = (fr.mask & QBT_OUT_EXP) ? f.exp : nullptr;
+        return QBT_OK;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t n = (size_t) fr.w * fr.h;
+    struct {
+        void*    dst;
+        unsigned bit;
+        int      k;
+        size_t   esz;
+    } m[] = {{out->hit, QBT_OUT_HIT, FR_HIT, 4},       {out->leaf, QBT_OUT_LEAF, FR_LEAF, 4},
+             {out->shadow, QBT_OUT_SHADOW, FR_SHADOW, 1}, {out->rgba8, QBT_OUT_RGBA8, FR_RGBA8, 4},
+             {out->rgbaf, QBT_OUT_RGBAF, FR_RGBAF, 16},   {out->isp, QBT_OUT_ISP, FR_ISP, 16},
+             {out->flags, QBT_OUT_FLAGS, FR_FLAGS, 1},    {out->dyn, QBT_OUT_DYN, FR_DYN, 4},
+             {out->exp, QBT_OUT_EXP, FR_EXP, 4}};
+    out->w = fr.w;
+    out->h = fr.h;
+    for (auto& e : m) {
+        if (!e.dst) continue;
+        if (!(fr.mask & e.bit)) return QBT_EINVAL; /* not rendered */
+        HIPCHK(hipMemcpy(e.dst, fr.buf[e.k].p, n * e.esz, hipMemcpyDeviceToHost));
+    }
+    return QBT_OK;
+}
+
+extern "C" int qbt_read_frame(qbt_ctx* c, int view, qbt_frame* out) { return frame_fields(c, view, out, false); }
+extern "C" int qbt_frame_device(qbt_ctx* c, int view, qbt_frame* out) { return frame_fields(c, view, out, true); }
+
+extern "C" int qbt_trace_rays(qbt_ctx* c, const float* origins, const float* dirs, size_t n, int semantics,
+                              const float basecube[4], int maxlevel, int32_t* status, int32_t* model, int32_t* leaf,
+                              float* isp, float* tlf)
+{
+    if (!c || !basecube || (n && (!origins || !dirs)) || (semantics != 0 && semantics != 1)) return QBT_EINVAL;
+    if (n == 0) return QBT_OK;
+    if (n > (size_t) 1 << 31) return QBT_ENOTSUP;
+    HIPCHK(hipSetDevice(c->device));
+    SceneDev s;
+    bool     dual, grid;
+    int      rc = c->scene_for(basecube, maxlevel, s, dual, grid);
+    if (rc) return rc;
+    if (semantics == 1) dual = false; /* octree_trace_line walks one tree */
+    /* ray scratch: org, dir (n x 12 B each), status/model/leaf (n x 4 B each),
+     * then isp and tlf (n x 16 B each) at a 16-byte aligned offset */
+    const size_t off_isp = ((n * 36 + 15) / 16) * 16;
+    if ((rc = c->dev_alloc(c->rays, off_isp + n * 32))) return rc;
+    char*   base = (char*) c->rays.p;
+    float*  dorg = (float*) base;
+    float*  ddir = (float*) (base + n * 12);
+    int*    dst  = (int*) (base + n * 24);
+    int*    dmod = (int*) (base + n * 28);
+    int*    dlf  = (int*) (base + n * 32);
+    float4* disp = (float4*) (base + off_isp);
+    float4* dtlf = (float4*) (base + off_isp + n * 16);
+    HIPCHK(hipMemcpyAsync(dorg, origins, n * 12, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(ddir, dirs, n * 12, hipMemcpyHostToDevice, c->stream));
+    k_zero_u64<<<1, 64, 0, c->stream>>>((unsigned long long*) c->counters.p, CNT_N);
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    const int    bs  = c->block_threads(maxlevel, dual);
+    const size_t lds = qbt_stack_bytes(maxlevel, dual, bs);
+    unsigned     nb  = blocks_for(n, bs);
+    unsigned long long* cnt = (unsigned long long*) c->counters.p;
+    if (semantics == 1) {
+        if (grid) k_trace_rays<false, true, true><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
+        else k_trace_rays<false, true, false><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
+    } else if (dual) {
+        if (grid) k_trace_rays<true, false, true><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
+        else k_trace_rays<true, false, false><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
+    } else {
+        if (grid) k_trace_rays<false, false, true><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
+        else k_trace_rays<false, false, false><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    if (status) HIPCHK(hipMemcpyAsync(status, dst, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (model) HIPCHK(hipMemcpyAsync(model, dmod, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (leaf) HIPCHK(hipMemcpyAsync(leaf, dlf, n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (isp) HIPCHK(hipMemcpyAsync(isp, disp, n * 16, hipMemcpyDeviceToHost, c->stream));
+    if (tlf) HIPCHK(hipMemcpyAsync(tlf, dtlf, n * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->last            = qbt_stats();
+    c->last.primary_rays = n;
+    c->last.grid_exact   = grid;
+    c->last.dual         = dual;
+    c->last.launches     = 1;
+    c->pending_stats     = true;
+    c->stats_has_trace   = false;
+    qbt_stats st;
+    return qbt_get_stats(c, &st);
+}

@@ -1,0 +1,125 @@
+/*
+ * qbt_internal.hpp - context and device-side structures shared by the
+ * translation units of libqbt.so (not part of the C ABI).
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "qbt.h"
+#include "trace.hpp"
+
+#define QBT_OUT_ALL 0x1FFu
+
+enum { CNT_SHADOW = 0, CNT_HITS, CNT_EXP, CNT_ABORT, CNT_PRED, CNT_QUEUE, CNT_Q2, CNT_Q3, CNT_N };
+enum { FR_HIT = 0, FR_LEAF, FR_SHADOW, FR_RGBA8, FR_RGBAF, FR_ISP, FR_FLAGS, FR_DYN, FR_EXP, FR_N };
+
+struct DevBuf {
+    void*  p   = nullptr;
+    size_t cap = 0; /* bytes */
+};
+
+struct TreeBuf {
+    DevBuf child, model;
+    size_t n         = 0; /* logical node count (size/48 of the last upload) */
+    size_t cap_nodes = 0;
+    bool   validated = true;
+    bool   full_check = false;
+    size_t dirty_lo = 0, dirty_hi = 0;
+};
+
+struct AttrBuf {
+    DevBuf ca;
+    size_t n = 0, n_col = 0, n_nrm = 0, cap = 0;
+};
+
+struct PointsBuf {
+    DevBuf xyz;
+    size_t n = 0, cap = 0;
+};
+
+/* per-view uniforms of the pixel program */
+struct ViewDev {
+    float cam[3];
+    float qz[4], qx[4];
+    float W, H;
+    int   w, h;
+    int   shoot;
+    int   nlights;
+    float light[QBT_MAX_LIGHTS_RENDER][3];
+    float cl[QBT_MAX_LIGHTS_RENDER][3]; /* normalize(light - camfp) */
+    float cos_thr;
+};
+
+struct FrameDev {
+    int*      hit;
+    int*      leaf;
+    uint8_t*  shadow;
+    uint32_t* rgba8;
+    float4*   rgbaf;
+    float4*   isp;
+    uint8_t*  flags;
+    int*      dyn;
+    int*      exp;
+};
+
+struct Frame {
+    DevBuf   buf[FR_N];
+    int      w = 0, h = 0;
+    unsigned mask = 0;
+};
+
+struct MlpState; /* surrogate.hip */
+struct BuildState; /* build.hip */
+
+struct qbt_ctx {
+    int          device    = 0;
+    size_t       max_bytes = 0;
+    size_t       held      = 0;
+    hipStream_t  stream    = nullptr;
+    hipEvent_t   ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+    DevBuf       stage, counters, sprite, rays, valflag;
+    TreeBuf      tree[2];
+    AttrBuf      attr[2];
+    PointsBuf    pts[2];
+    std::vector<Frame> frames;
+    qbt_stats    last{};
+    bool         pending_stats   = false;
+    bool         stats_has_trace = false;
+    uint64_t     aborted         = 0;
+    int          iter_cap        = 1 << 22;
+    int          tune_block      = 0;
+    MlpState*    mlp             = nullptr;
+    BuildState*  build           = nullptr;
+
+    int      dev_alloc(DevBuf& b, size_t bytes);
+    void     dev_free(DevBuf& b);
+    int      validate_tree(int t);
+    int      scene_for(const float basecube[4], int maxlevel, qbt::SceneDev& s, bool& dual, bool& grid);
+    int      ensure_frame(int view, int w, int h, unsigned mask);
+    FrameDev frame_dev(int view);
+    int      block_threads(int L, bool dual) const;
+    int      launch_render(const qbt::SceneDev& s, bool dual, bool grid, bool count, const ViewDev& v, const int* spr,
+                           FrameDev f, unsigned mask);
+    int      launch_sprite(const qbt::SceneDev& s, bool dual, bool grid, const ViewDev& v, int* spr);
+    int      render_surrogate(const qbt_view* views, int nviews, const float* lights, int nlights, unsigned out_mask);
+    void     mlp_free();
+    void     builder_free();
+};
+
+void   qbt_set_hip_error(hipError_t e, const char* what, const char* file, int line);
+bool   qbt_grid_exact(const float b[4], int L);
+size_t qbt_stack_bytes(int L, bool dual, int threads);
+void   qbt_make_view(const qbt_view& q, const float* lights, int nlights, ViewDev& v);
+
+#define QBT_HIPCHK(x)                                      \
+    do {                                                   \
+        hipError_t e_ = (x);                               \
+        if (e_ != hipSuccess) {                            \
+            qbt_set_hip_error(e_, #x, __FILE__, __LINE__); \
+            return QBT_EDEVICE;                            \
+        }                                                  \
+    } while (0)

@@ -1,0 +1,404 @@
+/*
+ * trace.hpp - per-ray sparse-octree traversal for gfx950.
+ *
+ * Semantics: cube_trace_line of src/qubatron/shaders/octree_fsh.c:138-379
+ * (static + dynamic tree in lockstep) and, with CPU = true, octree_trace_line
+ * of src/qubatron/octree.c:341-537. Results (leaf node, entry point bits,
+ * miss/discard) are identical to the reference algorithm; only the storage
+ * differs:
+ *
+ *  - The reference keeps per level the cube, four pending entry points
+ *    (4 x float4) and octets, the `ispsi` word and two node indices
+ *    (~108 B/level, octree_fsh.c:50-58). Here a level holds one 32-bit
+ *    pending word + the node index (8 B/level, 12 with the dynamic tree) in
+ *    LDS: the cube is a function of the integer cell coordinates (ix, iy, iz)
+ *    of the current node, and every pending point is re-derived from the
+ *    plane that produced it (same inputs, same fp32 operations, so the same
+ *    bits): a mid-plane hit of that level's node, or the node's entry point,
+ *    which is the point that was popped one level up (followed upwards to
+ *    the root entry).
+ *  - The node array is split into a 32-byte child block per node (two int4,
+ *    one aligned 32 B gather per expansion) and an int32 model index array.
+ *
+ * Pending word: bits 0-2 count n, bits 3-5 next index k, entry j at bit
+ * 8+5j: octet (3 bits) | source << 3 (0 = node entry point, 1 = z, 2 = x,
+ * 3 = y mid-plane).
+ *
+ * Compile with -ffp-contract=off; fp32 division and sqrt correctly rounded.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#pragma clang fp contract(off)
+
+namespace qbt {
+
+constexpr int kMaxLevel = 24;
+constexpr float kFltMax = 3.402823466e+38f;
+
+enum TraceStatus : int { ST_MISS = 0, ST_HIT = 1, ST_DISCARD = -1, ST_ABORT = -2 };
+
+struct TreeDev {
+    const int4* __restrict__ child; /* 2 per node */
+    const int* __restrict__ model;  /* 1 per node */
+    int n;                          /* logical node count */
+};
+
+struct AttrDev {
+    const float4* __restrict__ ca; /* [2m] = colour (r,g,b,1), [2m+1] = normal (x,y,z,1) */
+    int n;
+};
+
+struct SceneDev {
+    TreeDev stat, dyn;
+    AttrDev astat, adyn;
+    float bx, by, bz, bw; /* basecube */
+    int maxlevel;
+    int iter_cap;
+};
+
+struct Ray {
+    float px, py, pz, dx, dy, dz;
+};
+
+struct Hit {
+    float4 isp;
+    int ns, nd;     /* leaf node indices (static, dynamic) as the shader holds them */
+    int ix, iy, iz; /* leaf cell coordinates */
+    unsigned exps;  /* node expansions */
+};
+
+/* is_cube_{x,y,z}plane (octree_fsh.c:62-99 / octree.c:302-339) */
+template <bool CPU>
+__device__ __forceinline__ float4 plane_x(float c, const Ray& r)
+{
+    float4 o = CPU ? make_float4(0.0f, 0.0f, 0.0f, kFltMax) : make_float4(kFltMax, kFltMax, kFltMax, kFltMax);
+    if (r.dx != 0.0f) {
+        float t = (c - r.px) / r.dx;
+        o.w     = t;
+        o.y     = r.py + r.dy * t;
+        o.z     = r.pz + r.dz * t;
+        o.x     = c;
+    }
+    return o;
+}
+
+template <bool CPU>
+__device__ __forceinline__ float4 plane_y(float c, const Ray& r)
+{
+    float4 o = CPU ? make_float4(0.0f, 0.0f, 0.0f, kFltMax) : make_float4(kFltMax, kFltMax, kFltMax, kFltMax);
+    if (r.dy != 0.0f) {
+        float t = (c - r.py) / r.dy;
+        o.w     = t;
+        o.x     = r.px + r.dx * t;
+        o.z     = r.pz + r.dz * t;
+        o.y     = c;
+    }
+    return o;
+}
+
+template <bool CPU>
+__device__ __forceinline__ float4 plane_z(float c, const Ray& r)
+{
+    float4 o = CPU ? make_float4(0.0f, 0.0f, 0.0f, kFltMax) : make_float4(kFltMax, kFltMax, kFltMax, kFltMax);
+    if (r.dz != 0.0f) {
+        float t = (c - r.pz) / r.dz;
+        o.w     = t;
+        o.x     = r.px + r.dx * t;
+        o.y     = r.py + r.dy * t;
+        o.z     = c;
+    }
+    return o;
+}
+
+/* Node cube (tlf x, y, z) at `level` with cell coordinates (ix, iy, iz).
+ * GRID: every coordinate of the subdivision is an exact fp32 multiple of the
+ * leaf edge (checked on the host), so x = bx + ix*w is the value the
+ * reference's running sum reaches. Otherwise the running sum of the
+ * reference (tlf.x += xsft*halfs, octree_fsh.c:342-347) is replayed from the
+ * root along the path bits. */
+template <bool GRID>
+__device__ __forceinline__ void cube_at(const SceneDev& s, int level, int ix, int iy, int iz, float& x, float& y,
+                                        float& z, float& w)
+{
+    w = ldexpf(s.bw, -level);
+    if (GRID) {
+        x = s.bx + (float) ix * w;
+        y = s.by - (float) iy * w;
+        z = s.bz - (float) iz * w;
+    } else {
+        x = s.bx, y = s.by, z = s.bz;
+        for (int l = 0; l < level; l++) {
+            float h  = ldexpf(s.bw, -(l + 1));
+            int   sh = level - 1 - l;
+            x        = x + (float) ((ix >> sh) & 1) * h;
+            y        = y - (float) ((iy >> sh) & 1) * h;
+            z        = z - (float) ((iz >> sh) & 1) * h;
+        }
+    }
+}
+
+/* Mid-plane hit `src` (1 z, 2 x, 3 y) of the node cube (x, y, z, w). */
+template <bool CPU>
+__device__ __forceinline__ float4 midplane(int src, const Ray& r, float x, float y, float z, float w)
+{
+    float bxx = x + w, byy = y - w, bzz = z - w;
+    float hx = bxx + (x - bxx) * 0.5f;
+    float hy = byy + (y - byy) * 0.5f;
+    float hz = bzz + (z - bzz) * 0.5f;
+    return src == 1 ? plane_z<CPU>(hz, r) : src == 2 ? plane_x<CPU>(hx, r) : plane_y<CPU>(hy, r);
+}
+
+__device__ __forceinline__ int pick4(int4 v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+__device__ __forceinline__ int pick8(int4 lo, int4 hi, int o) { return pick4((o & 4) ? hi : lo, o & 3); }
+__device__ __forceinline__ unsigned exist_mask(int4 lo, int4 hi)
+{
+    return (lo.x > 0 ? 1u : 0u) | (lo.y > 0 ? 2u : 0u) | (lo.z > 0 ? 4u : 0u) | (lo.w > 0 ? 8u : 0u) |
+           (hi.x > 0 ? 16u : 0u) | (hi.y > 0 ? 32u : 0u) | (hi.z > 0 ? 64u : 0u) | (hi.w > 0 ? 128u : 0u);
+}
+
+/* slot k (< 8) of node i at `level` (oct_from_octets_for_index,
+ * octree_fsh.c:127-136): 0 for index 0 below the root. */
+__device__ __forceinline__ int child_slot(const TreeDev& t, int i, int k, int level)
+{
+    if (i == 0 && level > 0) return 0;
+    const int* c = reinterpret_cast<const int*>(t.child);
+    return c[8 * (size_t) i + k];
+}
+
+struct Cand {
+    float x, y, z, w;
+    int src;
+};
+
+__device__ __forceinline__ void cswap(Cand& a, Cand& b)
+{
+    if (b.w < a.w) { /* strict: ties keep order (octree_fsh.c:283) */
+        Cand t = a;
+        a      = b;
+        b      = t;
+    }
+}
+
+/* Stack columns in LDS: word j of the calling lane is at base[j * stride]. */
+struct Stack {
+    uint32_t* pw;
+    int* ns;
+    int* nd;
+    int stride;
+};
+
+/* Entry point of the node at `level` on the current path (level >= 1 uses
+ * the entry popped one level up). */
+template <bool CPU, bool GRID>
+__device__ __forceinline__ float4 entry_of(const SceneDev& s, const Ray& r, const Stack& st, int level, int ix, int iy,
+                                        int iz, float4 start)
+{
+    for (int m = level; m > 0; m--) {
+        uint32_t w   = st.pw[(m - 1) * st.stride];
+        int      k   = (int) ((w >> 3) & 7u) - 1;
+        int      src = (int) ((w >> (8 + 5 * k + 3)) & 3u);
+        if (src != 0) {
+            int   sh = level - (m - 1);
+            float x, y, z, ww;
+            cube_at<GRID>(s, m - 1, ix >> sh, iy >> sh, iz >> sh, x, y, z, ww);
+            return midplane<CPU>(src, r, x, y, z, ww);
+        }
+    }
+    return start;
+}
+
+/*
+ * Trace one ray. Returns ST_HIT / ST_MISS / ST_DISCARD / ST_ABORT.
+ * DUAL: the dynamic tree takes part. CPU: octree.c plane-test returns and
+ * root acceptance (single tree; callers pass DUAL = false).
+ */
+template <bool DUAL, bool CPU, bool GRID>
+__device__ int trace(const SceneDev& s, const Ray& r, const Stack& st, Hit& out)
+{
+    out.exps = 0;
+    /* root entry (octree_fsh.c:157-211) */
+    float4 h0 = make_float4(0.f, 0.f, 0.f, 0.f), h1 = h0;
+    int    hitc = 0;
+    {
+        const float tx = s.bx, ty = s.by, tz = s.bz, tw = s.bw;
+        const float rx = tx + tw, ry = ty - tw, rz = tz - tw;
+        float4      a;
+#define QBT_CONSIDER(cond)                       \
+    if ((!CPU || a.w < kFltMax) && (cond)) {     \
+        if (hitc == 0) h0 = a;                   \
+        else if (hitc == 1) h1 = a;              \
+        hitc++;                                  \
+    }
+        a = plane_z<CPU>(tz, r);
+        QBT_CONSIDER(tx < a.x && a.x <= rx && ty > a.y && a.y >= ry)
+        a = plane_z<CPU>(rz, r);
+        QBT_CONSIDER(tx < a.x && a.x <= rx && ty > a.y && a.y >= ry)
+        a = plane_x<CPU>(tx, r);
+        QBT_CONSIDER(ty > a.y && a.y >= ry && tz > a.z && a.z >= rz)
+        a = plane_x<CPU>(rx, r);
+        QBT_CONSIDER(ty > a.y && a.y >= ry && tz > a.z && a.z >= rz)
+        a = plane_y<CPU>(ty, r);
+        QBT_CONSIDER(tx < a.x && a.x <= rx && tz > a.z && a.z >= rz)
+        a = plane_y<CPU>(ry, r);
+        QBT_CONSIDER(tx < a.x && a.x <= rx && tz > a.z && a.z >= rz)
+#undef QBT_CONSIDER
+    }
+    if (hitc < 2) return ST_DISCARD;
+    if (h0.w < 0.0f && h1.w < 0.0f) return ST_DISCARD;
+    if (h1.w < h0.w) h0 = h1;
+    if (h0.w < 0.0f) h0 = make_float4(r.px, r.py, r.pz, 0.0f);
+    const float4 start = h0;
+
+    int    level = 0, ix = 0, iy = 0, iz = 0;
+    int    ns = 0, nd = 0;
+    float4 entry = start;
+    const int L = s.maxlevel;
+
+    for (int it = 0;; ++it) {
+        if (level == L) {
+            out.isp = entry;
+            out.ns  = ns;
+            out.nd  = nd;
+            out.ix  = ix;
+            out.iy  = iy;
+            out.iz  = iz;
+            return ST_HIT;
+        }
+        if (it > s.iter_cap) return ST_ABORT;
+
+        /* ---- expansion of the node just entered (octree_fsh.c:251-330) ---- */
+        int4 sl = make_int4(0, 0, 0, 0), sh = sl, dl = sl, dh = sl;
+        if (ns != 0 || level == 0) {
+            const int4* p = s.stat.child + 2 * (size_t) ns;
+            sl            = p[0];
+            sh            = p[1];
+            out.exps++;
+        }
+        if (DUAL && (nd != 0 || level == 0)) {
+            const int4* p = s.dyn.child + 2 * (size_t) nd;
+            dl            = p[0];
+            dh            = p[1];
+            out.exps++;
+        }
+        unsigned mask = exist_mask(sl, sh);
+        if (DUAL) mask |= exist_mask(dl, dh);
+
+        float tx, ty, tz, tw;
+        cube_at<GRID>(s, level, ix, iy, iz, tx, ty, tz, tw);
+        const float bxx = tx + tw, byy = ty - tw, bzz = tz - tw;
+        const float hx = bxx + (tx - bxx) * 0.5f;
+        const float hy = byy + (ty - byy) * 0.5f;
+        const float hz = bzz + (tz - bzz) * 0.5f;
+
+        const float qnan = __builtin_nanf("");
+        Cand        c0 = {entry.x, entry.y, entry.z, entry.w, 0};
+        Cand        c1, c2, c3;
+        {
+            float4 a = plane_z<CPU>(hz, r);
+            bool   ok = a.w > 0.0f && tx < a.x && a.x <= bxx && ty > a.y && a.y >= byy;
+            c1        = {a.x, a.y, a.z, ok ? a.w : qnan, 1};
+        }
+        {
+            float4 a = plane_x<CPU>(hx, r);
+            bool   ok = a.w > 0.0f && ty > a.y && a.y >= byy && tz > a.z && a.z >= bzz;
+            c2        = {a.x, a.y, a.z, ok ? a.w : qnan, 2};
+        }
+        {
+            float4 a = plane_y<CPU>(hy, r);
+            bool   ok = a.w > 0.0f && tx < a.x && a.x <= bxx && tz > a.z && a.z >= bzz;
+            c3        = {a.x, a.y, a.z, ok ? a.w : qnan, 3};
+        }
+        /* selection-swap sort; rejected slots hold NaN and never move */
+        cswap(c0, c1);
+        cswap(c0, c2);
+        cswap(c0, c3);
+        cswap(c1, c2);
+        cswap(c1, c3);
+        cswap(c2, c3);
+
+        int      pre = -1, n = 0, foct = 0;
+        uint32_t word = 0;
+        float4   first = entry;
+#define QBT_CLASSIFY(c)                                                                     \
+    if ((c).w == (c).w) {                                                                   \
+        int o = ((c).x > hx ? 1 : 0) | ((c).y < hy ? 2 : 0) | ((c).z < hz ? 4 : 0);         \
+        if (o == pre) {                                                                     \
+            if ((c).x == hx) o ^= 1;                                                        \
+            else if ((c).y == hy) o ^= 2;                                                   \
+            else if ((c).z == hz) o ^= 4;                                                   \
+        }                                                                                   \
+        pre = o;                                                                            \
+        if ((mask >> o) & 1u) {                                                             \
+            word |= (uint32_t) (o | ((c).src << 3)) << (8 + 5 * n);                         \
+            if (n == 0) {                                                                   \
+                first = make_float4((c).x, (c).y, (c).z, (c).w);                            \
+                foct  = o;                                                                  \
+            }                                                                               \
+            n++;                                                                            \
+        }                                                                                   \
+    }
+        QBT_CLASSIFY(c0)
+        QBT_CLASSIFY(c1)
+        QBT_CLASSIFY(c2)
+        QBT_CLASSIFY(c3)
+#undef QBT_CLASSIFY
+
+        if (n > 0) {
+            /* pop the first pending entry now (octree_fsh.c:336-367) */
+            st.pw[level * st.stride] = word | (uint32_t) n | (1u << 3);
+            st.ns[level * st.stride] = ns;
+            if (DUAL) st.nd[level * st.stride] = nd;
+            int cs = pick8(sl, sh, foct);
+            int cd = DUAL ? pick8(dl, dh, foct) : 0;
+            ix     = (ix << 1) | (foct & 1);
+            iy     = (iy << 1) | ((foct >> 1) & 1);
+            iz     = (iz << 1) | (foct >> 2);
+            ns     = cs;
+            nd     = cd;
+            entry  = first;
+            level++;
+            continue;
+        }
+
+        /* ---- backtrack (octree_fsh.c:368-375) and pop the next pending ---- */
+        for (;;) {
+            level--;
+            if (level < 0) return ST_MISS;
+            ix >>= 1;
+            iy >>= 1;
+            iz >>= 1;
+            uint32_t w = st.pw[level * st.stride];
+            int      k = (int) ((w >> 3) & 7u), cnt = (int) (w & 7u);
+            if (k < cnt) {
+                int e   = (int) ((w >> (8 + 5 * k)) & 31u);
+                int o   = e & 7, src = e >> 3;
+                st.pw[level * st.stride] = (w & ~(7u << 3)) | ((uint32_t) (k + 1) << 3);
+                float4 isp;
+                if (src == 0) {
+                    isp = entry_of<CPU, GRID>(s, r, st, level, ix, iy, iz, start);
+                } else {
+                    float x, y, z, ww;
+                    cube_at<GRID>(s, level, ix, iy, iz, x, y, z, ww);
+                    isp = midplane<CPU>(src, r, x, y, z, ww);
+                }
+                int ps = st.ns[level * st.stride];
+                int cs = child_slot(s.stat, ps, o, level);
+                int cd = 0;
+                if (DUAL) cd = child_slot(s.dyn, st.nd[level * st.stride], o, level);
+                ix    = (ix << 1) | (o & 1);
+                iy    = (iy << 1) | ((o >> 1) & 1);
+                iz    = (iz << 1) | (o >> 2);
+                ns    = cs;
+                nd    = cd;
+                entry = isp;
+                level++;
+                break;
+            }
+        }
+    }
+}
+
+} // namespace qbt

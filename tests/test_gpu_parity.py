@@ -1,0 +1,283 @@
+"""Parity of the gfx950 traversal/shading (libqbt.so, called through its C ABI)
+with the CPU oracle on the same seeded inputs.
+
+Bar: hit model index, leaf node index, dynamic model index, shadow bits, pixel
+flags and the primary entry point (x, y, z, t) bit-exact; colour within 1/255
+(float) and 1 unit of RGBA8; expansion counts equal.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import qubatron_amd as q
+from oracle import pyoracle as po
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+ALL = q.OUT_ALL | q.RENDER_COUNT
+
+
+def _render_both(ctx, osc, camfp, angle, W, H_, lights, basecube, maxlevel, shoot=0):
+    v = q.view(camfp, angle, W, H_, basecube, maxlevel, shoot)
+    print(f"[case] render cam={tuple(np.round(camfp, 2))} ang={tuple(np.round(angle, 3))} {W}x{H_} L={maxlevel}",
+          flush=True)
+    ctx.render(v, lights, ALL)
+    g = ctx.read_frame(0)
+    o = osc.render(camfp, angle, W, H_, lights, shoot=shoot)
+    return g, o
+
+
+def _assert_same(g, o, what):
+    mm = H.compare_frames(g, o)
+    assert all(v == 0 for v in mm.values()), f"{what}: mismatches {mm}"
+
+
+def _upload_scene(ctx, nodes, col, nrm, dyn_nodes=None, dyn_col=None, dyn_nrm=None):
+    ctx.upload_octree(nodes)
+    ctx.upload_model(col, nrm)
+    if dyn_nodes is None:
+        dyn_nodes = np.zeros((1, 12), np.int32)
+    ctx.upload_octree(dyn_nodes, dynamic=True)
+    if dyn_col is not None:
+        ctx.upload_model(dyn_col, dyn_nrm, dynamic=True)
+
+
+def test_config1_reference_view(gpu_ctx):
+    """Config 1: 1M points, depth 10, 320x240, camera at the centre looking +z."""
+    sc, pnt, nrm, col, nodes = H.scene(1)
+    _upload_scene(gpu_ctx, nodes, col, nrm)
+    osc = H.oracle_scene(1)
+    g, o = _render_both(gpu_ctx, osc, (900.0, 900.0, 900.0), (np.pi, 0.0), 320, 240, sc.lights, sc.basecube,
+                        sc.depth)
+    _assert_same(g, o, "config1")
+    st = gpu_ctx.stats()
+    assert st["primary_rays"] == 320 * 240
+    assert st["hits"] == int(np.count_nonzero(o["flags"] & po.PX_SHADED))
+    assert st["expansions"] == int(o["exp"].sum())
+    assert st["grid_exact"] == 1
+    assert (o["hit"] >= 0).mean() > 0.1  # the view sees surfaces
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_config1_random_poses(gpu_ctx, seed):
+    sc, pnt, nrm, col, nodes = H.scene(1)
+    _upload_scene(gpu_ctx, nodes, col, nrm)
+    osc = H.oracle_scene(1)
+    pos, ang = sc.poses(2, seed=seed)
+    for i in range(2):
+        g, o = _render_both(gpu_ctx, osc, pos[i], ang[i], 160, 120, sc.lights, sc.basecube, sc.depth,
+                            shoot=i)
+        _assert_same(g, o, f"pose {seed}/{i}")
+
+
+def test_multi_light_and_views(gpu_ctx):
+    """Two views in one call, four lights (multi-light extension)."""
+    sc, pnt, nrm, col, nodes = H.scene(1)
+    _upload_scene(gpu_ctx, nodes, col, nrm)
+    osc = H.oracle_scene(1)
+    lights = np.array([[900, 1700, 900], [300, 1500, 1400], [1500, 1600, 300], [1450, 900, 1500]], np.float32)
+    pos, ang = sc.poses(2, seed=11)
+    views = [q.view(pos[i], ang[i], 96 + 32 * i, 64 + 16 * i, sc.basecube, sc.depth) for i in range(2)]
+    print("[case] two views, four lights", flush=True)
+    gpu_ctx.render(views, lights, ALL)
+    for i in range(2):
+        g = gpu_ctx.read_frame(i)
+        o = osc.render(pos[i], ang[i], 96 + 32 * i, 64 + 16 * i, lights)
+        _assert_same(g, o, f"view {i}")
+
+
+def test_non_grid_basecube(gpu_ctx):
+    """A basecube whose subdivision is not exact in fp32: the running-sum cube
+    path replays the reference's additions."""
+    sc, pnt, nrm, col, nodes = H.scene(1, npoints=200000)
+    bc = np.array([0.3, 1000.7, 1001.1, 1000.7], np.float32)
+    lo = np.array([bc[0], bc[1] - bc[3], bc[2] - bc[3]], np.float32)
+    p2 = (pnt * np.float32(bc[3] / 1800.0)).astype(np.float32) + lo
+    p2 = np.clip(p2, lo + 0.01, lo + bc[3] - 0.01).astype(np.float32)
+    tree = q.octree_from_points(p2, bc, 9)
+    nodes2 = tree.nodes()
+    _upload_scene(gpu_ctx, nodes2, col, nrm)
+    osc = po.OracleScene(nodes2, col, nrm, bc, 9)
+    cam = np.array([500.0, 500.0, 500.0], np.float32)
+    g, o = _render_both(gpu_ctx, osc, cam, (2.0, 0.3), 128, 96, [[500.0, 950.0, 500.0]], bc, 9)
+    assert gpu_ctx.stats()["grid_exact"] == 0
+    _assert_same(g, o, "non-grid")
+
+
+def test_dual_tree(gpu_ctx):
+    """Static scene + a populated dynamic tree traversed in lockstep
+    (octree_fsh.c:223-241, 313-317, 355-356)."""
+    sc, pnt, nrm, col, nodes = H.scene(1, npoints=300000)
+    rng = np.random.default_rng(5)
+    # a dense "figure" of dynamic points in front of the camera + a few inside static walls
+    n_d = 60000
+    dp = np.stack([rng.uniform(820, 980, n_d), rng.uniform(700, 1100, n_d), rng.uniform(1150, 1250, n_d)], 1)
+    dp = dp.astype(np.float32)
+    dn = np.tile(np.array([[0, 0, -1]], np.float32), (n_d, 1))
+    dc = rng.uniform(0, 1, (n_d, 3)).astype(np.float32)
+    dtree = q.octree_from_points(dp, sc.basecube, sc.depth)
+    dnodes = dtree.nodes()
+    _upload_scene(gpu_ctx, nodes, col, nrm, dnodes, dc, dn)
+    osc = po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth, dyn_nodes=dnodes, dyn_col=dc, dyn_nrm=dn)
+    g, o = _render_both(gpu_ctx, osc, (900.0, 900.0, 700.0), (np.pi, 0.1), 200, 150, sc.lights, sc.basecube,
+                        sc.depth)
+    assert gpu_ctx.stats()["dual"] == 1
+    _assert_same(g, o, "dual")
+    assert np.count_nonzero(o["dyn"] > 0) > 100  # dynamic points are visible
+
+
+def test_camera_outside_and_discard(gpu_ctx):
+    """Camera outside the cube (rays entering through faces, discards at the
+    root, OCTTEST camera of qubatron.c:140) on the 5-point OCTTEST scene
+    (modelutil.c:91-110) at depth 12."""
+    pts = np.array([[10, 690, 10], [10, 340, 10], [10, 340, 690], [10, 10, 10], [690, 10, 690]], np.float32)
+    nrm = np.tile(np.array([[0, 0, -1]], np.float32), (5, 1))
+    col = np.ones((5, 3), np.float32)
+    bc = np.array([0.0, 1800.0, 1800.0, 1800.0], np.float32)
+    tree = q.octree_from_points(pts, bc, 12)
+    nodes = tree.nodes()
+    _upload_scene(gpu_ctx, nodes, col, nrm)
+    osc = po.OracleScene(nodes, col, nrm, bc, 12)
+    for cam, ang in [((900.0, 900.0, 3000.0), (0.0, 0.0)), ((-500.0, 2500.0, 900.0), (1.2, -0.5)),
+                     ((10.2, 690.2, 2000.0), (0.0, 0.0))]:
+        g, o = _render_both(gpu_ctx, osc, cam, ang, 128, 128, [[420.0, 200.0, 680.0]], bc, 12)
+        _assert_same(g, o, f"outside {cam}")
+
+
+@pytest.mark.parametrize("maxlevel", [0, 1, 2, 5])
+def test_shallow_trees(gpu_ctx, maxlevel):
+    sc, pnt, nrm, col, nodes = H.scene(1, npoints=5000)
+    tree = q.octree_from_points(pnt, sc.basecube, maxlevel)
+    nodes = tree.nodes()
+    _upload_scene(gpu_ctx, nodes, col, nrm)
+    osc = po.OracleScene(nodes, col, nrm, sc.basecube, maxlevel)
+    g, o = _render_both(gpu_ctx, osc, (900.0, 900.0, 900.0), (0.7, 0.2), 64, 48, sc.lights, sc.basecube, maxlevel)
+    _assert_same(g, o, f"maxlevel {maxlevel}")
+
+
+def test_empty_tree(gpu_ctx):
+    col = np.zeros((1, 3), np.float32)
+    nodes = np.zeros((1, 12), np.int32)
+    _upload_scene(gpu_ctx, nodes, col, col)
+    osc = po.OracleScene(nodes, col, col, (0, 1800, 1800, 1800), 10)
+    g, o = _render_both(gpu_ctx, osc, (900.0, 900.0, 900.0), (0.0, 0.0), 32, 32, [[900, 1700, 900]],
+                        (0, 1800, 1800, 1800), 10)
+    _assert_same(g, o, "empty")
+    assert np.all(g["hit"] == -1)
+
+
+@pytest.mark.parametrize("semantics", [0, 1])
+def test_trace_rays(gpu_ctx, semantics):
+    """Batched octree_trace_line: random rays, plus axis-aligned rays whose
+    zero direction components exercise the two plane-test conventions."""
+    sc, pnt, nrm, col, nodes = H.scene(1)
+    _upload_scene(gpu_ctx, nodes, col, nrm)
+    osc = H.oracle_scene(1)
+    rng = np.random.default_rng(7 + semantics)
+    n = 20000
+    org = rng.uniform(-200, 2000, (n, 3)).astype(np.float32)
+    dirs = rng.normal(size=(n, 3)).astype(np.float32) * 10
+    k = n // 4
+    ax = rng.integers(0, 3, k)
+    dirs[:k] = 0
+    dirs[np.arange(k), ax] = rng.choice([-10.0, 10.0], k)
+    org[:k // 2] = rng.uniform(1, 1799, (k // 2, 3))
+    # origins exactly on the y=0 / z=0 edge region (the CPU twin's (0,0,0,FLT_MAX) quirk)
+    org[k // 2:k // 2 + 200, 1] = rng.uniform(0, 5, 200)
+    org[k // 2:k // 2 + 200, 2] = rng.uniform(0, 5, 200)
+    print(f"[case] trace_rays semantics={semantics} n={n}", flush=True)
+    g = gpu_ctx.trace_rays(org, dirs, sc.basecube, sc.depth, semantics)
+    o = osc.trace_many(org, dirs, semantics)
+    for key in ("status", "model", "leaf"):
+        bad = np.count_nonzero(g[key] != o[key])
+        assert bad == 0, f"{key}: {bad} mismatches"
+    assert np.array_equal(g["isp"], o["isp"])
+    assert np.array_equal(g["tlf"], o["tlf"])
+    assert np.count_nonzero(o["status"] == 1) > 1000
+
+
+def test_range_updates(gpu_ctx):
+    """In-place zeroing and appended octets through partial uploads
+    (modelutil.c:425-500): the device tree equals the edited host tree."""
+    sc, pnt, nrm, col, _ = H.scene(1, npoints=300000)
+    tree = q.octree_from_points(pnt, sc.basecube, sc.depth)
+    nodes = tree.nodes()
+    cap = tree.len + 50000
+    big = np.zeros((cap, 12), np.int32)
+    big[:tree.len] = nodes
+    # first upload with a generous size, then range updates as the reference does
+    gpu_ctx.upload(big, q._ffi.TYPE_INT, tree.len * 48, 16, 0, tree.len * 48, q._ffi.BUFFER_STATIC_OCTREE)
+    gpu_ctx.upload_model(col, nrm)
+    gpu_ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
+    rng = np.random.default_rng(3)
+    victims = pnt[rng.choice(pnt.shape[0], 3000, replace=False)]
+    for p in victims:  # punch: zero slots, upload the one modified node
+        m, o = tree.remove_point(p)
+        if o >= 0:
+            nv = tree.nodes_view()
+            gpu_ctx.upload(nv, q._ffi.TYPE_INT, tree.len * 48, 16, o * 48, (o + 1) * 48,
+                           q._ffi.BUFFER_STATIC_OCTREE)
+    newp = (victims[:2000] + rng.normal(0, 3, (2000, 3))).astype(np.float32).clip(1, 1799)
+    for i, p in enumerate(newp):  # re-insert moved points: upload every touched node
+        oa = np.zeros(sc.depth + 1, np.int32)
+        tree.insert_point(i, p, octindarr=oa)
+        nv = tree.nodes_view()
+        for o in oa[oa > 0]:
+            gpu_ctx.upload(nv, q._ffi.TYPE_INT, tree.len * 48, 16, int(o) * 48, (int(o) + 1) * 48,
+                           q._ffi.BUFFER_STATIC_OCTREE)
+    final = tree.nodes()
+    back = gpu_ctx.download_octree(tree.len)
+    assert np.array_equal(back, final)
+    osc = po.OracleScene(final, col, nrm, sc.basecube, sc.depth)
+    g, o = _render_both(gpu_ctx, osc, (900.0, 900.0, 900.0), (np.pi, 0.0), 160, 120, sc.lights, sc.basecube,
+                        sc.depth)
+    _assert_same(g, o, "after edits")
+
+
+def test_update_entry_point(gpu_ctx):
+    """qbt_update derives light/basecube/dimensions like octree_glc_update."""
+    sc, pnt, nrm, col, nodes = H.scene(1)
+    _upload_scene(gpu_ctx, nodes, col, nrm)
+    lighta = 0.7
+    print("[case] qbt_update", flush=True)
+    gpu_ctx.update(640.0, 480.0, (900.0, 900.0, 900.0), (np.pi, 0.1, 0.0), lighta, 6, sc.depth, 1800.0, 1)
+    g = gpu_ctx.read_frame(0)
+    libm = ctypes.CDLL("libm.so.6")
+    libm.sinf.argtypes = [ctypes.c_float]
+    libm.sinf.restype = ctypes.c_float
+    s = float(libm.sinf(lighta))  # the host sinf the library uses
+    light = np.array([420.0, np.float32(200.0 - s * 20.0), np.float32(680.0 - s * 200.0)], np.float32)
+    osc = H.oracle_scene(1)
+    o = osc.render((900.0, 900.0, 900.0), (np.pi, 0.1), np.float32(640.0 / 3.0), np.float32(480.0 / 3.0), light,
+                   shoot=1)
+    assert g["hit"].shape == (160, 213)
+    _assert_same(g, o, "update")
+
+
+def test_invalid_tree_rejected(gpu_ctx):
+    nodes = np.zeros((3, 12), np.int32)
+    nodes[0, 0] = 1
+    nodes[1, 3] = 7  # out of range
+    col = np.zeros((3, 3), np.float32)
+    _upload_scene(gpu_ctx, nodes, col, col)
+    v = q.view((900, 900, 900), (0, 0), 16, 16, (0, 1800, 1800, 1800), 4)
+    with pytest.raises(q.QbtError) as e:
+        gpu_ctx.render(v, [[900, 1700, 900]])
+    assert e.value.code == q._ffi.QBT_ETREE
+
+
+def test_upload_argument_checks(gpu_ctx):
+    nodes = np.zeros((2, 12), np.int32)
+    T = q._ffi
+    for args in [(T.TYPE_FLOAT, 96, 16, 0, 96, T.BUFFER_STATIC_OCTREE),   # wrong type
+                 (T.TYPE_INT, 96, 12, 0, 96, T.BUFFER_STATIC_OCTREE),     # wrong itemsize
+                 (T.TYPE_INT, 96, 16, 32, 16, T.BUFFER_STATIC_OCTREE),    # start > end
+                 (T.TYPE_INT, 96, 16, 0, 112, T.BUFFER_STATIC_OCTREE),    # end > size
+                 (T.TYPE_INT, 80, 16, 0, 80, T.BUFFER_STATIC_OCTREE),     # partial node
+                 (T.TYPE_INT, 96, 16, 8, 96, T.BUFFER_STATIC_OCTREE),     # unaligned start
+                 (T.TYPE_INT, 96, 16, 0, 96, 9)]:                         # bad buffer id
+        with pytest.raises(q.QbtError) as e:
+            gpu_ctx.upload(nodes, *args)
+        assert e.value.code == T.QBT_EINVAL
