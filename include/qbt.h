@@ -168,6 +168,25 @@ typedef struct qbt_stats {
 
 int qbt_get_stats(qbt_ctx* ctx, qbt_stats* out);
 
+/* Wait for all work queued on the context stream. */
+int qbt_synchronize(qbt_ctx* ctx);
+
+/* Kernel timing: between qbt_profile_begin and qbt_profile_end every launch
+ * of the traversal kernel (render or ray batch) is bracketed by a HIP event
+ * pair on the context stream (up to 4096 launches). qbt_profile_end
+ * synchronises and reports launches and their durations in milliseconds. */
+typedef struct qbt_profile {
+    uint32_t launches;
+    uint32_t reserved;
+    double   total_ms;
+    double   avg_ms;
+    double   min_ms;
+    double   max_ms;
+} qbt_profile;
+
+int qbt_profile_begin(qbt_ctx* ctx);
+int qbt_profile_end(qbt_ctx* ctx, qbt_profile* out);
+
 /* Batched octree_trace_line: n rays (origins, dirs: n x 3 floats, host).
  * semantics 0 = shader (octree_fsh.c cube_trace_line, dual tree), 1 = CPU
  * twin (octree.c:341-537: static tree only, its plane-test returns). Outputs

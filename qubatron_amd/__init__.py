@@ -274,8 +274,11 @@ class Context:
         arr = (_ffi.View * len(views))(*views)
         lights = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 3)
         check(self._d.qbt_render(self._h, arr, len(views), fptr(lights), lights.shape[0], int(mask)), "qbt_render")
+        outs = int(mask) & OUT_ALL
+        if not (int(mask) & RENDER_COUNT):
+            outs &= ~OUT_EXP  # expansion counts exist only in counting renders
         for i, v in enumerate(views):
-            self._views[i] = (int(v.width), int(v.height), int(mask) & OUT_ALL)
+            self._views[i] = (int(v.width), int(v.height), outs)
 
     def update(self, width, height, position, angle, lighta, quality, maxlevel, basesize, shoot):
         pos = np.asarray(position, dtype=np.float32)
@@ -305,6 +308,17 @@ class Context:
         rc = self._d.qbt_get_stats(self._h, C.byref(st))
         check(rc, "qbt_get_stats")
         return {k: getattr(st, k) for k, _ in _ffi.Stats._fields_}
+
+    def synchronize(self):
+        check(self._d.qbt_synchronize(self._h), "qbt_synchronize")
+
+    def profile_begin(self):
+        check(self._d.qbt_profile_begin(self._h), "qbt_profile_begin")
+
+    def profile_end(self):
+        p = _ffi.Profile()
+        check(self._d.qbt_profile_end(self._h, C.byref(p)), "qbt_profile_end")
+        return {k: getattr(p, k) for k, _ in _ffi.Profile._fields_ if k != "reserved"}
 
     def trace_rays(self, origins, dirs, basecube, maxlevel, semantics=0):
         o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)

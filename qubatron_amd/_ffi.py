@@ -70,6 +70,11 @@ class Frame(C.Structure):
                 ("dyn", C.c_void_p), ("exp", C.c_void_p)]
 
 
+class Profile(C.Structure):
+    _fields_ = [("launches", C.c_uint32), ("reserved", C.c_uint32), ("total_ms", C.c_double), ("avg_ms", C.c_double),
+                ("min_ms", C.c_double), ("max_ms", C.c_double)]
+
+
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("trace_ms", C.c_double), ("primary_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("hits", C.c_uint64), ("expansions", C.c_uint64),
@@ -155,7 +160,7 @@ def host():
 DEV_SYMBOLS = [
     "qbt_create", "qbt_destroy", "qbt_strerror", "qbt_device_bytes", "qbt_upload", "qbt_download", "qbt_render",
     "qbt_update", "qbt_read_frame", "qbt_frame_device", "qbt_get_stats", "qbt_trace_rays", "qbt_build_octree",
-    "qbt_version", "qbt_last_hip_error",
+    "qbt_version", "qbt_last_hip_error", "qbt_synchronize", "qbt_profile_begin", "qbt_profile_end",
 ]
 
 
@@ -179,6 +184,9 @@ def _declare_dev(d):
     d.qbt_trace_rays.argtypes = [vp, f32p, f32p, C.c_size_t, C.c_int, f32p, C.c_int, i32p, i32p, i32p, f32p,
                                  f32p]
     d.qbt_build_octree.argtypes = [vp, C.c_int, f32p, C.c_int, C.c_size_t, C.POINTER(C.c_size_t)]
+    d.qbt_synchronize.argtypes = [vp]
+    d.qbt_profile_begin.argtypes = [vp]
+    d.qbt_profile_end.argtypes = [vp, C.POINTER(Profile)]
     d.qbt_version.restype = C.c_char_p
     d.qbt_last_hip_error.restype = C.c_char_p
 

@@ -1,13 +1,37 @@
 /*
- * build.hip - GPU octree construction (qbt_build_octree).
+ * build.hip - octree construction on the GPU (qbt_build_octree).
+ *
+ * Produces, node for node, the array that inserting points 0..n-1 in index
+ * order with octree_insert_point (src/qubatron/octree.c:95-147) produces:
+ *
+ *  - every point's octet path is computed with the reference's fp32 formula
+ *    ((int)(p/size)%2 with size halved per level, octree.c:102-109) and
+ *    packed into a key, 3 bits per level, first level highest;
+ *  - a stable radix sort groups equal prefixes while keeping index order;
+ *  - the node of a depth-d prefix is created by the smallest point index in
+ *    its group (the first inserter); per depth, a reduce-by-key with min over
+ *    the sorted keys gives the groups and their first inserters, walking up
+ *    from the leaves (a parent group is a run of child groups);
+ *  - a point that is the first inserter at depth d is so at every deeper
+ *    depth, so point i creates nodes at depths dmin_i..L consecutively and the
+ *    node index of (i, d) is 1 + sum_{j<i} count_j + (d - dmin_i): an
+ *    exclusive scan of the per-point counts in index order;
+ *  - slot 8 of a node is its first inserter (the model index), and its
+ *    parent's slot for the last octet of the prefix points to it.
  */
 #include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
 
 #include "qbt.h"
 #include "qbt_internal.hpp"
 
+#pragma clang fp contract(off)
+
 struct BuildState {
-    int unused;
+    int unused = 0;
 };
 
 void qbt_ctx::builder_free()
@@ -16,9 +40,254 @@ void qbt_ctx::builder_free()
     build = nullptr;
 }
 
+namespace {
+
+constexpr int kMaxBuildLevels = 21; /* 3 bits per level in a 64-bit key */
+
+__global__ void k_paths(const float* __restrict__ xyz, size_t n, float edge, int L, uint64_t* __restrict__ keys,
+                        uint32_t* __restrict__ vals, unsigned int* __restrict__ bad)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float    x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    float    size = edge;
+    uint64_t key  = 0;
+    bool     ok   = true;
+    for (int l = 0; l < L; l++) {
+        size = size / 2.0f;
+        float qx = x / size, qy = y / size, qz = z / size;
+        /* outside the int range (or NaN) the reference's conversion is undefined */
+        ok = ok && (qx > -2147483520.0f && qx < 2147483520.0f) && (qy > -2147483520.0f && qy < 2147483520.0f) &&
+             (qz > -2147483520.0f && qz < 2147483520.0f);
+        int o  = ok ? (int) qx % 2 : 0;
+        int yi = ok ? (int) qy % 2 : 1;
+        int zi = ok ? (int) qz % 2 : 1;
+        if (yi == 0) o += 2;
+        if (zi == 0) o += 4;
+        if (o < 0) ok = false;
+        key = (key << 3) | (uint64_t) (o & 7);
+    }
+    if (!ok) atomicAdd(bad, 1u);
+    keys[i] = key;
+    vals[i] = (uint32_t) i;
+}
+
+__global__ void k_shift_keys(const uint64_t* __restrict__ in, size_t n, uint64_t* __restrict__ out)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i] >> 3;
+}
+
+/* head[j] = 1 where the parent key of child group j starts a new parent group */
+__global__ void k_heads(const uint64_t* __restrict__ pk, size_t n, uint32_t* __restrict__ head)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) head[i] = (i == 0 || pk[i] != pk[i - 1]) ? 1u : 0u;
+}
+
+__global__ void k_fill_u32v(uint32_t* p, size_t n, uint32_t v)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_dmin(const uint32_t* __restrict__ first, size_t n, uint32_t depth, uint32_t* __restrict__ dmin)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicMin(&dmin[first[i]], depth);
+}
+
+__global__ void k_counts(const uint32_t* __restrict__ dmin, size_t n, uint32_t L, uint32_t* __restrict__ cnt)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = dmin[i] <= L ? L + 1 - dmin[i] : 0u;
+}
+
+/* node index of every group at one depth, slot 8, and the parent's child slot */
+__global__ void k_link(const uint64_t* __restrict__ ukey, const uint32_t* __restrict__ first, size_t n, uint32_t depth,
+                       const uint32_t* __restrict__ base, const uint32_t* __restrict__ dmin,
+                       const uint32_t* __restrict__ parent_pos, const uint32_t* __restrict__ parent_node,
+                       uint32_t* __restrict__ node_out, int* __restrict__ child, int* __restrict__ model)
+{
+    size_t j = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    uint32_t f   = first[j];
+    uint32_t idx = 1u + base[f] + (depth - dmin[f]);
+    node_out[j]  = idx;
+    model[idx]   = (int) f;
+    uint32_t par = depth == 1 ? 0u : parent_node[parent_pos[j] - 1u]; /* parent_pos is 1-based */
+    child[8 * (size_t) par + (ukey[j] & 7u)] = (int) idx;
+}
+
+inline unsigned nblk(size_t n) { return (unsigned) ((n + 255) / 256); }
+
+} // namespace
+
+#define HIPCHK(x) QBT_HIPCHK(x)
+
+/* scratch allocation local to one build, released at the end */
+struct Scratch {
+    qbt_ctx*            c;
+    std::vector<DevBuf> bufs;
+    explicit Scratch(qbt_ctx* cc) : c(cc) {}
+    ~Scratch()
+    {
+        for (auto& b : bufs) c->dev_free(b);
+    }
+    void* get(size_t bytes, int* rc)
+    {
+        DevBuf b;
+        *rc = c->dev_alloc(b, bytes ? bytes : 16);
+        if (*rc) return nullptr;
+        bufs.push_back(b);
+        return b.p;
+    }
+};
+
 extern "C" int qbt_build_octree(qbt_ctx* c, int tree, const float basecube[4], int levels, size_t npoints,
                                 size_t* nodes_out)
 {
-    (void) c, (void) tree, (void) basecube, (void) levels, (void) npoints, (void) nodes_out;
-    return QBT_ENOTSUP;
+    if (!c || !basecube) return QBT_EINVAL;
+    if (tree != QBT_BUFFER_STATIC_OCTREE && tree != QBT_BUFFER_DYNAMIC_OCTREE) return QBT_EINVAL;
+    if (levels < 0 || levels > kMaxBuildLevels) return QBT_ENOTSUP;
+    const int t = tree == QBT_BUFFER_STATIC_OCTREE ? 0 : 1;
+    if (npoints > c->pts[t].n || npoints >= (size_t) UINT32_MAX) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    hipStream_t    s = c->stream;
+    const uint32_t L = (uint32_t) levels;
+    int            rc = QBT_OK;
+    Scratch        sc(c);
+    size_t         n = npoints;
+
+    /* 1. paths and stable sort */
+    uint64_t*     k0   = (uint64_t*) sc.get(n * 8, &rc);
+    uint64_t*     k1   = rc ? nullptr : (uint64_t*) sc.get(n * 8, &rc);
+    uint32_t*     v0   = rc ? nullptr : (uint32_t*) sc.get(n * 4, &rc);
+    uint32_t*     v1   = rc ? nullptr : (uint32_t*) sc.get(n * 4, &rc);
+    unsigned int* bad  = rc ? nullptr : (unsigned int*) sc.get(64, &rc);
+    uint32_t*     ucnt = rc ? nullptr : (uint32_t*) sc.get(64, &rc);
+    if (rc) return rc;
+    k_fill_u32v<<<1, 64, 0, s>>>(bad, 1, 0u);
+    if (n) k_paths<<<nblk(n), 256, 0, s>>>((const float*) c->pts[t].xyz.p, n, basecube[3], levels, k0, v0, bad);
+    HIPCHK(hipGetLastError());
+    unsigned int nbad = 0;
+    HIPCHK(hipMemcpyAsync(&nbad, bad, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (nbad) return QBT_ERANGE;
+
+    size_t tmp_bytes = 0, tb;
+    if (n && L) {
+        HIPCHK(rocprim::radix_sort_pairs(nullptr, tb, k0, k1, v0, v1, n, 0, 3 * levels, s));
+        tmp_bytes = tb;
+    }
+    /* the largest reduce_by_key / scan temp over the sizes used below */
+    HIPCHK(rocprim::reduce_by_key(nullptr, tb, k0, v0, n, k1, v1, ucnt, rocprim::minimum<uint32_t>(),
+                                  rocprim::equal_to<uint64_t>(), s));
+    tmp_bytes = tb > tmp_bytes ? tb : tmp_bytes;
+    HIPCHK(rocprim::exclusive_scan(nullptr, tb, v0, v1, 0u, n, rocprim::plus<uint32_t>(), s));
+    tmp_bytes = tb > tmp_bytes ? tb : tmp_bytes;
+    HIPCHK(rocprim::inclusive_scan(nullptr, tb, v0, v1, n, rocprim::plus<uint32_t>(), s));
+    tmp_bytes = tb > tmp_bytes ? tb : tmp_bytes;
+    void* tmp = sc.get(tmp_bytes, &rc);
+    if (rc) return rc;
+
+    const uint64_t* skeys = k0;
+    const uint32_t* svals = v0;
+    if (n && L) {
+        tb = tmp_bytes;
+        HIPCHK(rocprim::radix_sort_pairs(tmp, tb, k0, k1, v0, v1, n, 0, 3 * levels, s));
+        skeys = k1;
+        svals = v1;
+    }
+
+    /* 2. groups per depth, leaves first: ukey[d], first[d] (d = 1..L) */
+    std::vector<uint64_t*> ukey(L + 2, nullptr);
+    std::vector<uint32_t*> first(L + 2, nullptr);
+    std::vector<uint32_t*> ppos(L + 2, nullptr); /* parent position of each group at depth d */
+    std::vector<size_t>    cnt(L + 2, 0);
+    if (n && L) {
+        ukey[L]  = (uint64_t*) sc.get(n * 8, &rc);
+        first[L] = rc ? nullptr : (uint32_t*) sc.get(n * 4, &rc);
+        if (rc) return rc;
+        tb = tmp_bytes;
+        HIPCHK(rocprim::reduce_by_key(tmp, tb, skeys, svals, n, ukey[L], first[L], ucnt, rocprim::minimum<uint32_t>(),
+                                      rocprim::equal_to<uint64_t>(), s));
+        uint32_t u = 0;
+        HIPCHK(hipMemcpyAsync(&u, ucnt, 4, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        cnt[L] = u;
+        for (uint32_t d = L; d > 1; d--) {
+            size_t    m  = cnt[d];
+            uint64_t* pk = (uint64_t*) sc.get(m * 8, &rc);
+            uint32_t* hd = rc ? nullptr : (uint32_t*) sc.get(m * 4, &rc);
+            ppos[d]      = rc ? nullptr : (uint32_t*) sc.get(m * 4, &rc);
+            if (rc) return rc;
+            k_shift_keys<<<nblk(m), 256, 0, s>>>(ukey[d], m, pk);
+            k_heads<<<nblk(m), 256, 0, s>>>(pk, m, hd);
+            HIPCHK(hipGetLastError());
+            tb = tmp_bytes;
+            HIPCHK(rocprim::inclusive_scan(tmp, tb, hd, ppos[d], m, rocprim::plus<uint32_t>(), s));
+            ukey[d - 1]  = (uint64_t*) sc.get(m * 8, &rc);
+            first[d - 1] = rc ? nullptr : (uint32_t*) sc.get(m * 4, &rc);
+            if (rc) return rc;
+            tb = tmp_bytes;
+            HIPCHK(rocprim::reduce_by_key(tmp, tb, pk, first[d], m, ukey[d - 1], first[d - 1], ucnt,
+                                          rocprim::minimum<uint32_t>(), rocprim::equal_to<uint64_t>(), s));
+            HIPCHK(hipMemcpyAsync(&u, ucnt, 4, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+            cnt[d - 1] = u;
+        }
+    }
+
+    /* 3. per-point node counts, exclusive scan in index order */
+    uint32_t* dmin = (uint32_t*) sc.get(n * 4, &rc);
+    uint32_t* pcnt = rc ? nullptr : (uint32_t*) sc.get(n * 4, &rc);
+    uint32_t* base = rc ? nullptr : (uint32_t*) sc.get((n + 1) * 4, &rc);
+    if (rc) return rc;
+    size_t total = 1;
+    if (n && L) {
+        k_fill_u32v<<<nblk(n), 256, 0, s>>>(dmin, n, L + 1);
+        for (uint32_t d = 1; d <= L; d++) k_dmin<<<nblk(cnt[d]), 256, 0, s>>>(first[d], cnt[d], d, dmin);
+        k_counts<<<nblk(n), 256, 0, s>>>(dmin, n, L, pcnt);
+        HIPCHK(hipGetLastError());
+        tb = tmp_bytes;
+        HIPCHK(rocprim::exclusive_scan(tmp, tb, pcnt, base, 0u, n, rocprim::plus<uint32_t>(), s));
+        for (uint32_t d = 1; d <= L; d++) total += cnt[d];
+    }
+    if (total >= (size_t) INT32_MAX) return QBT_ENOTSUP;
+
+    /* 4. the node arrays: zero, then link depth by depth (parents first) */
+    TreeBuf& tr = c->tree[t];
+    if (total > tr.cap_nodes) {
+        size_t cap = total + total / 4 + 1024;
+        c->dev_free(tr.child);
+        c->dev_free(tr.model);
+        tr.cap_nodes = 0;
+        tr.n         = 0;
+        if ((rc = c->dev_alloc(tr.child, cap * 32)) || (rc = c->dev_alloc(tr.model, cap * 4))) return rc;
+        tr.cap_nodes = cap;
+    }
+    k_fill_u32v<<<nblk(total * 8), 256, 0, s>>>((uint32_t*) tr.child.p, total * 8, 0u);
+    k_fill_u32v<<<nblk(total), 256, 0, s>>>((uint32_t*) tr.model.p, total, 0u);
+    HIPCHK(hipGetLastError());
+    if (n && L) {
+        std::vector<uint32_t*> node(L + 2, nullptr);
+        for (uint32_t d = 1; d <= L; d++) {
+            node[d] = (uint32_t*) sc.get(cnt[d] * 4, &rc);
+            if (rc) return rc;
+        }
+        for (uint32_t d = 1; d <= L; d++) {
+            const uint32_t* pp = d == 1 ? nullptr : ppos[d];
+            const uint32_t* pn = d == 1 ? nullptr : node[d - 1];
+            k_link<<<nblk(cnt[d]), 256, 0, s>>>(ukey[d], first[d], cnt[d], d, base, dmin, pp, pn, node[d],
+                                               (int*) tr.child.p, (int*) tr.model.p);
+            HIPCHK(hipGetLastError());
+        }
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    tr.n          = total;
+    tr.validated  = false;
+    tr.full_check = true;
+    if (nodes_out) *nodes_out = total;
+    return QBT_OK;
 }

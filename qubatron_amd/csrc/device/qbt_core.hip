@@ -489,6 +489,7 @@ extern "C" void qbt_destroy(qbt_ctx* c)
     c->dev_free(c->rays);
     c->mlp_free();
     c->builder_free();
+    for (hipEvent_t e : c->prof_ev) (void) hipEventDestroy(e);
     if (c->ev0) (void) hipEventDestroy(c->ev0);
     if (c->ev1) (void) hipEventDestroy(c->ev1);
     if (c->ev2) (void) hipEventDestroy(c->ev2);
@@ -837,7 +838,9 @@ static void launch_render_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, con
     const int    th    = bs / 16;
     const int    tiles = ((v.w + 15) / 16) * ((v.h + th - 1) / th);
     const size_t lds   = qbt_stack_bytes(s.maxlevel, DUAL, bs);
+    c->prof_mark_begin();
     k_render<DUAL, GRID, COUNT><<<tiles, bs, lds, c->stream>>>(s, v, spr, f, mask, (unsigned long long*) c->counters.p);
+    c->prof_mark_end();
 }
 
 int qbt_ctx::block_threads(int L, bool dual) const
@@ -975,6 +978,45 @@ extern "C" int qbt_get_stats(qbt_ctx* c, qbt_stats* out)
     return c->aborted ? QBT_EABORT : QBT_OK;
 }
 
+extern "C" int qbt_synchronize(qbt_ctx* c)
+{
+    if (!c) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return QBT_OK;
+}
+
+extern "C" int qbt_profile_begin(qbt_ctx* c)
+{
+    if (!c) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->profiling = true;
+    c->prof_n    = 0;
+    return QBT_OK;
+}
+
+extern "C" int qbt_profile_end(qbt_ctx* c, qbt_profile* out)
+{
+    if (!c || !out) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->profiling = false;
+    qbt_profile p{};
+    p.min_ms = 0.0;
+    for (uint32_t i = 0; i < c->prof_n; i++) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, c->prof_ev[2 * i], c->prof_ev[2 * i + 1]));
+        p.total_ms += ms;
+        if (i == 0 || ms < p.min_ms) p.min_ms = ms;
+        if (ms > p.max_ms) p.max_ms = ms;
+    }
+    p.launches = c->prof_n;
+    p.avg_ms   = c->prof_n ? p.total_ms / c->prof_n : 0.0;
+    *out       = p;
+    return QBT_OK;
+}
+
 static int frame_fields(qbt_ctx* c, int view, qbt_frame* out, bool device)
 {
     if (!c || !out || view < 0 || view >= (int) c->frames.size()) return QBT_EINVAL;
@@ -1053,6 +1095,7 @@ extern "C" int qbt_trace_rays(qbt_ctx* c, const float* origins, const float* dir
     const size_t lds = qbt_stack_bytes(maxlevel, dual, bs);
     unsigned     nb  = blocks_for(n, bs);
     unsigned long long* cnt = (unsigned long long*) c->counters.p;
+    c->prof_mark_begin();
     if (semantics == 1) {
         if (grid) k_trace_rays<false, true, true><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
         else k_trace_rays<false, true, false><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
@@ -1064,6 +1107,7 @@ extern "C" int qbt_trace_rays(qbt_ctx* c, const float* origins, const float* dir
         else k_trace_rays<false, false, false><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
     }
     HIPCHK(hipGetLastError());
+    c->prof_mark_end();
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     if (status) HIPCHK(hipMemcpyAsync(status, dst, n * 4, hipMemcpyDeviceToHost, c->stream));
     if (model) HIPCHK(hipMemcpyAsync(model, dmod, n * 4, hipMemcpyDeviceToHost, c->stream));

@@ -94,6 +94,29 @@ struct qbt_ctx {
     int          tune_block      = 0;
     MlpState*    mlp             = nullptr;
     BuildState*  build           = nullptr;
+    /* qbt_profile_begin/end: event pairs around traversal-kernel launches */
+    bool                    profiling = false;
+    std::vector<hipEvent_t> prof_ev;
+    uint32_t                prof_n = 0;
+
+    void prof_mark_begin()
+    {
+        if (profiling && prof_n < 4096 && ensure_prof_events()) (void) hipEventRecord(prof_ev[2 * prof_n], stream);
+    }
+    void prof_mark_end()
+    {
+        if (profiling && prof_n < 4096 && prof_ev.size() >= 2 * (size_t) (prof_n + 1))
+            (void) hipEventRecord(prof_ev[2 * prof_n + 1], stream), prof_n++;
+    }
+    bool ensure_prof_events()
+    {
+        while (prof_ev.size() < 2 * (size_t) (prof_n + 1)) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return false;
+            prof_ev.push_back(e);
+        }
+        return true;
+    }
 
     int      dev_alloc(DevBuf& b, size_t bytes);
     void     dev_free(DevBuf& b);

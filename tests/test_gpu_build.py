@@ -1,0 +1,83 @@
+"""GPU octree construction (qbt_build_octree) against the host builder, which
+follows octree_insert_point (src/qubatron/octree.c:95-147) point by point:
+the node arrays must be identical (same indices, same slot 8)."""
+import numpy as np
+import pytest
+
+import qubatron_amd as q
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_build(ctx, pts, basecube, levels):
+    ctx.upload_points(pts)
+    n = ctx.build_octree(basecube, levels, pts.shape[0])
+    return ctx.download_octree(n)
+
+
+def _host_build(pts, basecube, levels):
+    return q.octree_from_points(pts, basecube, levels).nodes()
+
+
+@pytest.mark.parametrize("config,npoints", [(1, 0), (2, 2000000), (3, 3000000)])
+def test_build_matches_host(gpu_ctx, config, npoints):
+    sc = q.Scene(config, seed=0, npoints=npoints)
+    pnt, _, _ = sc.points()
+    print(f"[case] build config {config} n={pnt.shape[0]}", flush=True)
+    g = _gpu_build(gpu_ctx, pnt, sc.basecube, sc.depth)
+    h = _host_build(pnt, sc.basecube, sc.depth)
+    assert g.shape == h.shape
+    assert np.array_equal(g, h)
+
+
+def test_build_special_points(gpu_ctx):
+    """Duplicates (first inserter keeps the leaf), p == edge (truncation wraps
+    to the low half, octree.c:104-109), small negatives (truncate toward 0),
+    and points in reverse spatial order."""
+    bc = np.array([0, 1800, 1800, 1800], np.float32)
+    rng = np.random.default_rng(0)
+    base = rng.uniform(0, 1800, (5000, 3)).astype(np.float32)
+    pts = np.concatenate([base, base[:1000], np.array([[1800, 1800, 1800], [0, 0, 0], [-0.1, 5, 5],
+                                                        [1799.9, -0.2, 1800]], np.float32), base[::-1]])
+    for levels in (0, 1, 3, 12, 16):
+        print(f"[case] special levels={levels}", flush=True)
+        g = _gpu_build(gpu_ctx, pts, bc, levels)
+        h = _host_build(pts, bc, levels)
+        assert np.array_equal(g, h), f"levels {levels}"
+
+
+def test_build_single_and_empty(gpu_ctx):
+    bc = np.array([0, 1800, 1800, 1800], np.float32)
+    one = np.array([[10, 690, 10]], np.float32)
+    g = _gpu_build(gpu_ctx, one, bc, 12)
+    assert g.shape == (13, 12)
+    assert np.array_equal(g, _host_build(one, bc, 12))
+    gpu_ctx.upload_points(one)
+    assert gpu_ctx.build_octree(bc, 12, 0) == 1
+
+
+def test_build_rejects_bad_point(gpu_ctx):
+    """x <= -size with y, z bits non-zero gives octet -1 in the reference."""
+    bc = np.array([0, 1800, 1800, 1800], np.float32)
+    pts = np.array([[-1000.0, 100.0, 100.0]], np.float32)
+    gpu_ctx.upload_points(pts)
+    with pytest.raises(q.QbtError) as e:
+        gpu_ctx.build_octree(bc, 4, 1)
+    assert e.value.code == q._ffi.QBT_ERANGE
+
+
+def test_build_then_render(gpu_ctx):
+    """A GPU-built tree renders identically to the host-built one."""
+    sc, pnt, nrm, col, nodes = H.scene(1)
+    gpu_ctx.upload_points(pnt)
+    gpu_ctx.upload_model(col, nrm)
+    gpu_ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
+    n = gpu_ctx.build_octree(sc.basecube, sc.depth, pnt.shape[0])
+    assert n == nodes.shape[0]
+    v = q.view((900.0, 900.0, 900.0), (np.pi, 0.0), 320, 240, sc.basecube, sc.depth)
+    gpu_ctx.render(v, sc.lights, q.OUT_ALL)
+    g = gpu_ctx.read_frame(0)
+    o = H.oracle_scene(1).render((900.0, 900.0, 900.0), (np.pi, 0.0), 320, 240, sc.lights)
+    mm = H.compare_frames(g, o)
+    assert all(v == 0 for k, v in mm.items() if k != "exp"), mm
