@@ -42,8 +42,15 @@ def test_build_special_points(gpu_ctx):
                                                         [1799.9, -0.2, 1800]], np.float32), base[::-1]])
     for levels in (0, 1, 3, 12, 16):
         print(f"[case] special levels={levels}", flush=True)
+        try:
+            h = _host_build(pts, bc, levels)
+        except q.QbtError as e:  # the small negatives reach an octet of -1 deep down
+            assert e.code == q._ffi.QBT_ERANGE
+            with pytest.raises(q.QbtError) as eg:
+                _gpu_build(gpu_ctx, pts, bc, levels)
+            assert eg.value.code == q._ffi.QBT_ERANGE
+            continue
         g = _gpu_build(gpu_ctx, pts, bc, levels)
-        h = _host_build(pts, bc, levels)
         assert np.array_equal(g, h), f"levels {levels}"
 
 
@@ -58,9 +65,11 @@ def test_build_single_and_empty(gpu_ctx):
 
 
 def test_build_rejects_bad_point(gpu_ctx):
-    """x <= -size with y, z bits non-zero gives octet -1 in the reference."""
+    """x <= -size with odd y and z cells gives octet -1 in the reference."""
     bc = np.array([0, 1800, 1800, 1800], np.float32)
-    pts = np.array([[-1000.0, 100.0, 100.0]], np.float32)
+    pts = np.array([[-1000.0, 1000.0, 1000.0]], np.float32)
+    with pytest.raises(q.QbtError):
+        _host_build(pts, bc, 4)
     gpu_ctx.upload_points(pts)
     with pytest.raises(q.QbtError) as e:
         gpu_ctx.build_octree(bc, 4, 1)
