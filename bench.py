@@ -9,13 +9,19 @@ of the K timed frames. The traversal kernel's average launch duration is
 measured with HIP events on the library's stream (qbt_profile_*), and its
 algorithmic bytes (32 B per node expansion + 28 B per primary leaf hit + the
 per-pixel outputs) give the roofline fraction against 8 TB/s. The CPU oracle
-(the C restatement of the reference traversal) is timed beside it on a
-bounded band of rows of the first pose, using the host's cores, and that band
-is checked bit-exactly against the GPU frame.
+(the C restatement of the reference traversal) is timed beside it on whole
+frames of the same camera path, using the host's cores, and each sampled
+frame is checked bit-exactly against the GPU frame.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
+
+``python bench.py --summarize-pmc DIR...`` instead reads rocprofv3 counter
+CSVs (FETCH_SIZE / WRITE_SIZE / TCC hits of a bench run) and writes the HBM
+traffic summary the roofline line quotes.
 """
 import argparse
+import csv
+import glob
 import json
 import os
 import sys
@@ -27,6 +33,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+COPY_BYTES = 1 << 30
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_render_config3_1080p.json")
+RENDER_KERNEL = "k_render<false, true, false>"
 
 
 def log(*a):
@@ -44,15 +53,59 @@ def parse():
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--pose-seed", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample duration")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample budget")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic_k_render_config3_1080p.json"))
+    ap.add_argument("--traffic", default=TRAFFIC_FILE)
+    ap.add_argument("--summarize-pmc", nargs="*", default=None, metavar="DIR")
     return ap.parse_args()
+
+
+def summarize_pmc(dirs, out_path):
+    """Average per-dispatch counters of the render kernel and of the copy
+    calibration kernel; scale byte counters by copy-kernel truth."""
+    vals = {}
+    for d in dirs:
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    name = row.get("Kernel_Name", "")
+                    kern = "render" if RENDER_KERNEL in name else "copy" if "k_copy16" in name else None
+                    if kern is None:
+                        continue
+                    key = (kern, row["Counter_Name"])
+                    vals.setdefault(key, {})[(f, row.get("Dispatch_Id"))] = float(row["Counter_Value"])
+    avg = {k: float(np.mean(list(v.values()))) for k, v in vals.items()}
+    n = {k: len(v) for k, v in vals.items()}
+    res = {"kernel": RENDER_KERNEL, "copy_bytes": COPY_BYTES, "counters_kib_or_count": avg, "dispatches": n}
+    fr, wr = avg.get(("render", "FETCH_SIZE")), avg.get(("render", "WRITE_SIZE"))
+    fc, wc = avg.get(("copy", "FETCH_SIZE")), avg.get(("copy", "WRITE_SIZE"))
+    if fr is not None and fc:
+        ff = (COPY_BYTES / 1024.0) / fc
+        res["fetch_scale"] = ff
+        res["fetch_bytes_per_launch"] = fr * 1024.0 * ff
+    if wr is not None and wc:
+        wf = (COPY_BYTES / 1024.0) / wc
+        res["write_scale"] = wf
+        res["write_bytes_per_launch"] = wr * 1024.0 * wf
+    if "fetch_bytes_per_launch" in res and "write_bytes_per_launch" in res:
+        res["hbm_bytes_per_launch"] = round(res["fetch_bytes_per_launch"] + res["write_bytes_per_launch"])
+    hit, miss = avg.get(("render", "TCC_HIT_sum")), avg.get(("render", "TCC_MISS_sum"))
+    if hit is not None and miss is not None and hit + miss > 0:
+        res["l2_hit_rate"] = hit / (hit + miss)
+    res["counters_kib_or_count"] = {"%s:%s" % k: v for k, v in avg.items()}
+    res["dispatches"] = {"%s:%s" % k: v for k, v in n.items()}
+    os.makedirs(os.path.dirname(out_path), exist_ok=True)
+    with open(out_path, "w") as fh:
+        json.dump(res, fh, indent=1, sort_keys=True)
+    print(json.dumps(res, indent=1, sort_keys=True))
 
 
 def main():
     args = parse()
+    if args.summarize_pmc is not None:
+        summarize_pmc(args.summarize_pmc, args.traffic)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for --gpus {args.gpus}); "
               "the traversal shards by view or screen tile with no per-frame collective, "
@@ -72,6 +125,8 @@ def main():
     log(f"generated {sc.npoints} points in {time.time() - t0:.2f}s (config {args.config}, seed {args.seed})")
 
     ctx = q.Context(0)
+    copy_ms, copy_gbps = ctx.copy_bandwidth(COPY_BYTES, 5)
+    log(f"copy kernel: {copy_gbps:.0f} GB/s (read+write) on {COPY_BYTES >> 20} MiB")
     t0 = time.time()
     ctx.upload_points(pnt)
     ctx.upload_model(col, nrm)
@@ -81,7 +136,7 @@ def main():
     nnodes = ctx.build_octree(sc.basecube, sc.depth, sc.npoints)
     ctx.synchronize()
     t_build = time.time() - t0
-    log(f"built octree on GPU: {nnodes} nodes ({nnodes * 48 / 1e9:.2f} GB reference layout) in {t_build:.2f}s")
+    log(f"built octree on GPU: {nnodes} nodes ({nnodes * 48 / 1e9:.2f} GB reference layout) in {t_build:.3f}s")
     del pnt
 
     K, Wm = args.steps, args.warmup
@@ -123,48 +178,47 @@ def main():
     out_bytes = W * H * (4 + 4 + len(lights) + 4)
     alg_bytes = [32.0 * e + 28.0 * h + out_bytes for e, h in zip(exps, leafhits)]
     achieved = (sum(alg_bytes) / len(alg_bytes)) / (prof["avg_ms"] * 1e-3) / 1e9
-    traffic = None
+    traffic, l2 = None, None
     if os.path.exists(args.traffic):
         try:
             with open(args.traffic) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
+                tj = json.load(f)
+            traffic, l2 = tj.get("hbm_bytes_per_launch"), tj.get("l2_hit_rate")
         except Exception:
             traffic = None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "alg_bytes_per_launch": round(float(np.mean(alg_bytes))), "kernel": "k_render",
-            "kernel_avg_ms": round(prof["avg_ms"], 4)}
+            "kernel_avg_ms": round(prof["avg_ms"], 4), "l2_hit_rate": l2,
+            "copy_kernel_gbps": round(copy_gbps, 1)}
 
-    # CPU baseline: the oracle on a band of rows of the first timed pose
+    # CPU baseline: the oracle on whole frames of the timed camera path
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle as po
         t0 = time.time()
         nodes = ctx.download_octree(nnodes)
         log(f"downloaded node array for the CPU oracle in {time.time() - t0:.2f}s")
-        v0 = views[Wm]
-        ctx.render(v0, lights, mask | q.OUT_FLAGS | q.OUT_ISP)
-        g = ctx.read_frame(0)
         osc = po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth)
-        mid = H // 2
-        probe = 4
-        ts = time.time()
-        osc.render(pos[Wm], ang[Wm], W, H, lights, rows=(mid, mid + probe), nthreads=nthreads)
-        per_row = (time.time() - ts) / probe
-        nrows = int(max(probe, min(H - mid, args.cpu_seconds / max(per_row, 1e-6))))
-        r0, r1 = max(0, mid - nrows // 2), max(0, mid - nrows // 2) + nrows
-        ts = time.time()
-        o = osc.render(pos[Wm], ang[Wm], W, H, lights, rows=(r0, r1), nthreads=nthreads)
-        tc = time.time() - ts
-        band = slice(r0, r1)
-        cpu_rays = W * (r1 - r0) + int(np.count_nonzero(o["flags"][band] & po.PX_SHADED)) * len(lights)
-        same = all(np.array_equal(g[k][band], o[k][band]) for k in ("hit", "leaf", "shadow", "flags", "isp", "rgba8"))
-        log(f"CPU oracle: rows [{r0},{r1}) in {tc:.2f}s on {nthreads} threads, bit-exact vs GPU: {same}")
+        tc, cpu_rays, frames, same = 0.0, 0, 0, True
+        for i in range(Wm, K + Wm):
+            ctx.render(views[i], lights, mask | q.OUT_FLAGS | q.OUT_ISP)
+            g = ctx.read_frame(0)
+            ts = time.time()
+            o = osc.render(pos[i], ang[i], W, H, lights, nthreads=nthreads)
+            tc += time.time() - ts
+            cpu_rays += W * H + int(np.count_nonzero(o["flags"] & po.PX_SHADED)) * len(lights)
+            frames += 1
+            same = same and all(np.array_equal(g[k], o[k]) for k in ("hit", "leaf", "shadow", "flags", "isp",
+                                                                    "rgba8"))
+            if tc >= args.cpu_seconds:
+                break
+        log(f"CPU oracle: {frames} frames in {tc:.2f}s on {nthreads} threads, bit-exact vs GPU: {same}")
         if not same:
-            raise SystemExit("CPU oracle and GPU frame differ on the sampled band")
+            raise SystemExit("CPU oracle and GPU frames differ")
         cpu = {"value": round(cpu_rays / tc / 1e6, 4), "unit": "Mrays/s", "cores": nthreads, "kind": "port",
-               "sample": f"rows {r0}-{r1} of pose 0 at {W}x{H} ({W * (r1 - r0)} primary + "
-                         f"{cpu_rays - W * (r1 - r0)} shadow rays), C oracle -O2 -ffp-contract=off, pthreads",
+               "sample": f"{frames} whole {W}x{H} frames of the timed camera path ({cpu_rays} rays), "
+                         f"C oracle -O2 -ffp-contract=off, pthreads",
                "seconds": round(tc, 2), "bit_exact_vs_gpu": same}
 
     res = {

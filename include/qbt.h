@@ -187,6 +187,12 @@ typedef struct qbt_profile {
 int qbt_profile_begin(qbt_ctx* ctx);
 int qbt_profile_end(qbt_ctx* ctx, qbt_profile* out);
 
+/* Diagnostic: copy `bytes` (a multiple of 16) between two scratch buffers
+ * with a plain streaming kernel (k_copy16) `reps` times and report the
+ * average launch time and (read + write) GB/s. Used to calibrate hardware
+ * byte counters and as the achievable-bandwidth reference. */
+int qbt_copy_bandwidth(qbt_ctx* ctx, size_t bytes, int reps, double* avg_ms, double* gbps);
+
 /* Batched octree_trace_line: n rays (origins, dirs: n x 3 floats, host).
  * semantics 0 = shader (octree_fsh.c cube_trace_line, dual tree), 1 = CPU
  * twin (octree.c:341-537: static tree only, its plane-test returns). Outputs

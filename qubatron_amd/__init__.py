@@ -320,6 +320,12 @@ class Context:
         check(self._d.qbt_profile_end(self._h, C.byref(p)), "qbt_profile_end")
         return {k: getattr(p, k) for k, _ in _ffi.Profile._fields_ if k != "reserved"}
 
+    def copy_bandwidth(self, nbytes=1 << 30, reps=10):
+        ms, gbps = C.c_double(0), C.c_double(0)
+        check(self._d.qbt_copy_bandwidth(self._h, int(nbytes), int(reps), C.byref(ms), C.byref(gbps)),
+              "qbt_copy_bandwidth")
+        return ms.value, gbps.value
+
     def trace_rays(self, origins, dirs, basecube, maxlevel, semantics=0):
         o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)
         d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)

@@ -161,6 +161,7 @@ DEV_SYMBOLS = [
     "qbt_create", "qbt_destroy", "qbt_strerror", "qbt_device_bytes", "qbt_upload", "qbt_download", "qbt_render",
     "qbt_update", "qbt_read_frame", "qbt_frame_device", "qbt_get_stats", "qbt_trace_rays", "qbt_build_octree",
     "qbt_version", "qbt_last_hip_error", "qbt_synchronize", "qbt_profile_begin", "qbt_profile_end",
+    "qbt_copy_bandwidth",
 ]
 
 
@@ -187,6 +188,7 @@ def _declare_dev(d):
     d.qbt_synchronize.argtypes = [vp]
     d.qbt_profile_begin.argtypes = [vp]
     d.qbt_profile_end.argtypes = [vp, C.POINTER(Profile)]
+    d.qbt_copy_bandwidth.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     d.qbt_version.restype = C.c_char_p
     d.qbt_last_hip_error.restype = C.c_char_p
 

@@ -108,6 +108,13 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
     if (i < n) p[i] = v;
 }
 
+/* plain streaming copy, grid-stride over 16-byte elements */
+__global__ void __launch_bounds__(256) k_copy16(const int4* __restrict__ src, int4* __restrict__ dst, size_t n)
+{
+    size_t stride = (size_t) gridDim.x * blockDim.x;
+    for (size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 /* ------------------------------------------------------------------------ */
 /* pixel program (octree_fsh.c:381-464)                                     */
 /* ------------------------------------------------------------------------ */
@@ -1014,6 +1021,35 @@ extern "C" int qbt_profile_end(qbt_ctx* c, qbt_profile* out)
     p.launches = c->prof_n;
     p.avg_ms   = c->prof_n ? p.total_ms / c->prof_n : 0.0;
     *out       = p;
+    return QBT_OK;
+}
+
+extern "C" int qbt_copy_bandwidth(qbt_ctx* c, size_t bytes, int reps, double* avg_ms, double* gbps)
+{
+    if (!c || bytes == 0 || bytes % 16 || reps <= 0 || reps > 1000) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    DevBuf a, b;
+    int    rc = c->dev_alloc(a, bytes);
+    if (!rc) rc = c->dev_alloc(b, bytes);
+    if (rc) {
+        c->dev_free(a);
+        c->dev_free(b);
+        return rc;
+    }
+    const size_t n = bytes / 16;
+    k_fill_u32<<<blocks_for(n * 4, 256), 256, 0, c->stream>>>((uint32_t*) a.p, n * 4, 1u);
+    k_copy16<<<4096, 256, 0, c->stream>>>((const int4*) a.p, (int4*) b.p, n); /* warm */
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    for (int r = 0; r < reps; r++) k_copy16<<<4096, 256, 0, c->stream>>>((const int4*) a.p, (int4*) b.p, n);
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    c->dev_free(a);
+    c->dev_free(b);
+    double avg = ms / reps;
+    if (avg_ms) *avg_ms = avg;
+    if (gbps) *gbps = 2.0 * (double) bytes / (avg * 1e-3) / 1e9;
     return QBT_OK;
 }
 
