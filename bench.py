@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--traffic", default=TRAFFIC_FILE)
     ap.add_argument("--summarize-pmc", nargs="*", default=None, metavar="DIR")
+    ap.add_argument("--path", type=int, default=None, help="render path (0 fused, 1 wavefront); default: library's")
+    ap.add_argument("--block", type=int, default=None, help="threads per block (64/128/256)")
+    ap.add_argument("--refill", type=int, default=None, help="idle lanes that trigger a wave refill (1..64)")
     return ap.parse_args()
 
 
@@ -125,6 +128,8 @@ def main():
     log(f"generated {sc.npoints} points in {time.time() - t0:.2f}s (config {args.config}, seed {args.seed})")
 
     ctx = q.Context(0)
+    ctx.set_tuning(path=args.path, block_threads=args.block, refill=args.refill)
+    tuning = ctx.tuning()
     copy_ms, copy_gbps = ctx.copy_bandwidth(COPY_BYTES, 5)
     log(f"copy kernel: {copy_gbps:.0f} GB/s (read+write) on {COPY_BYTES >> 20} MiB")
     t0 = time.time()
@@ -233,6 +238,7 @@ def main():
                    "rays_per_frame": round(total_rays / K), "expansions_per_frame": round(float(np.mean(exps)))},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "tuning": tuning,
         "build_s": round(t_build, 3),
         "total_s": round(time.time() - t_all, 1),
     }

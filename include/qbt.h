@@ -187,6 +187,22 @@ typedef struct qbt_profile {
 int qbt_profile_begin(qbt_ctx* ctx);
 int qbt_profile_end(qbt_ctx* ctx, qbt_profile* out);
 
+/* Launch parameters of the render path (what the autotuner chooses).
+ * path: 0 = one fused kernel per view (every lane runs its pixel's primary
+ * and shadow rays back to back), 1 = wavefront stages with persistent,
+ * lane-refilling traversal kernels. block_threads: 64..256 (0 = default 256;
+ * halved while the LDS stack exceeds 64 KiB). refill: idle lanes that make a
+ * wave fetch new rays, 1..64 (0 = 32). Results never depend on these. */
+typedef struct qbt_tuning {
+    int path;
+    int block_threads;
+    int refill;
+    int reserved[5];
+} qbt_tuning;
+
+int qbt_get_tuning(qbt_ctx* ctx, qbt_tuning* out);
+int qbt_set_tuning(qbt_ctx* ctx, const qbt_tuning* in);
+
 /* Diagnostic: copy `bytes` (a multiple of 16) between two scratch buffers
  * with a plain streaming kernel (k_copy16) `reps` times and report the
  * average launch time and (read + write) GB/s. Used to calibrate hardware

@@ -320,6 +320,22 @@ class Context:
         check(self._d.qbt_profile_end(self._h, C.byref(p)), "qbt_profile_end")
         return {k: getattr(p, k) for k, _ in _ffi.Profile._fields_ if k != "reserved"}
 
+    def tuning(self):
+        t = _ffi.Tuning()
+        check(self._d.qbt_get_tuning(self._h, C.byref(t)), "qbt_get_tuning")
+        return {"path": t.path, "block_threads": t.block_threads, "refill": t.refill}
+
+    def set_tuning(self, path=None, block_threads=None, refill=None):
+        t = _ffi.Tuning()
+        check(self._d.qbt_get_tuning(self._h, C.byref(t)), "qbt_get_tuning")
+        if path is not None:
+            t.path = int(path)
+        if block_threads is not None:
+            t.block_threads = int(block_threads)
+        if refill is not None:
+            t.refill = int(refill)
+        check(self._d.qbt_set_tuning(self._h, C.byref(t)), "qbt_set_tuning")
+
     def copy_bandwidth(self, nbytes=1 << 30, reps=10):
         ms, gbps = C.c_double(0), C.c_double(0)
         check(self._d.qbt_copy_bandwidth(self._h, int(nbytes), int(reps), C.byref(ms), C.byref(gbps)),

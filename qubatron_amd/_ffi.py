@@ -75,6 +75,10 @@ class Profile(C.Structure):
                 ("min_ms", C.c_double), ("max_ms", C.c_double)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("path", C.c_int), ("block_threads", C.c_int), ("refill", C.c_int), ("reserved", C.c_int * 5)]
+
+
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("trace_ms", C.c_double), ("primary_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("hits", C.c_uint64), ("expansions", C.c_uint64),
@@ -161,7 +165,7 @@ DEV_SYMBOLS = [
     "qbt_create", "qbt_destroy", "qbt_strerror", "qbt_device_bytes", "qbt_upload", "qbt_download", "qbt_render",
     "qbt_update", "qbt_read_frame", "qbt_frame_device", "qbt_get_stats", "qbt_trace_rays", "qbt_build_octree",
     "qbt_version", "qbt_last_hip_error", "qbt_synchronize", "qbt_profile_begin", "qbt_profile_end",
-    "qbt_copy_bandwidth",
+    "qbt_copy_bandwidth", "qbt_get_tuning", "qbt_set_tuning",
 ]
 
 
@@ -188,6 +192,8 @@ def _declare_dev(d):
     d.qbt_synchronize.argtypes = [vp]
     d.qbt_profile_begin.argtypes = [vp]
     d.qbt_profile_end.argtypes = [vp, C.POINTER(Profile)]
+    d.qbt_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    d.qbt_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
     d.qbt_copy_bandwidth.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     d.qbt_version.restype = C.c_char_p
     d.qbt_last_hip_error.restype = C.c_char_p

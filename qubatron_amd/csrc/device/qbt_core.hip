@@ -21,6 +21,7 @@
 #include "qbt.h"
 #include "qbt_internal.hpp"
 #include "trace.hpp"
+#include "pixel.hpp"
 
 #pragma clang fp contract(off)
 
@@ -118,92 +119,6 @@ __global__ void __launch_bounds__(256) k_copy16(const int4* __restrict__ src, in
 /* ------------------------------------------------------------------------ */
 /* pixel program (octree_fsh.c:381-464)                                     */
 /* ------------------------------------------------------------------------ */
-
-__device__ __forceinline__ void cross3(const float a[3], const float b[3], float r[3])
-{
-    r[0] = a[1] * b[2] - a[2] * b[1];
-    r[1] = a[2] * b[0] - a[0] * b[2];
-    r[2] = a[0] * b[1] - a[1] * b[0];
-}
-
-/* v + 2 * cross(q.xyz, cross(q.xyz, v) + q.w * v) (octree_fsh.c:392-395) */
-__device__ __forceinline__ void quat_rotate(const float q[4], const float v[3], float r[3])
-{
-    float c1[3], s[3], c2[3];
-    cross3(q, v, c1);
-    s[0] = c1[0] + q[3] * v[0];
-    s[1] = c1[1] + q[3] * v[1];
-    s[2] = c1[2] + q[3] * v[2];
-    cross3(q, s, c2);
-    r[0] = v[0] + 2.0f * c2[0];
-    r[1] = v[1] + 2.0f * c2[1];
-    r[2] = v[2] + 2.0f * c2[2];
-}
-
-__device__ __forceinline__ float dot3(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-
-__device__ __forceinline__ void normalize3(const float v[3], float r[3])
-{
-    float s = 1.0f / sqrtf(dot3(v, v));
-    r[0]    = v[0] * s;
-    r[1]    = v[1] * s;
-    r[2]    = v[2] * s;
-}
-
-__device__ __forceinline__ float gmax(float x, float y) { return x < y ? y : x; } /* GLSL max */
-
-__device__ __forceinline__ uint32_t pack_rgba8(float c0, float c1, float c2, float c3)
-{
-    float    c[4] = {c0, c1, c2, c3};
-    uint32_t r    = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        float x = c[k];
-        if (!(x > 0.0f)) x = 0.0f;
-        if (x > 1.0f) x = 1.0f;
-        r |= ((uint32_t) (x * 255.0f + 0.5f)) << (8 * k);
-    }
-    return r;
-}
-
-__device__ __forceinline__ void fetch_attr(const AttrDev& a, int m, float col[4], float nrm[4])
-{
-    if (m >= 0 && m < a.n) {
-        float4 c = a.ca[2 * (size_t) m], v = a.ca[2 * (size_t) m + 1];
-        col[0] = c.x, col[1] = c.y, col[2] = c.z, col[3] = 1.0f;
-        nrm[0] = v.x, nrm[1] = v.y, nrm[2] = v.z, nrm[3] = 1.0f;
-    } else {
-        col[0] = col[1] = col[2] = col[3] = 0.0f;
-        nrm[0] = nrm[1] = nrm[2] = nrm[3] = 0.0f;
-    }
-}
-
-__device__ __forceinline__ void pixel_dir(const ViewDev& v, int px, int py, float csv[3])
-{
-    float c0[3] = {((float) px + 0.5f) - v.W / 2.0f, ((float) py + 0.5f) - v.H / 2.0f, 0.0f - (v.W / 2.0f) / 1.0f};
-    float c1[3];
-    quat_rotate(v.qz, c0, c1);
-    quat_rotate(v.qx, c1, csv);
-}
-
-__device__ __forceinline__ unsigned wave_sum(unsigned x)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-    return x;
-}
-
-__device__ __forceinline__ Stack make_stack(uint32_t* lds, int L, int dual)
-{
-    Stack st;
-    int   nt  = blockDim.x;
-    int   tid = threadIdx.x;
-    st.stride = nt;
-    st.pw     = lds + tid;
-    st.ns     = reinterpret_cast<int*>(lds) + L * nt + tid;
-    st.nd     = dual ? reinterpret_cast<int*>(lds) + 2 * L * nt + tid : st.ns;
-    return st;
-}
 
 /* per-view, per-light sprite trace (octree_fsh.c:452-459): the ray from the
  * camera to the light is the same for every pixel of a view. */
@@ -461,6 +376,11 @@ extern "C" int qbt_create(const qbt_opts* opts, qbt_ctx** out)
     if (!c) return QBT_ENOMEM;
     c->device    = dev;
     c->max_bytes = opts ? opts->max_device_bytes : 0;
+    if (hipDeviceGetAttribute(&c->num_cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        c->num_cus <= 0) {
+        (void) hipGetLastError();
+        c->num_cus = 256;
+    }
     if (hipSetDevice(dev) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess) {
@@ -494,6 +414,9 @@ extern "C" void qbt_destroy(qbt_ctx* c)
     c->dev_free(c->counters);
     c->dev_free(c->sprite);
     c->dev_free(c->rays);
+    for (DevBuf* b : {&c->wb_isp, &c->wb_ms, &c->wb_md, &c->wb_leaf, &c->wb_flags, &c->wb_disc, &c->wb_lstat,
+                      &c->wb_exp, &c->wb_sel, &c->wb_items, &c->wb_q, &c->wb_tmp, &c->valflag})
+        c->dev_free(*b);
     c->mlp_free();
     c->builder_free();
     for (hipEvent_t e : c->prof_ev) (void) hipEventDestroy(e);
@@ -919,11 +842,16 @@ extern "C" int qbt_render(qbt_ctx* c, const qbt_view* views, int nviews, const f
         if ((rc = c->ensure_frame(i, v.w, v.h, mask))) return rc;
         int* spr = (int*) c->sprite.p + i * QBT_MAX_LIGHTS_RENDER;
         if ((rc = c->launch_sprite(s, dual, grid, v, spr))) return rc;
-        if ((rc = c->launch_render(s, dual, grid, count, v, spr, c->frame_dev(i), mask))) return rc;
+        if (c->path == 0) {
+            if ((rc = c->launch_render(s, dual, grid, count, v, spr, c->frame_dev(i), mask))) return rc;
+            c->last.launches += 1;
+        } else if ((rc = c->render_view_wavefront(s, dual, grid, count, v, spr, c->frame_dev(i), mask))) {
+            return rc;
+        }
         prim += (uint64_t) v.w * (uint64_t) v.h;
         c->last.grid_exact = grid;
         c->last.dual       = dual;
-        c->last.launches += 2;
+        c->last.launches += 1;
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     c->last.primary_rays = prim;
@@ -1021,6 +949,29 @@ extern "C" int qbt_profile_end(qbt_ctx* c, qbt_profile* out)
     p.launches = c->prof_n;
     p.avg_ms   = c->prof_n ? p.total_ms / c->prof_n : 0.0;
     *out       = p;
+    return QBT_OK;
+}
+
+extern "C" int qbt_get_tuning(qbt_ctx* c, qbt_tuning* out)
+{
+    if (!c || !out) return QBT_EINVAL;
+    memset(out, 0, sizeof(*out));
+    out->path          = c->path;
+    out->block_threads = c->tune_block;
+    out->refill        = c->tune_refill;
+    return QBT_OK;
+}
+
+extern "C" int qbt_set_tuning(qbt_ctx* c, const qbt_tuning* in)
+{
+    if (!c || !in) return QBT_EINVAL;
+    if (in->path < 0 || in->path > 1) return QBT_EINVAL;
+    if (in->block_threads != 0 && in->block_threads != 64 && in->block_threads != 128 && in->block_threads != 256)
+        return QBT_EINVAL;
+    if (in->refill < 0 || in->refill > 64) return QBT_EINVAL;
+    c->path        = in->path;
+    c->tune_block  = in->block_threads;
+    c->tune_refill = in->refill;
     return QBT_OK;
 }
 

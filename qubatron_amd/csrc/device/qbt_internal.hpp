@@ -92,6 +92,11 @@ struct qbt_ctx {
     uint64_t     aborted         = 0;
     int          iter_cap        = 1 << 22;
     int          tune_block      = 0;
+    int          tune_refill     = 0;  /* idle lanes that trigger a wave refill (1..64; 0 = 32) */
+    int          path            = 1;  /* 0 = fused per-pixel kernel, 1 = wavefront stages */
+    int          num_cus         = 256;
+    /* wavefront render buffers */
+    DevBuf       wb_isp, wb_ms, wb_md, wb_leaf, wb_flags, wb_disc, wb_lstat, wb_exp, wb_sel, wb_items, wb_q, wb_tmp;
     MlpState*    mlp             = nullptr;
     BuildState*  build           = nullptr;
     /* qbt_profile_begin/end: event pairs around traversal-kernel launches */
@@ -129,6 +134,9 @@ struct qbt_ctx {
                            FrameDev f, unsigned mask);
     int      launch_sprite(const qbt::SceneDev& s, bool dual, bool grid, const ViewDev& v, int* spr);
     int      render_surrogate(const qbt_view* views, int nviews, const float* lights, int nlights, unsigned out_mask);
+    int      ensure_wave_bufs(size_t npix, int nlights);
+    int      render_view_wavefront(const qbt::SceneDev& s, bool dual, bool grid, bool count, const ViewDev& v,
+                                   const int* spr, FrameDev f, unsigned mask);
     void     mlp_free();
     void     builder_free();
 };

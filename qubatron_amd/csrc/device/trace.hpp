@@ -210,16 +210,24 @@ __device__ __forceinline__ float4 entry_of(const SceneDev& s, const Ray& r, cons
     return start;
 }
 
-/*
- * Trace one ray. Returns ST_HIT / ST_MISS / ST_DISCARD / ST_ABORT.
- * DUAL: the dynamic tree takes part. CPU: octree.c plane-test returns and
- * root acceptance (single tree; callers pass DUAL = false).
- */
-template <bool DUAL, bool CPU, bool GRID>
-__device__ int trace(const SceneDev& s, const Ray& r, const Stack& st, Hit& out)
+/* Resumable traversal state of one ray (registers; the per-level pending
+ * words and node indices live in the LDS Stack). */
+struct TraceState {
+    float4   entry; /* entry point of the current node */
+    float4   start; /* root entry point */
+    int      level, ix, iy, iz;
+    int      ns, nd; /* current static / dynamic node */
+    int      it;
+    unsigned exps;
+};
+
+enum : int { ST_CONT = 2 };
+
+/* Root entry (octree_fsh.c:157-211). Returns ST_CONT, ST_HIT (maxlevel 0)
+ * or ST_DISCARD. */
+template <bool CPU>
+__device__ __forceinline__ int trace_begin(const SceneDev& s, const Ray& r, TraceState& t)
 {
-    out.exps = 0;
-    /* root entry (octree_fsh.c:157-211) */
     float4 h0 = make_float4(0.f, 0.f, 0.f, 0.f), h1 = h0;
     int    hitc = 0;
     {
@@ -246,55 +254,58 @@ __device__ int trace(const SceneDev& s, const Ray& r, const Stack& st, Hit& out)
         QBT_CONSIDER(tx < a.x && a.x <= rx && tz > a.z && a.z >= rz)
 #undef QBT_CONSIDER
     }
+    t.exps = 0;
     if (hitc < 2) return ST_DISCARD;
     if (h0.w < 0.0f && h1.w < 0.0f) return ST_DISCARD;
     if (h1.w < h0.w) h0 = h1;
     if (h0.w < 0.0f) h0 = make_float4(r.px, r.py, r.pz, 0.0f);
-    const float4 start = h0;
+    t.start = h0;
+    t.entry = h0;
+    t.level = t.ix = t.iy = t.iz = 0;
+    t.ns = t.nd = 0;
+    t.it = 0;
+    return s.maxlevel == 0 ? ST_HIT : ST_CONT;
+}
 
-    int    level = 0, ix = 0, iy = 0, iz = 0;
-    int    ns = 0, nd = 0;
-    float4 entry = start;
-    const int L = s.maxlevel;
+/*
+ * One step: expand the node just entered and descend into its nearest
+ * pending child, or backtrack and descend into the next pending child of an
+ * ancestor. Returns ST_CONT, ST_HIT (t.level == maxlevel; the leaf is
+ * t.ns / t.nd with entry point t.entry), ST_MISS or ST_ABORT.
+ */
+template <bool DUAL, bool CPU, bool GRID>
+__device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const Stack& st, TraceState& t)
+{
+    if (++t.it > s.iter_cap) return ST_ABORT;
+    const int level = t.level;
 
-    for (int it = 0;; ++it) {
-        if (level == L) {
-            out.isp = entry;
-            out.ns  = ns;
-            out.nd  = nd;
-            out.ix  = ix;
-            out.iy  = iy;
-            out.iz  = iz;
-            return ST_HIT;
-        }
-        if (it > s.iter_cap) return ST_ABORT;
+    /* ---- expansion of the node just entered (octree_fsh.c:251-330) ---- */
+    int4 sl = make_int4(0, 0, 0, 0), sh = sl, dl = sl, dh = sl;
+    if (t.ns != 0 || level == 0) {
+        const int4* p = s.stat.child + 2 * (size_t) t.ns;
+        sl            = p[0];
+        sh            = p[1];
+        t.exps++;
+    }
+    if (DUAL && (t.nd != 0 || level == 0)) {
+        const int4* p = s.dyn.child + 2 * (size_t) t.nd;
+        dl            = p[0];
+        dh            = p[1];
+        t.exps++;
+    }
+    unsigned mask = exist_mask(sl, sh);
+    if (DUAL) mask |= exist_mask(dl, dh);
 
-        /* ---- expansion of the node just entered (octree_fsh.c:251-330) ---- */
-        int4 sl = make_int4(0, 0, 0, 0), sh = sl, dl = sl, dh = sl;
-        if (ns != 0 || level == 0) {
-            const int4* p = s.stat.child + 2 * (size_t) ns;
-            sl            = p[0];
-            sh            = p[1];
-            out.exps++;
-        }
-        if (DUAL && (nd != 0 || level == 0)) {
-            const int4* p = s.dyn.child + 2 * (size_t) nd;
-            dl            = p[0];
-            dh            = p[1];
-            out.exps++;
-        }
-        unsigned mask = exist_mask(sl, sh);
-        if (DUAL) mask |= exist_mask(dl, dh);
-
+    if (mask) {
         float tx, ty, tz, tw;
-        cube_at<GRID>(s, level, ix, iy, iz, tx, ty, tz, tw);
+        cube_at<GRID>(s, level, t.ix, t.iy, t.iz, tx, ty, tz, tw);
         const float bxx = tx + tw, byy = ty - tw, bzz = tz - tw;
         const float hx = bxx + (tx - bxx) * 0.5f;
         const float hy = byy + (ty - byy) * 0.5f;
         const float hz = bzz + (tz - bzz) * 0.5f;
 
         const float qnan = __builtin_nanf("");
-        Cand        c0 = {entry.x, entry.y, entry.z, entry.w, 0};
+        Cand        c0 = {t.entry.x, t.entry.y, t.entry.z, t.entry.w, 0};
         Cand        c1, c2, c3;
         {
             float4 a = plane_z<CPU>(hz, r);
@@ -321,7 +332,7 @@ __device__ int trace(const SceneDev& s, const Ray& r, const Stack& st, Hit& out)
 
         int      pre = -1, n = 0, foct = 0;
         uint32_t word = 0;
-        float4   first = entry;
+        float4   first = t.entry;
 #define QBT_CLASSIFY(c)                                                                     \
     if ((c).w == (c).w) {                                                                   \
         int o = ((c).x > hx ? 1 : 0) | ((c).y < hy ? 2 : 0) | ((c).z < hz ? 4 : 0);         \
@@ -349,56 +360,74 @@ __device__ int trace(const SceneDev& s, const Ray& r, const Stack& st, Hit& out)
         if (n > 0) {
             /* pop the first pending entry now (octree_fsh.c:336-367) */
             st.pw[level * st.stride] = word | (uint32_t) n | (1u << 3);
-            st.ns[level * st.stride] = ns;
-            if (DUAL) st.nd[level * st.stride] = nd;
-            int cs = pick8(sl, sh, foct);
-            int cd = DUAL ? pick8(dl, dh, foct) : 0;
-            ix     = (ix << 1) | (foct & 1);
-            iy     = (iy << 1) | ((foct >> 1) & 1);
-            iz     = (iz << 1) | (foct >> 2);
-            ns     = cs;
-            nd     = cd;
-            entry  = first;
-            level++;
-            continue;
-        }
-
-        /* ---- backtrack (octree_fsh.c:368-375) and pop the next pending ---- */
-        for (;;) {
-            level--;
-            if (level < 0) return ST_MISS;
-            ix >>= 1;
-            iy >>= 1;
-            iz >>= 1;
-            uint32_t w = st.pw[level * st.stride];
-            int      k = (int) ((w >> 3) & 7u), cnt = (int) (w & 7u);
-            if (k < cnt) {
-                int e   = (int) ((w >> (8 + 5 * k)) & 31u);
-                int o   = e & 7, src = e >> 3;
-                st.pw[level * st.stride] = (w & ~(7u << 3)) | ((uint32_t) (k + 1) << 3);
-                float4 isp;
-                if (src == 0) {
-                    isp = entry_of<CPU, GRID>(s, r, st, level, ix, iy, iz, start);
-                } else {
-                    float x, y, z, ww;
-                    cube_at<GRID>(s, level, ix, iy, iz, x, y, z, ww);
-                    isp = midplane<CPU>(src, r, x, y, z, ww);
-                }
-                int ps = st.ns[level * st.stride];
-                int cs = child_slot(s.stat, ps, o, level);
-                int cd = 0;
-                if (DUAL) cd = child_slot(s.dyn, st.nd[level * st.stride], o, level);
-                ix    = (ix << 1) | (o & 1);
-                iy    = (iy << 1) | ((o >> 1) & 1);
-                iz    = (iz << 1) | (o >> 2);
-                ns    = cs;
-                nd    = cd;
-                entry = isp;
-                level++;
-                break;
-            }
+            st.ns[level * st.stride] = t.ns;
+            if (DUAL) st.nd[level * st.stride] = t.nd;
+            t.ns    = pick8(sl, sh, foct);
+            t.nd    = DUAL ? pick8(dl, dh, foct) : 0;
+            t.ix    = (t.ix << 1) | (foct & 1);
+            t.iy    = (t.iy << 1) | ((foct >> 1) & 1);
+            t.iz    = (t.iz << 1) | (foct >> 2);
+            t.entry = first;
+            t.level = level + 1;
+            return t.level == s.maxlevel ? ST_HIT : ST_CONT;
         }
     }
+
+    /* ---- backtrack (octree_fsh.c:368-375) and pop the next pending ---- */
+    int lv = level, ix = t.ix, iy = t.iy, iz = t.iz;
+    for (;;) {
+        lv--;
+        if (lv < 0) return ST_MISS;
+        ix >>= 1;
+        iy >>= 1;
+        iz >>= 1;
+        uint32_t w = st.pw[lv * st.stride];
+        int      k = (int) ((w >> 3) & 7u), cnt = (int) (w & 7u);
+        if (k < cnt) {
+            int e   = (int) ((w >> (8 + 5 * k)) & 31u);
+            int o   = e & 7, src = e >> 3;
+            st.pw[lv * st.stride] = (w & ~(7u << 3)) | ((uint32_t) (k + 1) << 3);
+            float4 isp;
+            if (src == 0) {
+                isp = entry_of<CPU, GRID>(s, r, st, lv, ix, iy, iz, t.start);
+            } else {
+                float x, y, z, ww;
+                cube_at<GRID>(s, lv, ix, iy, iz, x, y, z, ww);
+                isp = midplane<CPU>(src, r, x, y, z, ww);
+            }
+            t.ns = child_slot(s.stat, st.ns[lv * st.stride], o, lv);
+            t.nd = DUAL ? child_slot(s.dyn, st.nd[lv * st.stride], o, lv) : 0;
+            t.ix    = (ix << 1) | (o & 1);
+            t.iy    = (iy << 1) | ((o >> 1) & 1);
+            t.iz    = (iz << 1) | (o >> 2);
+            t.entry = isp;
+            t.level = lv + 1;
+            return t.level == s.maxlevel ? ST_HIT : ST_CONT;
+        }
+    }
+}
+
+/*
+ * Trace one ray to the end. Returns ST_HIT / ST_MISS / ST_DISCARD / ST_ABORT.
+ * DUAL: the dynamic tree takes part. CPU: octree.c plane-test returns and
+ * root acceptance (single tree; callers pass DUAL = false).
+ */
+template <bool DUAL, bool CPU, bool GRID>
+__device__ int trace(const SceneDev& s, const Ray& r, const Stack& st, Hit& out)
+{
+    TraceState t;
+    int        stt = trace_begin<CPU>(s, r, t);
+    while (stt == ST_CONT) stt = trace_step<DUAL, CPU, GRID>(s, r, st, t);
+    out.exps = t.exps;
+    if (stt == ST_HIT) {
+        out.isp = t.entry;
+        out.ns  = t.ns;
+        out.nd  = t.nd;
+        out.ix  = t.ix;
+        out.iy  = t.iy;
+        out.iz  = t.iz;
+    }
+    return stt;
 }
 
 } // namespace qbt

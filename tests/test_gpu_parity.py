@@ -19,6 +19,14 @@ pytestmark = pytest.mark.gpu
 ALL = q.OUT_ALL | q.RENDER_COUNT
 
 
+@pytest.fixture(params=[1, 0], ids=["wavefront", "fused"])
+def ctx(gpu_ctx, request):
+    """Both render paths: wavefront stages (default) and the fused kernel."""
+    gpu_ctx.set_tuning(path=request.param)
+    yield gpu_ctx
+    gpu_ctx.set_tuning(path=1)
+
+
 def _render_both(ctx, osc, camfp, angle, W, H_, lights, basecube, maxlevel, shoot=0):
     v = q.view(camfp, angle, W, H_, basecube, maxlevel, shoot)
     print(f"[case] render cam={tuple(np.round(camfp, 2))} ang={tuple(np.round(angle, 3))} {W}x{H_} L={maxlevel}",
@@ -44,15 +52,15 @@ def _upload_scene(ctx, nodes, col, nrm, dyn_nodes=None, dyn_col=None, dyn_nrm=No
         ctx.upload_model(dyn_col, dyn_nrm, dynamic=True)
 
 
-def test_config1_reference_view(gpu_ctx):
+def test_config1_reference_view(ctx):
     """Config 1: 1M points, depth 10, 320x240, camera at the centre looking +z."""
     sc, pnt, nrm, col, nodes = H.scene(1)
-    _upload_scene(gpu_ctx, nodes, col, nrm)
+    _upload_scene(ctx, nodes, col, nrm)
     osc = H.oracle_scene(1)
-    g, o = _render_both(gpu_ctx, osc, (900.0, 900.0, 900.0), (np.pi, 0.0), 320, 240, sc.lights, sc.basecube,
+    g, o = _render_both(ctx, osc, (900.0, 900.0, 900.0), (np.pi, 0.0), 320, 240, sc.lights, sc.basecube,
                         sc.depth)
     _assert_same(g, o, "config1")
-    st = gpu_ctx.stats()
+    st = ctx.stats()
     assert st["primary_rays"] == 320 * 240
     assert st["hits"] == int(np.count_nonzero(o["flags"] & po.PX_SHADED))
     assert st["expansions"] == int(o["exp"].sum())
@@ -61,34 +69,34 @@ def test_config1_reference_view(gpu_ctx):
 
 
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_config1_random_poses(gpu_ctx, seed):
+def test_config1_random_poses(ctx, seed):
     sc, pnt, nrm, col, nodes = H.scene(1)
-    _upload_scene(gpu_ctx, nodes, col, nrm)
+    _upload_scene(ctx, nodes, col, nrm)
     osc = H.oracle_scene(1)
     pos, ang = sc.poses(2, seed=seed)
     for i in range(2):
-        g, o = _render_both(gpu_ctx, osc, pos[i], ang[i], 160, 120, sc.lights, sc.basecube, sc.depth,
+        g, o = _render_both(ctx, osc, pos[i], ang[i], 160, 120, sc.lights, sc.basecube, sc.depth,
                             shoot=i)
         _assert_same(g, o, f"pose {seed}/{i}")
 
 
-def test_multi_light_and_views(gpu_ctx):
+def test_multi_light_and_views(ctx):
     """Two views in one call, four lights (multi-light extension)."""
     sc, pnt, nrm, col, nodes = H.scene(1)
-    _upload_scene(gpu_ctx, nodes, col, nrm)
+    _upload_scene(ctx, nodes, col, nrm)
     osc = H.oracle_scene(1)
     lights = np.array([[900, 1700, 900], [300, 1500, 1400], [1500, 1600, 300], [1450, 900, 1500]], np.float32)
     pos, ang = sc.poses(2, seed=11)
     views = [q.view(pos[i], ang[i], 96 + 32 * i, 64 + 16 * i, sc.basecube, sc.depth) for i in range(2)]
     print("[case] two views, four lights", flush=True)
-    gpu_ctx.render(views, lights, ALL)
+    ctx.render(views, lights, ALL)
     for i in range(2):
-        g = gpu_ctx.read_frame(i)
+        g = ctx.read_frame(i)
         o = osc.render(pos[i], ang[i], 96 + 32 * i, 64 + 16 * i, lights)
         _assert_same(g, o, f"view {i}")
 
 
-def test_non_grid_basecube(gpu_ctx):
+def test_non_grid_basecube(ctx):
     """A basecube whose subdivision is not exact in fp32: the running-sum cube
     path replays the reference's additions."""
     sc, pnt, nrm, col, nodes = H.scene(1, npoints=200000)
@@ -98,15 +106,15 @@ def test_non_grid_basecube(gpu_ctx):
     p2 = np.clip(p2, lo + 0.01, lo + bc[3] - 0.01).astype(np.float32)
     tree = q.octree_from_points(p2, bc, 9)
     nodes2 = tree.nodes()
-    _upload_scene(gpu_ctx, nodes2, col, nrm)
+    _upload_scene(ctx, nodes2, col, nrm)
     osc = po.OracleScene(nodes2, col, nrm, bc, 9)
     cam = np.array([500.0, 500.0, 500.0], np.float32)
-    g, o = _render_both(gpu_ctx, osc, cam, (2.0, 0.3), 128, 96, [[500.0, 950.0, 500.0]], bc, 9)
-    assert gpu_ctx.stats()["grid_exact"] == 0
+    g, o = _render_both(ctx, osc, cam, (2.0, 0.3), 128, 96, [[500.0, 950.0, 500.0]], bc, 9)
+    assert ctx.stats()["grid_exact"] == 0
     _assert_same(g, o, "non-grid")
 
 
-def test_dual_tree(gpu_ctx):
+def test_dual_tree(ctx):
     """Static scene + a populated dynamic tree traversed in lockstep
     (octree_fsh.c:223-241, 313-317, 355-356)."""
     sc, pnt, nrm, col, nodes = H.scene(1, npoints=300000)
@@ -119,16 +127,16 @@ def test_dual_tree(gpu_ctx):
     dc = rng.uniform(0, 1, (n_d, 3)).astype(np.float32)
     dtree = q.octree_from_points(dp, sc.basecube, sc.depth)
     dnodes = dtree.nodes()
-    _upload_scene(gpu_ctx, nodes, col, nrm, dnodes, dc, dn)
+    _upload_scene(ctx, nodes, col, nrm, dnodes, dc, dn)
     osc = po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth, dyn_nodes=dnodes, dyn_col=dc, dyn_nrm=dn)
-    g, o = _render_both(gpu_ctx, osc, (900.0, 900.0, 700.0), (np.pi, 0.1), 200, 150, sc.lights, sc.basecube,
+    g, o = _render_both(ctx, osc, (900.0, 900.0, 700.0), (np.pi, 0.1), 200, 150, sc.lights, sc.basecube,
                         sc.depth)
-    assert gpu_ctx.stats()["dual"] == 1
+    assert ctx.stats()["dual"] == 1
     _assert_same(g, o, "dual")
     assert np.count_nonzero(o["dyn"] > 0) > 100  # dynamic points are visible
 
 
-def test_camera_outside_and_discard(gpu_ctx):
+def test_camera_outside_and_discard(ctx):
     """Camera outside the cube (rays entering through faces, discards at the
     root, OCTTEST camera of qubatron.c:140) on the 5-point OCTTEST scene
     (modelutil.c:91-110) at depth 12."""
@@ -138,42 +146,42 @@ def test_camera_outside_and_discard(gpu_ctx):
     bc = np.array([0.0, 1800.0, 1800.0, 1800.0], np.float32)
     tree = q.octree_from_points(pts, bc, 12)
     nodes = tree.nodes()
-    _upload_scene(gpu_ctx, nodes, col, nrm)
+    _upload_scene(ctx, nodes, col, nrm)
     osc = po.OracleScene(nodes, col, nrm, bc, 12)
     for cam, ang in [((900.0, 900.0, 3000.0), (0.0, 0.0)), ((-500.0, 2500.0, 900.0), (1.2, -0.5)),
                      ((10.2, 690.2, 2000.0), (0.0, 0.0))]:
-        g, o = _render_both(gpu_ctx, osc, cam, ang, 128, 128, [[420.0, 200.0, 680.0]], bc, 12)
+        g, o = _render_both(ctx, osc, cam, ang, 128, 128, [[420.0, 200.0, 680.0]], bc, 12)
         _assert_same(g, o, f"outside {cam}")
 
 
 @pytest.mark.parametrize("maxlevel", [0, 1, 2, 5])
-def test_shallow_trees(gpu_ctx, maxlevel):
+def test_shallow_trees(ctx, maxlevel):
     sc, pnt, nrm, col, nodes = H.scene(1, npoints=5000)
     tree = q.octree_from_points(pnt, sc.basecube, maxlevel)
     nodes = tree.nodes()
-    _upload_scene(gpu_ctx, nodes, col, nrm)
+    _upload_scene(ctx, nodes, col, nrm)
     osc = po.OracleScene(nodes, col, nrm, sc.basecube, maxlevel)
-    g, o = _render_both(gpu_ctx, osc, (900.0, 900.0, 900.0), (0.7, 0.2), 64, 48, sc.lights, sc.basecube, maxlevel)
+    g, o = _render_both(ctx, osc, (900.0, 900.0, 900.0), (0.7, 0.2), 64, 48, sc.lights, sc.basecube, maxlevel)
     _assert_same(g, o, f"maxlevel {maxlevel}")
 
 
-def test_empty_tree(gpu_ctx):
+def test_empty_tree(ctx):
     col = np.zeros((1, 3), np.float32)
     nodes = np.zeros((1, 12), np.int32)
-    _upload_scene(gpu_ctx, nodes, col, col)
+    _upload_scene(ctx, nodes, col, col)
     osc = po.OracleScene(nodes, col, col, (0, 1800, 1800, 1800), 10)
-    g, o = _render_both(gpu_ctx, osc, (900.0, 900.0, 900.0), (0.0, 0.0), 32, 32, [[900, 1700, 900]],
+    g, o = _render_both(ctx, osc, (900.0, 900.0, 900.0), (0.0, 0.0), 32, 32, [[900, 1700, 900]],
                         (0, 1800, 1800, 1800), 10)
     _assert_same(g, o, "empty")
     assert np.all(g["hit"] == -1)
 
 
 @pytest.mark.parametrize("semantics", [0, 1])
-def test_trace_rays(gpu_ctx, semantics):
+def test_trace_rays(ctx, semantics):
     """Batched octree_trace_line: random rays, plus axis-aligned rays whose
     zero direction components exercise the two plane-test conventions."""
     sc, pnt, nrm, col, nodes = H.scene(1)
-    _upload_scene(gpu_ctx, nodes, col, nrm)
+    _upload_scene(ctx, nodes, col, nrm)
     osc = H.oracle_scene(1)
     rng = np.random.default_rng(7 + semantics)
     n = 20000
@@ -188,7 +196,7 @@ def test_trace_rays(gpu_ctx, semantics):
     org[k // 2:k // 2 + 200, 1] = rng.uniform(0, 5, 200)
     org[k // 2:k // 2 + 200, 2] = rng.uniform(0, 5, 200)
     print(f"[case] trace_rays semantics={semantics} n={n}", flush=True)
-    g = gpu_ctx.trace_rays(org, dirs, sc.basecube, sc.depth, semantics)
+    g = ctx.trace_rays(org, dirs, sc.basecube, sc.depth, semantics)
     o = osc.trace_many(org, dirs, semantics)
     for key in ("status", "model", "leaf"):
         bad = np.count_nonzero(g[key] != o[key])
@@ -198,7 +206,7 @@ def test_trace_rays(gpu_ctx, semantics):
     assert np.count_nonzero(o["status"] == 1) > 1000
 
 
-def test_range_updates(gpu_ctx):
+def test_range_updates(ctx):
     """In-place zeroing and appended octets through partial uploads
     (modelutil.c:425-500): the device tree equals the edited host tree."""
     sc, pnt, nrm, col, _ = H.scene(1, npoints=300000)
@@ -208,16 +216,16 @@ def test_range_updates(gpu_ctx):
     big = np.zeros((cap, 12), np.int32)
     big[:tree.len] = nodes
     # first upload with a generous size, then range updates as the reference does
-    gpu_ctx.upload(big, q._ffi.TYPE_INT, tree.len * 48, 16, 0, tree.len * 48, q._ffi.BUFFER_STATIC_OCTREE)
-    gpu_ctx.upload_model(col, nrm)
-    gpu_ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
+    ctx.upload(big, q._ffi.TYPE_INT, tree.len * 48, 16, 0, tree.len * 48, q._ffi.BUFFER_STATIC_OCTREE)
+    ctx.upload_model(col, nrm)
+    ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
     rng = np.random.default_rng(3)
     victims = pnt[rng.choice(pnt.shape[0], 3000, replace=False)]
     for p in victims:  # punch: zero slots, upload the one modified node
         m, o = tree.remove_point(p)
         if o >= 0:
             nv = tree.nodes_view()
-            gpu_ctx.upload(nv, q._ffi.TYPE_INT, tree.len * 48, 16, o * 48, (o + 1) * 48,
+            ctx.upload(nv, q._ffi.TYPE_INT, tree.len * 48, 16, o * 48, (o + 1) * 48,
                            q._ffi.BUFFER_STATIC_OCTREE)
     newp = (victims[:2000] + rng.normal(0, 3, (2000, 3))).astype(np.float32).clip(1, 1799)
     for i, p in enumerate(newp):  # re-insert moved points: upload every touched node
@@ -225,25 +233,25 @@ def test_range_updates(gpu_ctx):
         tree.insert_point(i, p, octindarr=oa)
         nv = tree.nodes_view()
         for o in oa[oa > 0]:
-            gpu_ctx.upload(nv, q._ffi.TYPE_INT, tree.len * 48, 16, int(o) * 48, (int(o) + 1) * 48,
+            ctx.upload(nv, q._ffi.TYPE_INT, tree.len * 48, 16, int(o) * 48, (int(o) + 1) * 48,
                            q._ffi.BUFFER_STATIC_OCTREE)
     final = tree.nodes()
-    back = gpu_ctx.download_octree(tree.len)
+    back = ctx.download_octree(tree.len)
     assert np.array_equal(back, final)
     osc = po.OracleScene(final, col, nrm, sc.basecube, sc.depth)
-    g, o = _render_both(gpu_ctx, osc, (900.0, 900.0, 900.0), (np.pi, 0.0), 160, 120, sc.lights, sc.basecube,
+    g, o = _render_both(ctx, osc, (900.0, 900.0, 900.0), (np.pi, 0.0), 160, 120, sc.lights, sc.basecube,
                         sc.depth)
     _assert_same(g, o, "after edits")
 
 
-def test_update_entry_point(gpu_ctx):
+def test_update_entry_point(ctx):
     """qbt_update derives light/basecube/dimensions like octree_glc_update."""
     sc, pnt, nrm, col, nodes = H.scene(1)
-    _upload_scene(gpu_ctx, nodes, col, nrm)
+    _upload_scene(ctx, nodes, col, nrm)
     lighta = 0.7
     print("[case] qbt_update", flush=True)
-    gpu_ctx.update(640.0, 480.0, (900.0, 900.0, 900.0), (np.pi, 0.1, 0.0), lighta, 6, sc.depth, 1800.0, 1)
-    g = gpu_ctx.read_frame(0)
+    ctx.update(640.0, 480.0, (900.0, 900.0, 900.0), (np.pi, 0.1, 0.0), lighta, 6, sc.depth, 1800.0, 1)
+    g = ctx.read_frame(0)
     libm = ctypes.CDLL("libm.so.6")
     libm.sinf.argtypes = [ctypes.c_float]
     libm.sinf.restype = ctypes.c_float
@@ -256,19 +264,19 @@ def test_update_entry_point(gpu_ctx):
     _assert_same(g, o, "update")
 
 
-def test_invalid_tree_rejected(gpu_ctx):
+def test_invalid_tree_rejected(ctx):
     nodes = np.zeros((3, 12), np.int32)
     nodes[0, 0] = 1
     nodes[1, 3] = 7  # out of range
     col = np.zeros((3, 3), np.float32)
-    _upload_scene(gpu_ctx, nodes, col, col)
+    _upload_scene(ctx, nodes, col, col)
     v = q.view((900, 900, 900), (0, 0), 16, 16, (0, 1800, 1800, 1800), 4)
     with pytest.raises(q.QbtError) as e:
-        gpu_ctx.render(v, [[900, 1700, 900]])
+        ctx.render(v, [[900, 1700, 900]])
     assert e.value.code == q._ffi.QBT_ETREE
 
 
-def test_upload_argument_checks(gpu_ctx):
+def test_upload_argument_checks(ctx):
     nodes = np.zeros((2, 12), np.int32)
     T = q._ffi
     for args in [(T.TYPE_FLOAT, 96, 16, 0, 96, T.BUFFER_STATIC_OCTREE),   # wrong type
@@ -279,5 +287,5 @@ def test_upload_argument_checks(gpu_ctx):
                  (T.TYPE_INT, 96, 16, 8, 96, T.BUFFER_STATIC_OCTREE),     # unaligned start
                  (T.TYPE_INT, 96, 16, 0, 96, 9)]:                         # bad buffer id
         with pytest.raises(q.QbtError) as e:
-            gpu_ctx.upload(nodes, *args)
+            ctx.upload(nodes, *args)
         assert e.value.code == T.QBT_EINVAL
