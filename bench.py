@@ -24,6 +24,7 @@ import csv
 import glob
 import json
 import os
+import re
 import sys
 import time
 
@@ -64,38 +65,68 @@ def parse():
     return ap.parse_args()
 
 
+TRAVERSAL_KERNELS = ("k_render", "k_primary", "k_shadow")
+
+
 def summarize_pmc(dirs, out_path):
-    """Average per-dispatch counters of the render kernel and of the copy
-    calibration kernel; scale byte counters by copy-kernel truth."""
+    """Average per-dispatch counters of each traversal kernel (non-counting
+    variants) and of the copy calibration kernel; byte counters are scaled by
+    the copy kernel's known traffic. The traversal kernels of one frame are
+    summed into a per-frame figure."""
     vals = {}
     for d in dirs:
         for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
             with open(f) as fh:
                 for row in csv.DictReader(fh):
                     name = row.get("Kernel_Name", "")
-                    kern = "render" if RENDER_KERNEL in name else "copy" if "k_copy16" in name else None
+                    kern = None
+                    if "k_copy16" in name:
+                        kern = "copy"
+                    m = re.search(r"\b(k_render|k_primary|k_shadow)<([^>]*)>", name)
+                    if m and m.group(2).split(",")[-1].strip() == "false":  # not the counting variant
+                        kern = m.group(1)
                     if kern is None:
                         continue
                     key = (kern, row["Counter_Name"])
                     vals.setdefault(key, {})[(f, row.get("Dispatch_Id"))] = float(row["Counter_Value"])
     avg = {k: float(np.mean(list(v.values()))) for k, v in vals.items()}
+    tot = {k: float(np.sum(list(v.values()))) for k, v in vals.items()}
     n = {k: len(v) for k, v in vals.items()}
-    res = {"kernel": RENDER_KERNEL, "copy_bytes": COPY_BYTES, "counters_kib_or_count": avg, "dispatches": n}
-    fr, wr = avg.get(("render", "FETCH_SIZE")), avg.get(("render", "WRITE_SIZE"))
+    res = {"copy_bytes": COPY_BYTES}
     fc, wc = avg.get(("copy", "FETCH_SIZE")), avg.get(("copy", "WRITE_SIZE"))
-    if fr is not None and fc:
-        ff = (COPY_BYTES / 1024.0) / fc
-        res["fetch_scale"] = ff
-        res["fetch_bytes_per_launch"] = fr * 1024.0 * ff
-    if wr is not None and wc:
-        wf = (COPY_BYTES / 1024.0) / wc
-        res["write_scale"] = wf
-        res["write_bytes_per_launch"] = wr * 1024.0 * wf
-    if "fetch_bytes_per_launch" in res and "write_bytes_per_launch" in res:
-        res["hbm_bytes_per_launch"] = round(res["fetch_bytes_per_launch"] + res["write_bytes_per_launch"])
-    hit, miss = avg.get(("render", "TCC_HIT_sum")), avg.get(("render", "TCC_MISS_sum"))
-    if hit is not None and miss is not None and hit + miss > 0:
-        res["l2_hit_rate"] = hit / (hit + miss)
+    ff = (COPY_BYTES / 1024.0) / fc if fc else None
+    wf = (COPY_BYTES / 1024.0) / wc if wc else None
+    res["fetch_scale"], res["write_scale"] = ff, wf
+    per_kernel = {}
+    for k in TRAVERSAL_KERNELS:
+        e = {}
+        if (k, "FETCH_SIZE") in avg and ff:
+            e["fetch_bytes_per_launch"] = avg[(k, "FETCH_SIZE")] * 1024.0 * ff
+            e["launches"] = n[(k, "FETCH_SIZE")]
+            e["fetch_bytes_total"] = tot[(k, "FETCH_SIZE")] * 1024.0 * ff
+        if (k, "WRITE_SIZE") in avg and wf:
+            e["write_bytes_per_launch"] = avg[(k, "WRITE_SIZE")] * 1024.0 * wf
+            e["write_bytes_total"] = tot[(k, "WRITE_SIZE")] * 1024.0 * wf
+        h, m = avg.get((k, "TCC_HIT_sum")), avg.get((k, "TCC_MISS_sum"))
+        if h is not None and m is not None and h + m > 0:
+            e["l2_hit_rate"] = h / (h + m)
+        if e:
+            per_kernel[k] = e
+    res["kernels"] = per_kernel
+    # per frame: all traversal launches of the timed frames
+    frames = None
+    for k, e in per_kernel.items():
+        if k in ("k_primary", "k_render") and "launches" in e:
+            frames = e["launches"]
+    if frames:
+        fb = sum(e.get("fetch_bytes_total", 0.0) for e in per_kernel.values()) / frames
+        wb = sum(e.get("write_bytes_total", 0.0) for e in per_kernel.values()) / frames
+        res["hbm_bytes_per_launch"] = round(fb + wb)
+        res["hbm_bytes_per_launch_note"] = "traversal kernels of one frame (one launch of the fused kernel, or " \
+                                           "the primary + per-light shadow launches of the wavefront path)"
+        hits = [e["l2_hit_rate"] for e in per_kernel.values() if "l2_hit_rate" in e]
+        if hits:
+            res["l2_hit_rate"] = float(np.mean(hits))
     res["counters_kib_or_count"] = {"%s:%s" % k: v for k, v in avg.items()}
     res["dispatches"] = {"%s:%s" % k: v for k, v in n.items()}
     os.makedirs(os.path.dirname(out_path), exist_ok=True)
