@@ -62,6 +62,7 @@ def parse():
     ap.add_argument("--path", type=int, default=None, help="render path (0 fused, 1 wavefront); default: library's")
     ap.add_argument("--block", type=int, default=None, help="threads per block (64/128/256)")
     ap.add_argument("--refill", type=int, default=None, help="idle lanes that trigger a wave refill (1..64)")
+    ap.add_argument("--layout", type=int, default=None, help="0 compact node layout, 1 direct")
     return ap.parse_args()
 
 
@@ -159,7 +160,7 @@ def main():
     log(f"generated {sc.npoints} points in {time.time() - t0:.2f}s (config {args.config}, seed {args.seed})")
 
     ctx = q.Context(0)
-    ctx.set_tuning(path=args.path, block_threads=args.block, refill=args.refill)
+    ctx.set_tuning(path=args.path, block_threads=args.block, refill=args.refill, layout=args.layout)
     tuning = ctx.tuning()
     copy_ms, copy_gbps = ctx.copy_bandwidth(COPY_BYTES, 5)
     log(f"copy kernel: {copy_gbps:.0f} GB/s (read+write) on {COPY_BYTES >> 20} MiB")
@@ -188,6 +189,7 @@ def main():
         fl = ctx.read_frame(0, fields=("flags",))["flags"]
         rays.append(st["primary_rays"] + st["shadow_rays"])
         exps.append(st["expansions"])
+        tuning["compact_layout_used"] = bool(st["compact"])
         leafhits.append(int(np.count_nonzero(fl & q.PX_LEAF)))
     log(f"counting pass: {np.mean(rays) / 1e6:.3f} Mrays/frame, {np.mean(exps) / 1e6:.2f} M expansions/frame")
 

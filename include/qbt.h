@@ -163,7 +163,7 @@ typedef struct qbt_stats {
     uint32_t launches;       /* kernel launches of the last call */
     uint32_t grid_exact;     /* the basecube grid was exact in fp32 */
     uint32_t dual;           /* the dynamic tree took part */
-    uint32_t reserved;
+    uint32_t compact;        /* the compact node layout was traversed */
 } qbt_stats;
 
 int qbt_get_stats(qbt_ctx* ctx, qbt_stats* out);
@@ -192,12 +192,16 @@ int qbt_profile_end(qbt_ctx* ctx, qbt_profile* out);
  * and shadow rays back to back), 1 = wavefront stages with persistent,
  * lane-refilling traversal kernels. block_threads: 64..256 (0 = default 256;
  * halved while the LDS stack exceeds 64 KiB). refill: idle lanes that make a
- * wave fetch new rays, 1..64 (0 = 32). Results never depend on these. */
+ * wave fetch new rays, 1..64 (0 = 32). layout: 0 = traverse the compact
+ * breadth-first node layout (8 bytes per node, children contiguous) whenever
+ * the node array is a proper tree, 1 = always the direct 32-byte child
+ * blocks. Results never depend on these. */
 typedef struct qbt_tuning {
     int path;
     int block_threads;
     int refill;
-    int reserved[5];
+    int layout;
+    int reserved[4];
 } qbt_tuning;
 
 int qbt_get_tuning(qbt_ctx* ctx, qbt_tuning* out);

@@ -323,11 +323,13 @@ class Context:
     def tuning(self):
         t = _ffi.Tuning()
         check(self._d.qbt_get_tuning(self._h, C.byref(t)), "qbt_get_tuning")
-        return {"path": t.path, "block_threads": t.block_threads, "refill": t.refill}
+        return {"path": t.path, "block_threads": t.block_threads, "refill": t.refill, "layout": t.layout}
 
-    def set_tuning(self, path=None, block_threads=None, refill=None):
+    def set_tuning(self, path=None, block_threads=None, refill=None, layout=None):
         t = _ffi.Tuning()
         check(self._d.qbt_get_tuning(self._h, C.byref(t)), "qbt_get_tuning")
+        if layout is not None:
+            t.layout = int(layout)
         if path is not None:
             t.path = int(path)
         if block_threads is not None:

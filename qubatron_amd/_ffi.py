@@ -76,14 +76,15 @@ class Profile(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("path", C.c_int), ("block_threads", C.c_int), ("refill", C.c_int), ("reserved", C.c_int * 5)]
+    _fields_ = [("path", C.c_int), ("block_threads", C.c_int), ("refill", C.c_int), ("layout", C.c_int),
+                ("reserved", C.c_int * 4)]
 
 
 class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("trace_ms", C.c_double), ("primary_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("hits", C.c_uint64), ("expansions", C.c_uint64),
                 ("predicted", C.c_uint64), ("launches", C.c_uint32), ("grid_exact", C.c_uint32),
-                ("dual", C.c_uint32), ("reserved", C.c_uint32)]
+                ("dual", C.c_uint32), ("compact", C.c_uint32)]
 
 
 BUFFER_STATIC_COLOR, BUFFER_STATIC_NORMAL, BUFFER_STATIC_OCTREE = 0, 1, 2

@@ -288,6 +288,7 @@ extern "C" int qbt_build_octree(qbt_ctx* c, int tree, const float basecube[4], i
     tr.n          = total;
     tr.validated  = false;
     tr.full_check = true;
+    tr.cmp_stale  = true;
     if (nodes_out) *nodes_out = total;
     return QBT_OK;
 }

@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(256) k_copy16(const int4* __restrict__ src, in
 
 /* per-view, per-light sprite trace (octree_fsh.c:452-459): the ray from the
  * camera to the light is the same for every pixel of a view. */
-template <bool DUAL, bool GRID>
+template <bool CMP, bool DUAL, bool GRID>
 __global__ void k_sprite(SceneDev s, ViewDev v, int* __restrict__ out)
 {
     extern __shared__ uint32_t lds[];
@@ -132,7 +132,7 @@ __global__ void k_sprite(SceneDev s, ViewDev v, int* __restrict__ out)
     Ray r = {v.cam[0], v.cam[1], v.cam[2], v.light[l][0] - v.cam[0], v.light[l][1] - v.cam[1],
              v.light[l][2] - v.cam[2]};
     Hit h;
-    int stt   = trace<DUAL, false, GRID>(s, r, st, h);
+    int stt   = trace<CMP, DUAL, false, GRID>(s, r, st, h);
     float ix  = stt == ST_HIT ? h.isp.x : 0.0f;
     float resv = ix - v.cam[0];
     int   f   = 0;
@@ -163,7 +163,7 @@ __global__ void __launch_bounds__(256) k_render(SceneDev s, ViewDev v, const int
         pixel_dir(v, px, py, csv);
         Ray r = {v.cam[0], v.cam[1], v.cam[2], csv[0], csv[1], csv[2]};
         Hit h;
-        int stt = trace<DUAL, false, GRID>(s, r, st, h);
+        int stt = trace<false, DUAL, false, GRID>(s, r, st, h); /* fused path: direct layout */
         if (COUNT) n_exp += h.exps;
 
         int32_t  hit = -1, leaf = -1, dyn = -1;
@@ -200,7 +200,7 @@ __global__ void __launch_bounds__(256) k_render(SceneDev s, ViewDev v, const int
                     Ray lr = {v.light[l][0], v.light[l][1], v.light[l][2], h.isp.x - v.light[l][0],
                               h.isp.y - v.light[l][1], h.isp.z - v.light[l][2]};
                     Hit lh;
-                    int ls = trace<DUAL, false, GRID>(s, lr, st, lh);
+                    int ls = trace<false, DUAL, false, GRID>(s, lr, st, lh);
                     n_shadow++;
                     if (COUNT) n_exp += lh.exps;
                     if (ls == ST_ABORT) n_abort++;
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(256) k_render(SceneDev s, ViewDev v, const int
 }
 
 /* batched octree_trace_line */
-template <bool DUAL, bool CPU, bool GRID>
+template <bool CMP, bool DUAL, bool CPU, bool GRID>
 __global__ void __launch_bounds__(256) k_trace_rays(SceneDev s, const float* __restrict__ org,
                                                      const float* __restrict__ dir, size_t n, int* __restrict__ status,
                                                      int* __restrict__ model, int* __restrict__ leaf,
@@ -289,14 +289,14 @@ __global__ void __launch_bounds__(256) k_trace_rays(SceneDev s, const float* __r
     if (i < n) {
         Ray r = {org[3 * i], org[3 * i + 1], org[3 * i + 2], dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
         Hit h;
-        int stt = trace<DUAL, CPU, GRID>(s, r, st, h);
+        int stt = trace<CMP, DUAL, CPU, GRID>(s, r, st, h);
         if (stt == ST_ABORT) ab = 1;
         int    m = -1, lf = -1;
         float4 ip = make_float4(0.f, 0.f, 0.f, 0.f), tc = ip;
         if (stt == ST_HIT) {
             const int L = s.maxlevel;
             m           = (h.ns == 0 && L > 0) ? 0 : s.stat.model[h.ns];
-            lf          = h.ns;
+            lf          = CMP ? s.stat.ref[h.ns] : h.ns;
             ip          = h.isp;
             float x, y, z, w;
             cube_at<GRID>(s, L, h.ix, h.iy, h.iz, x, y, z, w);
@@ -405,6 +405,9 @@ extern "C" void qbt_destroy(qbt_ctx* c)
     for (int t = 0; t < 2; t++) {
         c->dev_free(c->tree[t].child);
         c->dev_free(c->tree[t].model);
+        c->dev_free(c->tree[t].cmp.rec);
+        c->dev_free(c->tree[t].cmp.model);
+        c->dev_free(c->tree[t].cmp.ref);
         c->dev_free(c->attr[t].ca);
         c->dev_free(c->pts[t].xyz);
     }
@@ -497,6 +500,7 @@ extern "C" int qbt_upload(qbt_ctx* c, const void* data, int type, size_t size, s
             else tb.dirty_lo = lo < tb.dirty_lo ? lo : tb.dirty_lo, tb.dirty_hi = hi > tb.dirty_hi ? hi : tb.dirty_hi;
         }
         tb.validated = false;
+        tb.cmp_stale = true;
         const size_t per = kStageBytes / 16;
         for (size_t a = t0; a < t1; a += per) {
             size_t cnt = t1 - a < per ? t1 - a : per;
@@ -658,7 +662,8 @@ bool qbt_grid_exact(const float b[4], int L)
     return true;
 }
 
-int qbt_ctx::scene_for(const float basecube[4], int maxlevel, SceneDev& s, bool& dual, bool& grid)
+int qbt_ctx::scene_for(const float basecube[4], int maxlevel, SceneDev& s, bool& dual, bool& grid, bool& cmp,
+                       bool allow_compact)
 {
     if (maxlevel < 0 || maxlevel > kMaxLevel) return QBT_EINVAL;
     if (!(basecube[3] > 0.0f) || !isfinite(basecube[0]) || !isfinite(basecube[1]) || !isfinite(basecube[2]) ||
@@ -669,16 +674,33 @@ int qbt_ctx::scene_for(const float basecube[4], int maxlevel, SceneDev& s, bool&
     int rc;
     if ((rc = validate_tree(0))) return rc;
     if ((rc = validate_tree(1))) return rc;
+    dual = tree[1].n > 1 || (maxlevel == 0 && tree[1].n >= 1);
+    grid = qbt_grid_exact(basecube, maxlevel);
+    /* compact layout: rebuilt lazily after the node arrays change */
+    cmp = false;
+    if (allow_compact && use_compact) {
+        for (int t = 0; t < (dual ? 2 : 1); t++)
+            if (tree[t].cmp_stale) {
+                if ((rc = relayout(t))) return rc;
+                tree[t].cmp_stale = false;
+            }
+        cmp = tree[0].cmp.ok && (!dual || tree[1].cmp.ok);
+    }
     memset(&s, 0, sizeof(s));
-    s.stat  = {(const int4*) tree[0].child.p, (const int*) tree[0].model.p, (int) tree[0].n};
-    s.dyn   = {(const int4*) tree[1].child.p, (const int*) tree[1].model.p, (int) tree[1].n};
+    for (int t = 0; t < 2; t++) {
+        TreeDev& d = t ? s.dyn : s.stat;
+        if (cmp) {
+            const CompactTree& cm = tree[t].cmp;
+            d = {nullptr, (const int*) cm.model.p, (int) cm.n, (const uint2*) cm.rec.p, (const int*) cm.ref.p};
+        } else {
+            d = {(const int4*) tree[t].child.p, (const int*) tree[t].model.p, (int) tree[t].n, nullptr, nullptr};
+        }
+    }
     s.astat = {(const float4*) attr[0].ca.p, (int) attr[0].n};
     s.adyn  = {(const float4*) attr[1].ca.p, (int) attr[1].n};
     s.bx = basecube[0], s.by = basecube[1], s.bz = basecube[2], s.bw = basecube[3];
     s.maxlevel = maxlevel;
     s.iter_cap = iter_cap;
-    dual       = tree[1].n > 1 || (maxlevel == 0 && tree[1].n >= 1);
-    grid       = qbt_grid_exact(basecube, maxlevel);
     return QBT_OK;
 }
 
@@ -795,17 +817,14 @@ int qbt_ctx::launch_render(const SceneDev& s, bool dual, bool grid, bool count, 
     return QBT_OK;
 }
 
-int qbt_ctx::launch_sprite(const SceneDev& s, bool dual, bool grid, const ViewDev& v, int* spr)
+int qbt_ctx::launch_sprite(const SceneDev& s, bool cmp, bool dual, bool grid, const ViewDev& v, int* spr)
 {
     if (v.nlights <= 0) return QBT_OK;
     const size_t lds = qbt_stack_bytes(s.maxlevel, dual, 64);
-    if (dual) {
-        if (grid) k_sprite<true, true><<<1, 64, lds, stream>>>(s, v, spr);
-        else k_sprite<true, false><<<1, 64, lds, stream>>>(s, v, spr);
-    } else {
-        if (grid) k_sprite<false, true><<<1, 64, lds, stream>>>(s, v, spr);
-        else k_sprite<false, false><<<1, 64, lds, stream>>>(s, v, spr);
-    }
+    qbt_dispatch3(cmp, dual, grid, [&](auto C_, auto D_, auto G_) {
+        k_sprite<decltype(C_)::value, decltype(D_)::value, decltype(G_)::value><<<1, 64, lds, stream>>>(s, v, spr);
+        return 0;
+    });
     HIPCHK(hipGetLastError());
     return QBT_OK;
 }
@@ -834,20 +853,22 @@ extern "C" int qbt_render(qbt_ctx* c, const qbt_view* views, int nviews, const f
     for (int i = 0; i < nviews; i++) {
         const qbt_view& q = views[i];
         SceneDev        s;
-        bool            dual, grid;
-        int             rc = c->scene_for(q.basecube, q.maxlevel, s, dual, grid);
+        bool            dual, grid, cmp;
+        /* the fused kernel walks the direct layout; the wavefront kernels either */
+        int rc = c->scene_for(q.basecube, q.maxlevel, s, dual, grid, cmp, c->path == 1);
         if (rc) return rc;
         ViewDev v;
         qbt_make_view(q, lights, nlights, v);
         if ((rc = c->ensure_frame(i, v.w, v.h, mask))) return rc;
         int* spr = (int*) c->sprite.p + i * QBT_MAX_LIGHTS_RENDER;
-        if ((rc = c->launch_sprite(s, dual, grid, v, spr))) return rc;
+        if ((rc = c->launch_sprite(s, cmp, dual, grid, v, spr))) return rc;
         if (c->path == 0) {
             if ((rc = c->launch_render(s, dual, grid, count, v, spr, c->frame_dev(i), mask))) return rc;
             c->last.launches += 1;
-        } else if ((rc = c->render_view_wavefront(s, dual, grid, count, v, spr, c->frame_dev(i), mask))) {
+        } else if ((rc = c->render_view_wavefront(s, cmp, dual, grid, count, v, spr, c->frame_dev(i), mask))) {
             return rc;
         }
+        c->last.compact = cmp;
         prim += (uint64_t) v.w * (uint64_t) v.h;
         c->last.grid_exact = grid;
         c->last.dual       = dual;
@@ -959,6 +980,7 @@ extern "C" int qbt_get_tuning(qbt_ctx* c, qbt_tuning* out)
     out->path          = c->path;
     out->block_threads = c->tune_block;
     out->refill        = c->tune_refill;
+    out->layout        = c->use_compact ? 0 : 1;
     return QBT_OK;
 }
 
@@ -969,9 +991,11 @@ extern "C" int qbt_set_tuning(qbt_ctx* c, const qbt_tuning* in)
     if (in->block_threads != 0 && in->block_threads != 64 && in->block_threads != 128 && in->block_threads != 256)
         return QBT_EINVAL;
     if (in->refill < 0 || in->refill > 64) return QBT_EINVAL;
+    if (in->layout < 0 || in->layout > 1) return QBT_EINVAL;
     c->path        = in->path;
     c->tune_block  = in->block_threads;
     c->tune_refill = in->refill;
+    c->use_compact = in->layout == 0;
     return QBT_OK;
 }
 
@@ -1058,8 +1082,8 @@ extern "C" int qbt_trace_rays(qbt_ctx* c, const float* origins, const float* dir
     if (n > (size_t) 1 << 31) return QBT_ENOTSUP;
     HIPCHK(hipSetDevice(c->device));
     SceneDev s;
-    bool     dual, grid;
-    int      rc = c->scene_for(basecube, maxlevel, s, dual, grid);
+    bool     dual, grid, cmp;
+    int      rc = c->scene_for(basecube, maxlevel, s, dual, grid, cmp, true);
     if (rc) return rc;
     if (semantics == 1) dual = false; /* octree_trace_line walks one tree */
     /* ray scratch: org, dir (n x 12 B each), status/model/leaf (n x 4 B each),
@@ -1083,16 +1107,13 @@ extern "C" int qbt_trace_rays(qbt_ctx* c, const float* origins, const float* dir
     unsigned     nb  = blocks_for(n, bs);
     unsigned long long* cnt = (unsigned long long*) c->counters.p;
     c->prof_mark_begin();
-    if (semantics == 1) {
-        if (grid) k_trace_rays<false, true, true><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
-        else k_trace_rays<false, true, false><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
-    } else if (dual) {
-        if (grid) k_trace_rays<true, false, true><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
-        else k_trace_rays<true, false, false><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
-    } else {
-        if (grid) k_trace_rays<false, false, true><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
-        else k_trace_rays<false, false, false><<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
-    }
+    /* (DUAL, CPU) in {(0,0), (1,0), (0,1)}: the CPU twin walks the static tree only */
+    qbt_dispatch4(cmp, dual, semantics == 1, grid, [&](auto C_, auto D_, auto P_, auto G_) {
+        constexpr bool kD = decltype(D_)::value && !decltype(P_)::value;
+        k_trace_rays<decltype(C_)::value, kD, decltype(P_)::value, decltype(G_)::value>
+            <<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
+        return 0;
+    });
     HIPCHK(hipGetLastError());
     c->prof_mark_end();
     HIPCHK(hipEventRecord(c->ev1, c->stream));

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "qbt.h"
@@ -22,6 +23,13 @@ struct DevBuf {
     size_t cap = 0; /* bytes */
 };
 
+/* breadth-first compact layout of a tree (layout.hip) */
+struct CompactTree {
+    DevBuf rec, model, ref;
+    size_t n  = 0;
+    bool   ok = false; /* built and the node array is a proper tree */
+};
+
 struct TreeBuf {
     DevBuf child, model;
     size_t n         = 0; /* logical node count (size/48 of the last upload) */
@@ -29,6 +37,8 @@ struct TreeBuf {
     bool   validated = true;
     bool   full_check = false;
     size_t dirty_lo = 0, dirty_hi = 0;
+    CompactTree cmp;
+    bool        cmp_stale = true; /* node array changed since the last relayout */
 };
 
 struct AttrBuf {
@@ -126,17 +136,20 @@ struct qbt_ctx {
     int      dev_alloc(DevBuf& b, size_t bytes);
     void     dev_free(DevBuf& b);
     int      validate_tree(int t);
-    int      scene_for(const float basecube[4], int maxlevel, qbt::SceneDev& s, bool& dual, bool& grid);
+    int      relayout(int t);
+    int      scene_for(const float basecube[4], int maxlevel, qbt::SceneDev& s, bool& dual, bool& grid, bool& cmp,
+                       bool allow_compact);
+    bool     use_compact = true; /* traverse the compact layout when both trees have one */
     int      ensure_frame(int view, int w, int h, unsigned mask);
     FrameDev frame_dev(int view);
     int      block_threads(int L, bool dual) const;
     int      launch_render(const qbt::SceneDev& s, bool dual, bool grid, bool count, const ViewDev& v, const int* spr,
                            FrameDev f, unsigned mask);
-    int      launch_sprite(const qbt::SceneDev& s, bool dual, bool grid, const ViewDev& v, int* spr);
+    int      launch_sprite(const qbt::SceneDev& s, bool cmp, bool dual, bool grid, const ViewDev& v, int* spr);
     int      render_surrogate(const qbt_view* views, int nviews, const float* lights, int nlights, unsigned out_mask);
     int      ensure_wave_bufs(size_t npix, int nlights);
-    int      render_view_wavefront(const qbt::SceneDev& s, bool dual, bool grid, bool count, const ViewDev& v,
-                                   const int* spr, FrameDev f, unsigned mask);
+    int      render_view_wavefront(const qbt::SceneDev& s, bool cmp, bool dual, bool grid, bool count,
+                                   const ViewDev& v, const int* spr, FrameDev f, unsigned mask);
     void     mlp_free();
     void     builder_free();
 };
@@ -146,7 +159,32 @@ bool   qbt_grid_exact(const float b[4], int L);
 size_t qbt_stack_bytes(int L, bool dual, int threads);
 void   qbt_make_view(const qbt_view& q, const float* lights, int nlights, ViewDev& v);
 
-#define QBT_HIPCHK(x)                                      \
+/* Runtime bools -> template arguments: f receives std::integral_constant<bool>
+ * values, one per flag. */
+template <class F>
+inline int qbt_dispatch1(bool a, F&& f)
+{
+    return a ? f(std::true_type{}) : f(std::false_type{});
+}
+template <class F>
+inline int qbt_dispatch2(bool a, bool b, F&& f)
+{
+    return qbt_dispatch1(a, [&](auto A) { return qbt_dispatch1(b, [&](auto B) { return f(A, B); }); });
+}
+template <class F>
+inline int qbt_dispatch3(bool a, bool b, bool c, F&& f)
+{
+    return qbt_dispatch1(a, [&](auto A) { return qbt_dispatch2(b, c, [&](auto B, auto C) { return f(A, B, C); }); });
+}
+template <class F>
+inline int qbt_dispatch4(bool a, bool b, bool c, bool d, F&& f)
+{
+    return qbt_dispatch1(a, [&](auto A) {
+        return qbt_dispatch3(b, c, d, [&](auto B, auto C, auto D) { return f(A, B, C, D); });
+    });
+}
+
+#define QBT_HIPCHK(x)                                    \
     do {                                                   \
         hipError_t e_ = (x);                               \
         if (e_ != hipSuccess) {                            \

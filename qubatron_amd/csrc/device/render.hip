@@ -99,7 +99,7 @@ __device__ __forceinline__ bool item_pixel(uint32_t item, int tiles_x, int w, in
     return px < w && py < h;
 }
 
-template <bool DUAL, bool GRID, bool COUNT>
+template <bool CMP, bool DUAL, bool GRID, bool COUNT>
 __global__ void __launch_bounds__(256) k_primary(SceneDev s, ViewDev v, PrimOut o, unsigned* __restrict__ q,
                                                  unsigned refill, unsigned long long* __restrict__ counters)
 {
@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, ViewDev v, PrimOut 
             ms          = (t.ns == 0 && L > 0) ? 0 : s.stat.model[t.ns];
             md          = 0;
             if (DUAL) md = (t.nd == 0 && L > 0) ? 0 : s.dyn.model[t.nd];
-            leaf = t.ns;
+            leaf = CMP ? s.stat.ref[t.ns] : t.ns; /* reference node index */
             isp  = t.entry;
             f    = QBT_PX_LEAF;
             if (isp.w > 0.0f) {
@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, ViewDev v, PrimOut 
             continue;
         }
         if (active) {
-            int stt = trace_step<DUAL, false, GRID>(s, r, st, t);
+            int stt = trace_step<CMP, DUAL, false, GRID>(s, r, st, t);
             if (stt != ST_CONT) {
                 finish(stt);
                 active = false;
@@ -193,7 +193,7 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, ViewDev v, PrimOut 
 /* lstat values */
 enum : uint8_t { LS_OCCLUDED = 0, LS_LIT = 1, LS_DISCARD = 2, LS_SKIPPED = 3 };
 
-template <bool DUAL, bool GRID, bool COUNT>
+template <bool CMP, bool DUAL, bool GRID, bool COUNT>
 __global__ void __launch_bounds__(256) k_shadow(SceneDev s, ViewDev v, int l, const uint32_t* __restrict__ sel_items,
                                                 const float4* __restrict__ isp, uint8_t* __restrict__ lstat,
                                                 uint8_t* __restrict__ disc, unsigned* __restrict__ q,
@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, ViewDev v, int l, co
             continue;
         }
         if (active) {
-            int stt = trace_step<DUAL, false, GRID>(s, r, st, t);
+            int stt = trace_step<CMP, DUAL, false, GRID>(s, r, st, t);
             if (stt != ST_CONT) {
                 finish(stt);
                 active = false;
@@ -408,7 +408,7 @@ int qbt_ctx::ensure_wave_bufs(size_t npix, int nlights)
     return QBT_OK;
 }
 
-template <bool DUAL, bool GRID, bool COUNT>
+template <bool CMP, bool DUAL, bool GRID, bool COUNT>
 static int render_view_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const int* spr, FrameDev f, unsigned mask)
 {
     hipStream_t  st      = c->stream;
@@ -444,7 +444,7 @@ static int render_view_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const 
     const size_t lds = qbt_stack_bytes(s.maxlevel, DUAL, bs);
     const unsigned refill = (unsigned) (c->tune_refill > 0 && c->tune_refill <= 64 ? c->tune_refill : 32);
     {
-        auto kp = k_primary<DUAL, GRID, COUNT>;
+        auto kp = k_primary<CMP, DUAL, GRID, COUNT>;
         int  g  = persistent_grid(c, kp, bs, lds);
         c->prof_mark_begin();
         kp<<<g, bs, lds, st>>>(s, v, po, q, refill, cnt);
@@ -455,7 +455,7 @@ static int render_view_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const 
     HIPCHK(rocprim::select(c->wb_tmp.p, tb, rocprim::counting_iterator<uint32_t>(0u), (const uint8_t*) c->wb_sel.p,
                            (uint32_t*) c->wb_items.p, q + Q_NSEL, items, st));
     for (int l = 0; l < v.nlights; l++) {
-        auto ks = k_shadow<DUAL, GRID, COUNT>;
+        auto ks = k_shadow<CMP, DUAL, GRID, COUNT>;
         int  g  = persistent_grid(c, ks, bs, lds);
         c->prof_mark_begin();
         ks<<<g, bs, lds, st>>>(s, v, l, (const uint32_t*) c->wb_items.p, (const float4*) c->wb_isp.p,
@@ -470,14 +470,12 @@ static int render_view_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const 
     return QBT_OK;
 }
 
-int qbt_ctx::render_view_wavefront(const SceneDev& s, bool dual, bool grid, bool count, const ViewDev& v,
+int qbt_ctx::render_view_wavefront(const SceneDev& s, bool cmp, bool dual, bool grid, bool count, const ViewDev& v,
                                    const int* spr, FrameDev f, unsigned mask)
 {
     if (v.w <= 0 || v.h <= 0) return QBT_OK;
-    if (dual) {
-        if (grid) return count ? render_view_t<true, true, true>(this, s, v, spr, f, mask) : render_view_t<true, true, false>(this, s, v, spr, f, mask);
-        return count ? render_view_t<true, false, true>(this, s, v, spr, f, mask) : render_view_t<true, false, false>(this, s, v, spr, f, mask);
-    }
-    if (grid) return count ? render_view_t<false, true, true>(this, s, v, spr, f, mask) : render_view_t<false, true, false>(this, s, v, spr, f, mask);
-    return count ? render_view_t<false, false, true>(this, s, v, spr, f, mask) : render_view_t<false, false, false>(this, s, v, spr, f, mask);
+    return qbt_dispatch4(cmp, dual, grid, count, [&](auto C_, auto D_, auto G_, auto N_) {
+        return render_view_t<decltype(C_)::value, decltype(D_)::value, decltype(G_)::value, decltype(N_)::value>(
+            this, s, v, spr, f, mask);
+    });
 }

@@ -41,9 +41,11 @@ constexpr float kFltMax = 3.402823466e+38f;
 enum TraceStatus : int { ST_MISS = 0, ST_HIT = 1, ST_DISCARD = -1, ST_ABORT = -2 };
 
 struct TreeDev {
-    const int4* __restrict__ child; /* 2 per node */
-    const int* __restrict__ model;  /* 1 per node */
-    int n;                          /* logical node count */
+    const int4* __restrict__ child; /* direct layout: 2 per node */
+    const int* __restrict__ model;  /* slot 8, 1 per node (of the layout traversed) */
+    int n;                          /* node count of the layout traversed */
+    const uint2* __restrict__ rec;  /* compact layout: {first child, child mask} */
+    const int* __restrict__ ref;    /* compact layout: reference node index */
 };
 
 struct AttrDev {
@@ -159,13 +161,46 @@ __device__ __forceinline__ unsigned exist_mask(int4 lo, int4 hi)
            (hi.x > 0 ? 16u : 0u) | (hi.y > 0 ? 32u : 0u) | (hi.z > 0 ? 64u : 0u) | (hi.w > 0 ? 128u : 0u);
 }
 
+/* The child block of one node in either layout: the 8 child indices of the
+ * direct layout (two int4), or the compact {first child, mask} record. */
+template <bool CMP>
+struct Block;
+
+template <>
+struct Block<false> {
+    int4 lo, hi;
+    __device__ __forceinline__ void clear() { lo = hi = make_int4(0, 0, 0, 0); }
+    __device__ __forceinline__ void load(const TreeDev& t, int i)
+    {
+        const int4* p = t.child + 2 * (size_t) i;
+        lo            = p[0];
+        hi            = p[1];
+    }
+    __device__ __forceinline__ unsigned mask() const { return exist_mask(lo, hi); }
+    __device__ __forceinline__ int      child(int o) const { return pick8(lo, hi, o); }
+};
+
+template <>
+struct Block<true> {
+    uint2 r;
+    __device__ __forceinline__ void     clear() { r = make_uint2(0u, 0u); }
+    __device__ __forceinline__ void     load(const TreeDev& t, int i) { r = t.rec[i]; }
+    __device__ __forceinline__ unsigned mask() const { return r.y; }
+    __device__ __forceinline__ int      child(int o) const
+    {
+        return ((r.y >> o) & 1u) ? (int) (r.x + (uint32_t) __popc(r.y & ((1u << o) - 1u))) : 0;
+    }
+};
+
 /* slot k (< 8) of node i at `level` (oct_from_octets_for_index,
  * octree_fsh.c:127-136): 0 for index 0 below the root. */
+template <bool CMP>
 __device__ __forceinline__ int child_slot(const TreeDev& t, int i, int k, int level)
 {
     if (i == 0 && level > 0) return 0;
-    const int* c = reinterpret_cast<const int*>(t.child);
-    return c[8 * (size_t) i + k];
+    Block<CMP> b;
+    b.load(t, i);
+    return b.child(k);
 }
 
 struct Cand {
@@ -273,28 +308,26 @@ __device__ __forceinline__ int trace_begin(const SceneDev& s, const Ray& r, Trac
  * ancestor. Returns ST_CONT, ST_HIT (t.level == maxlevel; the leaf is
  * t.ns / t.nd with entry point t.entry), ST_MISS or ST_ABORT.
  */
-template <bool DUAL, bool CPU, bool GRID>
+template <bool CMP, bool DUAL, bool CPU, bool GRID>
 __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const Stack& st, TraceState& t)
 {
     if (++t.it > s.iter_cap) return ST_ABORT;
     const int level = t.level;
 
     /* ---- expansion of the node just entered (octree_fsh.c:251-330) ---- */
-    int4 sl = make_int4(0, 0, 0, 0), sh = sl, dl = sl, dh = sl;
+    Block<CMP> sb, db;
+    sb.clear();
+    db.clear();
     if (t.ns != 0 || level == 0) {
-        const int4* p = s.stat.child + 2 * (size_t) t.ns;
-        sl            = p[0];
-        sh            = p[1];
+        sb.load(s.stat, t.ns);
         t.exps++;
     }
     if (DUAL && (t.nd != 0 || level == 0)) {
-        const int4* p = s.dyn.child + 2 * (size_t) t.nd;
-        dl            = p[0];
-        dh            = p[1];
+        db.load(s.dyn, t.nd);
         t.exps++;
     }
-    unsigned mask = exist_mask(sl, sh);
-    if (DUAL) mask |= exist_mask(dl, dh);
+    unsigned mask = sb.mask();
+    if (DUAL) mask |= db.mask();
 
     if (mask) {
         float tx, ty, tz, tw;
@@ -362,8 +395,8 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
             st.pw[level * st.stride] = word | (uint32_t) n | (1u << 3);
             st.ns[level * st.stride] = t.ns;
             if (DUAL) st.nd[level * st.stride] = t.nd;
-            t.ns    = pick8(sl, sh, foct);
-            t.nd    = DUAL ? pick8(dl, dh, foct) : 0;
+            t.ns    = sb.child(foct);
+            t.nd    = DUAL ? db.child(foct) : 0;
             t.ix    = (t.ix << 1) | (foct & 1);
             t.iy    = (t.iy << 1) | ((foct >> 1) & 1);
             t.iz    = (t.iz << 1) | (foct >> 2);
@@ -395,8 +428,8 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
                 cube_at<GRID>(s, lv, ix, iy, iz, x, y, z, ww);
                 isp = midplane<CPU>(src, r, x, y, z, ww);
             }
-            t.ns = child_slot(s.stat, st.ns[lv * st.stride], o, lv);
-            t.nd = DUAL ? child_slot(s.dyn, st.nd[lv * st.stride], o, lv) : 0;
+            t.ns = child_slot<CMP>(s.stat, st.ns[lv * st.stride], o, lv);
+            t.nd = DUAL ? child_slot<CMP>(s.dyn, st.nd[lv * st.stride], o, lv) : 0;
             t.ix    = (ix << 1) | (o & 1);
             t.iy    = (iy << 1) | ((o >> 1) & 1);
             t.iz    = (iz << 1) | (o >> 2);
@@ -412,12 +445,12 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
  * DUAL: the dynamic tree takes part. CPU: octree.c plane-test returns and
  * root acceptance (single tree; callers pass DUAL = false).
  */
-template <bool DUAL, bool CPU, bool GRID>
+template <bool CMP, bool DUAL, bool CPU, bool GRID>
 __device__ int trace(const SceneDev& s, const Ray& r, const Stack& st, Hit& out)
 {
     TraceState t;
     int        stt = trace_begin<CPU>(s, r, t);
-    while (stt == ST_CONT) stt = trace_step<DUAL, CPU, GRID>(s, r, st, t);
+    while (stt == ST_CONT) stt = trace_step<CMP, DUAL, CPU, GRID>(s, r, st, t);
     out.exps = t.exps;
     if (stt == ST_HIT) {
         out.isp = t.entry;

@@ -19,12 +19,34 @@ pytestmark = pytest.mark.gpu
 ALL = q.OUT_ALL | q.RENDER_COUNT
 
 
-@pytest.fixture(params=[1, 0], ids=["wavefront", "fused"])
+@pytest.fixture(params=[(1, 0), (1, 1), (0, 1)], ids=["wavefront-compact", "wavefront-direct", "fused"])
 def ctx(gpu_ctx, request):
-    """Both render paths: wavefront stages (default) and the fused kernel."""
-    gpu_ctx.set_tuning(path=request.param)
+    """Every render path: wavefront stages on the compact node layout
+    (default), wavefront on the direct layout, and the fused kernel."""
+    path, layout = request.param
+    gpu_ctx.set_tuning(path=path, layout=layout)
     yield gpu_ctx
-    gpu_ctx.set_tuning(path=1)
+    gpu_ctx.set_tuning(path=1, layout=0)
+
+
+def test_shared_child_falls_back_to_direct(ctx):
+    """A node array in which two slots point at the same node is not a tree:
+    the compact relayout refuses it and the direct layout renders it, still
+    equal to the oracle (which follows the slots as the shader does)."""
+    sc, pnt, nrm, col, nodes = H.scene(1, npoints=50000)
+    nodes = nodes.copy()
+    root = nodes[0]
+    kids = [o for o in range(8) if root[o] > 0]
+    empty = [o for o in range(8) if root[o] == 0]
+    if empty:
+        nodes[0, empty[0]] = nodes[0, kids[0]]  # a second root slot at the same subtree
+    else:
+        nodes[0, kids[1]] = nodes[0, kids[0]]
+    _upload_scene(ctx, nodes, col, nrm)
+    osc = po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth)
+    g, o = _render_both(ctx, osc, (900.0, 900.0, 900.0), (2.3, 0.2), 96, 64, sc.lights, sc.basecube, sc.depth)
+    assert ctx.stats()["compact"] == 0
+    _assert_same(g, o, "shared child")
 
 
 def _render_both(ctx, osc, camfp, angle, W, H_, lights, basecube, maxlevel, shoot=0):
