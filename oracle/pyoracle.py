@@ -73,9 +73,9 @@ def lib():
             L.orc_mlp_features.restype = None
             L.orc_bf16_round.argtypes = [C.c_float]
             L.orc_bf16_round.restype = C.c_float
-            L.orc_mlp_forward.argtypes = [f32p, f32p, C.c_int, f32p, f32p]
+            L.orc_mlp_forward.argtypes = [f32p, f32p, C.c_int, f32p, f32p, C.c_int]
             L.orc_mlp_forward.restype = None
-            L.orc_mlp_backward.argtypes = [f32p, f32p, f32p, C.c_int, f32p]
+            L.orc_mlp_backward.argtypes = [f32p, f32p, f32p, C.c_int, f32p, C.c_int]
             L.orc_mlp_backward.restype = C.c_float
             L.orc_mlp_adam.argtypes = [f32p, f32p, f32p, f32p, C.c_int, C.c_float, C.c_float, C.c_float,
                                        C.c_float]
@@ -184,3 +184,47 @@ def pixel_dirs(angle, W, H, pixels):
     for i, (px, py) in enumerate(pixels):
         lib().orc_pixel_dir(C.byref(v), _f(qz), _f(qx), int(px), int(py), _f(out[i]))
     return out
+
+
+# ---------------- shadow-visibility MLP (no reference counterpart) ---------
+
+def mlp_features(isp, nrm, light, basecube):
+    isp = np.ascontiguousarray(isp, np.float32).reshape(-1, 3)
+    nrm = np.ascontiguousarray(nrm, np.float32).reshape(-1, 3)
+    light = np.ascontiguousarray(light, np.float32).reshape(-1, 3)
+    if light.shape[0] == 1 and isp.shape[0] > 1:
+        light = np.repeat(light, isp.shape[0], 0)
+    bc = np.asarray(basecube, np.float32)
+    out = np.zeros((isp.shape[0], MLP_IN), np.float32)
+    for i in range(isp.shape[0]):
+        lib().orc_mlp_features(_f(isp[i]), _f(nrm[i]), _f(light[i]), _f(bc), _f(out[i]))
+    return out
+
+
+def mlp_forward(params, x, bf16=True):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_IN)
+    p = np.zeros(x.shape[0], np.float32)
+    lg = np.zeros(x.shape[0], np.float32)
+    lib().orc_mlp_forward(_f(params), _f(x), x.shape[0], _f(p), _f(lg), int(bf16))
+    return p, lg
+
+
+def mlp_backward(params, x, y, bf16=True):
+    params = np.ascontiguousarray(params, np.float32)
+    x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_IN)
+    y = np.ascontiguousarray(y, np.float32)
+    g = np.zeros(MLP_NPARAM, np.float32)
+    loss = lib().orc_mlp_backward(_f(params), _f(x), _f(y), x.shape[0], _f(g), int(bf16))
+    return loss, g
+
+
+def mlp_adam(params, grads, m, v, t, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    for a in (params, m, v):
+        assert a.dtype == np.float32 and a.flags.c_contiguous
+    g = np.ascontiguousarray(grads, np.float32)
+    lib().orc_mlp_adam(_f(params), _f(g), _f(m), _f(v), int(t), lr, b1, b2, eps)
+
+
+def bf16_round(x):
+    return float(lib().orc_bf16_round(float(x)))

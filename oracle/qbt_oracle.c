@@ -560,3 +560,151 @@ int orc_render(const orc_scene* sc, const orc_view* v, int nlights, const float*
     free(th);
     return err ? ORC_ERR : 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* shadow-visibility MLP: 42 -> 64 -> 64 -> 64 -> 1 (no reference counterpart) */
+/* ------------------------------------------------------------------------ */
+
+float orc_bf16_round(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if ((u & 0x7F800000u) == 0x7F800000u) { /* inf / nan: keep, quiet nan */
+        if (u & 0x007FFFFFu) u |= 0x00400000u;
+        u &= 0xFFFF0000u;
+    } else {
+        uint32_t lsb = (u >> 16) & 1u;
+        u += 0x7FFFu + lsb;
+        u &= 0xFFFF0000u;
+    }
+    float r;
+    memcpy(&r, &u, 4);
+    return r;
+}
+
+void orc_mlp_features(const float isp[3], const float nrm[3], const float light[3], const float basecube[4],
+                      float out[ORC_MLP_IN])
+{
+    const float w     = basecube[3];
+    const float lo[3] = {basecube[0], basecube[1] - w, basecube[2] - w};
+    float       u[3];
+    for (int a = 0; a < 3; a++) u[a] = (isp[a] - lo[a]) / w;
+    for (int k = 0; k < 6; k++) {
+        float f = (float) (1 << k) * 3.14159265358979f;
+        for (int a = 0; a < 3; a++) {
+            float arg          = f * u[a];
+            out[6 * k + 2 * a]     = sinf(arg);
+            out[6 * k + 2 * a + 1] = cosf(arg);
+        }
+    }
+    for (int a = 0; a < 3; a++) {
+        out[36 + a] = nrm[a];
+        out[39 + a] = (light[a] - lo[a]) / w;
+    }
+}
+
+static inline float rb(float x, int bf16) { return bf16 ? orc_bf16_round(x) : x; }
+
+/* one dense layer: z[o] = sum_i W[o][i] * h[i] + b[o], sequential fp32 sum */
+static void dense(const float* W, const float* b, const float* h, int nin, int nout, float* z, int bf16)
+{
+    for (int o = 0; o < nout; o++) {
+        float acc = 0.0f;
+        for (int i = 0; i < nin; i++) acc += rb(W[o * nin + i], bf16) * h[i];
+        z[o] = acc + b[o];
+    }
+}
+
+typedef struct mlp_acts {
+    float x[ORC_MLP_IN];
+    float z1[64], h1[64], z2[64], h2[64], z3[64], h3[64];
+    float logit, p;
+} mlp_acts;
+
+static void mlp_forward_one(const float* P, const float* xin, mlp_acts* a, int bf16)
+{
+    for (int i = 0; i < ORC_MLP_IN; i++) a->x[i] = rb(xin[i], bf16);
+    dense(P + ORC_MLP_W1, P + ORC_MLP_B1, a->x, ORC_MLP_IN, 64, a->z1, bf16);
+    for (int o = 0; o < 64; o++) a->h1[o] = rb(a->z1[o] > 0.0f ? a->z1[o] : 0.0f, bf16);
+    dense(P + ORC_MLP_W2, P + ORC_MLP_B2, a->h1, 64, 64, a->z2, bf16);
+    for (int o = 0; o < 64; o++) a->h2[o] = rb(a->z2[o] > 0.0f ? a->z2[o] : 0.0f, bf16);
+    dense(P + ORC_MLP_W3, P + ORC_MLP_B3, a->h2, 64, 64, a->z3, bf16);
+    for (int o = 0; o < 64; o++) a->h3[o] = rb(a->z3[o] > 0.0f ? a->z3[o] : 0.0f, bf16);
+    float z4;
+    dense(P + ORC_MLP_W4, P + ORC_MLP_B4, a->h3, 64, 1, &z4, bf16);
+    a->logit = z4;
+    a->p     = 1.0f / (1.0f + expf(-z4));
+}
+
+void orc_mlp_forward(const float* params, const float* x, int n, float* p, float* logits, int bf16)
+{
+    mlp_acts a;
+    for (int s = 0; s < n; s++) {
+        mlp_forward_one(params, x + (size_t) s * ORC_MLP_IN, &a, bf16);
+        p[s] = a.p;
+        if (logits) logits[s] = a.logit;
+    }
+}
+
+float orc_mlp_backward(const float* params, const float* x, const float* y, int n, float* g, int bf16)
+{
+    const float* P = params;
+    memset(g, 0, sizeof(float) * ORC_MLP_NPARAM);
+    double   loss = 0.0;
+    mlp_acts a;
+    float    dz4, dz3[64], dz2[64], dz1[64], dh[64];
+    for (int s = 0; s < n; s++) {
+        mlp_forward_one(P, x + (size_t) s * ORC_MLP_IN, &a, bf16);
+        float pc = a.p < 1e-7f ? 1e-7f : a.p > 1.0f - 1e-7f ? 1.0f - 1e-7f : a.p;
+        loss -= y[s] > 0.5f ? log(pc) : log(1.0 - pc);
+        dz4 = rb((a.p - y[s]) / (float) n, bf16);
+        /* layer 4 */
+        for (int i = 0; i < 64; i++) g[ORC_MLP_W4 + i] += dz4 * a.h3[i];
+        g[ORC_MLP_B4] += dz4;
+        for (int i = 0; i < 64; i++) dh[i] = rb(P[ORC_MLP_W4 + i], bf16) * dz4;
+        for (int o = 0; o < 64; o++) dz3[o] = rb(a.z3[o] > 0.0f ? dh[o] : 0.0f, bf16);
+        /* layer 3 */
+        for (int o = 0; o < 64; o++) {
+            for (int i = 0; i < 64; i++) g[ORC_MLP_W3 + o * 64 + i] += dz3[o] * a.h2[i];
+            g[ORC_MLP_B3 + o] += dz3[o];
+        }
+        for (int i = 0; i < 64; i++) {
+            float acc = 0.0f;
+            for (int o = 0; o < 64; o++) acc += rb(P[ORC_MLP_W3 + o * 64 + i], bf16) * dz3[o];
+            dh[i] = acc;
+        }
+        for (int o = 0; o < 64; o++) dz2[o] = rb(a.z2[o] > 0.0f ? dh[o] : 0.0f, bf16);
+        /* layer 2 */
+        for (int o = 0; o < 64; o++) {
+            for (int i = 0; i < 64; i++) g[ORC_MLP_W2 + o * 64 + i] += dz2[o] * a.h1[i];
+            g[ORC_MLP_B2 + o] += dz2[o];
+        }
+        for (int i = 0; i < 64; i++) {
+            float acc = 0.0f;
+            for (int o = 0; o < 64; o++) acc += rb(P[ORC_MLP_W2 + o * 64 + i], bf16) * dz2[o];
+            dh[i] = acc;
+        }
+        for (int o = 0; o < 64; o++) dz1[o] = rb(a.z1[o] > 0.0f ? dh[o] : 0.0f, bf16);
+        /* layer 1 */
+        for (int o = 0; o < 64; o++) {
+            for (int i = 0; i < ORC_MLP_IN; i++) g[ORC_MLP_W1 + o * ORC_MLP_IN + i] += dz1[o] * a.x[i];
+            g[ORC_MLP_B1 + o] += dz1[o];
+        }
+    }
+    return (float) (loss / (n > 0 ? n : 1));
+}
+
+void orc_mlp_adam(float* params, const float* grads, float* m, float* v, int t, float lr, float beta1, float beta2,
+                  float eps)
+{
+    const float c1 = 1.0f - powf(beta1, (float) t);
+    const float c2 = 1.0f - powf(beta2, (float) t);
+    for (int i = 0; i < ORC_MLP_NPARAM; i++) {
+        float gi  = grads[i];
+        m[i]      = beta1 * m[i] + (1.0f - beta1) * gi;
+        v[i]      = beta2 * v[i] + (1.0f - beta2) * gi * gi;
+        float mh  = m[i] / c1;
+        float vh  = v[i] / c2;
+        params[i] = params[i] - lr * mh / (sqrtf(vh) + eps);
+    }
+}

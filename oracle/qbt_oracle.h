@@ -137,14 +137,26 @@ float orc_bf16_round(float x); /* round to nearest even bf16, back to fp32 */
  * W1 64x42, b1 64, W2 64x64, b2 64, W3 64x64, b3 64, W4 1x64, b4 1. */
 #define ORC_MLP_NPARAM (64 * 42 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1)
 
-/* Forward of n rows (x: n x 42) with weights/activations rounded to bf16 and
- * fp32 accumulation; writes sigmoid outputs p (n). If acts != NULL it
- * receives the bf16-rounded post-ReLU activations h1,h2,h3 (n x 192) and the
- * logits (n) for the backward pass. */
-void orc_mlp_forward(const float* params, const float* x, int n, float* p, float* acts);
-/* Mean binary cross-entropy gradient over n rows, labels y in {0,1}:
- * writes grads (ORC_MLP_NPARAM) and returns the mean loss. */
-float orc_mlp_backward(const float* params, const float* x, const float* y, int n, float* grads);
+/* Forward of n rows (x: n x 42). bf16 = 1: inputs, weights and post-ReLU
+ * activations rounded to bf16 (round to nearest even), fp32 accumulation and
+ * fp32 biases, as the GPU kernels compute; bf16 = 0: plain fp32 (for
+ * finite-difference checks). Writes sigmoid outputs p (n); logits (n) if
+ * not NULL. */
+void orc_mlp_forward(const float* params, const float* x, int n, float* p, float* logits, int bf16);
+/* Mean binary cross-entropy over n rows with labels y in {0,1}: writes the
+ * gradient (ORC_MLP_NPARAM) and returns the mean loss. In bf16 mode the
+ * back-propagated pre-activation gradients are rounded to bf16 before they
+ * meet weights or activations, as in the GPU kernels. */
+float orc_mlp_backward(const float* params, const float* x, const float* y, int n, float* grads, int bf16);
+/* Offsets of the parameter blocks in the flat array. */
+#define ORC_MLP_W1 0
+#define ORC_MLP_B1 (64 * 42)
+#define ORC_MLP_W2 (ORC_MLP_B1 + 64)
+#define ORC_MLP_B2 (ORC_MLP_W2 + 64 * 64)
+#define ORC_MLP_W3 (ORC_MLP_B2 + 64)
+#define ORC_MLP_B3 (ORC_MLP_W3 + 64 * 64)
+#define ORC_MLP_W4 (ORC_MLP_B3 + 64)
+#define ORC_MLP_B4 (ORC_MLP_W4 + 64)
 /* One Adam step (t = 1-based step count). */
 void orc_mlp_adam(float* params, const float* grads, float* m, float* v, int t, float lr, float beta1, float beta2,
                   float eps);
