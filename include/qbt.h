@@ -106,6 +106,7 @@ typedef struct qbt_view {
 /* Render flags (high bits of the same word). */
 #define QBT_RENDER_COUNT 0x10000u     /* count node expansions (stats + QBT_OUT_EXP) */
 #define QBT_RENDER_SURROGATE 0x20000u /* shadow bits from the MLP outside its band */
+#define QBT_RENDER_COLLECT 0x40000u   /* sample exact shadow results for training */
 
 /* Pixel flags. */
 #define QBT_PX_DISCARD 1u /* fragment discarded: colour (0,0,0,0) */
@@ -164,6 +165,7 @@ typedef struct qbt_stats {
     uint32_t grid_exact;     /* the basecube grid was exact in fp32 */
     uint32_t dual;           /* the dynamic tree took part */
     uint32_t compact;        /* the compact node layout was traversed */
+    uint64_t expansions_primary; /* of which by primary rays (QBT_RENDER_COUNT only) */
 } qbt_stats;
 
 int qbt_get_stats(qbt_ctx* ctx, qbt_stats* out);
@@ -175,6 +177,9 @@ int qbt_synchronize(qbt_ctx* ctx);
  * of the traversal kernel (render or ray batch) is bracketed by a HIP event
  * pair on the context stream (up to 4096 launches). qbt_profile_end
  * synchronises and reports launches and their durations in milliseconds. */
+#define QBT_PROF_PRIMARY 0   /* fused render kernel, primary kernel, ray batches */
+#define QBT_PROF_SHADOW 1    /* per-light shadow kernel of the wavefront path */
+#define QBT_PROF_SURROGATE 2 /* surrogate decision kernel */
 typedef struct qbt_profile {
     uint32_t launches;
     uint32_t reserved;
@@ -182,6 +187,8 @@ typedef struct qbt_profile {
     double   avg_ms;
     double   min_ms;
     double   max_ms;
+    uint32_t kind_launches[4]; /* per QBT_PROF_* */
+    double   kind_ms[4];       /* summed duration per QBT_PROF_* */
 } qbt_profile;
 
 int qbt_profile_begin(qbt_ctx* ctx);
@@ -230,6 +237,57 @@ int qbt_trace_rays(qbt_ctx* ctx, const float* origins, const float* dirs, size_t
  * node. *nodes_out receives the node count (the reference's len). */
 int qbt_build_octree(qbt_ctx* ctx, int tree, const float basecube[4], int levels, size_t npoints,
                      size_t* nodes_out);
+
+/* ---------------- learned shadow surrogate (no reference counterpart) ----
+ * A 42 -> 64 -> 64 -> 64 -> 1 MLP (ReLU, sigmoid; bf16 weights and
+ * activations on the matrix cores, fp32 accumulation and biases) predicting
+ * the shadow bit of a (hit point, normal, light) query. Inputs: the hit
+ * position scaled to the base cube, u = (p - (x, y - w, z - w)) / w, as
+ * sin/cos of 2^k pi u for k = 0..5 (36 values: index 6k + 2a + {0 sin,
+ * 1 cos} for axis a), the normal (3), the light position scaled the same
+ * way (3). Parameters: flat fp32, row-major W[out][in]: W1 64x42, b1, W2
+ * 64x64, b2, W3 64x64, b3, W4 1x64, b4.
+ *
+ * qbt_render with QBT_RENDER_SURROGATE (wavefront path): primary rays are
+ * traced exactly; each (shaded pixel, light) query whose predicted
+ * probability is below band_lo or above band_hi takes the predicted bit
+ * (QBT_PX_PREDICTED set), the others are traced exactly. stats.predicted
+ * counts the predicted queries. With QBT_RENDER_COLLECT an exact render
+ * appends every stride-th (shaded pixel, light) query with its exact bit to
+ * the training store (capacity from qbt_surrogate_reserve). */
+#define QBT_MLP_INPUTS 42
+#define QBT_MLP_PARAMS 11137
+
+/* He-uniform weights from seed, zero biases, Adam state reset. */
+int qbt_surrogate_init(qbt_ctx* ctx, uint64_t seed);
+int qbt_surrogate_set_params(qbt_ctx* ctx, const float* params, size_t n);
+int qbt_surrogate_get_params(qbt_ctx* ctx, float* params, size_t n);
+/* Uncertainty band, 0 < lo <= hi < 1 (default 0.1, 0.9). */
+int qbt_surrogate_set_band(qbt_ctx* ctx, float lo, float hi);
+/* (Re)allocate the training store for nsamples queries and empty it;
+ * collecting renders keep every stride-th query (stride >= 1). */
+int qbt_surrogate_reserve(qbt_ctx* ctx, size_t nsamples, uint32_t stride);
+/* Samples held (saturates at the capacity); lit = how many are labelled 1. */
+int qbt_surrogate_samples(qbt_ctx* ctx, size_t* n, size_t* lit);
+/* Copy samples [start, start + n) of the store: 10 floats each (hit point
+ * xyz, normal xyz, light xyz, label). */
+int qbt_surrogate_read_samples(qbt_ctx* ctx, size_t start, size_t n, float* out);
+/* Probabilities of n feature rows (x: n x 42 host floats). */
+int qbt_surrogate_forward(qbt_ctx* ctx, const float* x, size_t n, float* p);
+/* Probabilities of n queries from raw inputs (host, n x 3 floats each), the
+ * feature computation and fused inference kernel the render path uses. */
+int qbt_surrogate_predict(qbt_ctx* ctx, const float* isp, const float* nrm, const float* light, size_t n,
+                          const float basecube[4], float* p);
+/* Mean binary cross-entropy of n labelled rows and its gradient (flat
+ * parameter order; grads may be NULL). */
+int qbt_surrogate_grad(qbt_ctx* ctx, const float* x, const float* y, size_t n, float* grads, float* loss);
+/* qbt_surrogate_grad followed by one Adam step (beta1 0.9, beta2 0.999,
+ * eps 1e-8, bias-corrected) with learning rate lr. */
+int qbt_surrogate_step(qbt_ctx* ctx, const float* x, const float* y, size_t n, float lr, float* loss);
+/* steps Adam steps on minibatches (batch: multiple of 16) drawn without
+ * replacement from a seeded shuffle of the store; loss of the last batch. */
+int qbt_surrogate_train(qbt_ctx* ctx, const float basecube[4], int steps, int batch, float lr, uint64_t seed,
+                        float* loss);
 
 /* Library version string. */
 const char* qbt_version(void);

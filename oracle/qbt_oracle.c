@@ -662,7 +662,8 @@ float orc_mlp_backward(const float* params, const float* x, const float* y, int 
         for (int i = 0; i < 64; i++) g[ORC_MLP_W4 + i] += dz4 * a.h3[i];
         g[ORC_MLP_B4] += dz4;
         for (int i = 0; i < 64; i++) dh[i] = rb(P[ORC_MLP_W4 + i], bf16) * dz4;
-        for (int o = 0; o < 64; o++) dz3[o] = rb(a.z3[o] > 0.0f ? dh[o] : 0.0f, bf16);
+        /* ReLU derivative from the stored activation (h > 0), as the GPU sees it */
+        for (int o = 0; o < 64; o++) dz3[o] = rb(a.h3[o] > 0.0f ? dh[o] : 0.0f, bf16);
         /* layer 3 */
         for (int o = 0; o < 64; o++) {
             for (int i = 0; i < 64; i++) g[ORC_MLP_W3 + o * 64 + i] += dz3[o] * a.h2[i];
@@ -673,7 +674,7 @@ float orc_mlp_backward(const float* params, const float* x, const float* y, int 
             for (int o = 0; o < 64; o++) acc += rb(P[ORC_MLP_W3 + o * 64 + i], bf16) * dz3[o];
             dh[i] = acc;
         }
-        for (int o = 0; o < 64; o++) dz2[o] = rb(a.z2[o] > 0.0f ? dh[o] : 0.0f, bf16);
+        for (int o = 0; o < 64; o++) dz2[o] = rb(a.h2[o] > 0.0f ? dh[o] : 0.0f, bf16);
         /* layer 2 */
         for (int o = 0; o < 64; o++) {
             for (int i = 0; i < 64; i++) g[ORC_MLP_W2 + o * 64 + i] += dz2[o] * a.h1[i];
@@ -684,7 +685,7 @@ float orc_mlp_backward(const float* params, const float* x, const float* y, int 
             for (int o = 0; o < 64; o++) acc += rb(P[ORC_MLP_W2 + o * 64 + i], bf16) * dz2[o];
             dh[i] = acc;
         }
-        for (int o = 0; o < 64; o++) dz1[o] = rb(a.z1[o] > 0.0f ? dh[o] : 0.0f, bf16);
+        for (int o = 0; o < 64; o++) dz1[o] = rb(a.h1[o] > 0.0f ? dh[o] : 0.0f, bf16);
         /* layer 1 */
         for (int o = 0; o < 64; o++) {
             for (int i = 0; i < ORC_MLP_IN; i++) g[ORC_MLP_W1 + o * ORC_MLP_IN + i] += dz1[o] * a.x[i];

@@ -20,6 +20,7 @@ import numpy as np
 from . import _ffi
 from ._ffi import (QbtError, check, fptr, iptr, u8ptr, OUT_HIT, OUT_LEAF, OUT_SHADOW, OUT_RGBA8, OUT_RGBAF,
                    OUT_ISP, OUT_FLAGS, OUT_DYN, OUT_EXP, OUT_DEFAULT, OUT_ALL, RENDER_COUNT, RENDER_SURROGATE,
+                   RENDER_COLLECT, MLP_INPUTS, MLP_PARAMS,
                    PX_DISCARD, PX_LEAF, PX_SHADED, PX_SPRITE, PX_PREDICTED)
 
 __all__ = ["HostOctree", "Scene", "Context", "OctreeGLC", "QbtError", "view", "octree_from_points"]
@@ -318,7 +319,10 @@ class Context:
     def profile_end(self):
         p = _ffi.Profile()
         check(self._d.qbt_profile_end(self._h, C.byref(p)), "qbt_profile_end")
-        return {k: getattr(p, k) for k, _ in _ffi.Profile._fields_ if k != "reserved"}
+        out = {k: getattr(p, k) for k, _ in _ffi.Profile._fields_ if k != "reserved"}
+        out["kind_launches"] = list(p.kind_launches)
+        out["kind_ms"] = list(p.kind_ms)
+        return out
 
     def tuning(self):
         t = _ffi.Tuning()
@@ -343,6 +347,81 @@ class Context:
         check(self._d.qbt_copy_bandwidth(self._h, int(nbytes), int(reps), C.byref(ms), C.byref(gbps)),
               "qbt_copy_bandwidth")
         return ms.value, gbps.value
+
+    # -- learned shadow surrogate ------------------------------------------
+    def surrogate_init(self, seed=0):
+        check(self._d.qbt_surrogate_init(self._h, int(seed)), "qbt_surrogate_init")
+
+    def surrogate_params(self):
+        out = np.empty(_ffi.MLP_PARAMS, dtype=np.float32)
+        check(self._d.qbt_surrogate_get_params(self._h, fptr(out), out.size), "qbt_surrogate_get_params")
+        return out
+
+    def surrogate_set_params(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        check(self._d.qbt_surrogate_set_params(self._h, fptr(p), p.size), "qbt_surrogate_set_params")
+
+    def surrogate_set_band(self, lo=0.1, hi=0.9):
+        check(self._d.qbt_surrogate_set_band(self._h, float(lo), float(hi)), "qbt_surrogate_set_band")
+
+    def surrogate_reserve(self, nsamples, stride=1):
+        check(self._d.qbt_surrogate_reserve(self._h, int(nsamples), int(stride)), "qbt_surrogate_reserve")
+
+    def surrogate_samples(self):
+        """(samples held, of which lit)."""
+        n, lit = C.c_size_t(0), C.c_size_t(0)
+        check(self._d.qbt_surrogate_samples(self._h, C.byref(n), C.byref(lit)), "qbt_surrogate_samples")
+        return int(n.value), int(lit.value)
+
+    def surrogate_read_samples(self, start=0, count=None):
+        """Samples as an (n, 10) array: hit point, normal, light, label."""
+        if count is None:
+            count = self.surrogate_samples()[0] - start
+        out = np.empty((count, 10), dtype=np.float32)
+        check(self._d.qbt_surrogate_read_samples(self._h, int(start), int(count), fptr(out)),
+              "qbt_surrogate_read_samples")
+        return out
+
+    def surrogate_forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, _ffi.MLP_INPUTS)
+        p = np.empty(x.shape[0], dtype=np.float32)
+        check(self._d.qbt_surrogate_forward(self._h, fptr(x), x.shape[0], fptr(p)), "qbt_surrogate_forward")
+        return p
+
+    def surrogate_predict(self, isp, nrm, light, basecube):
+        isp = np.ascontiguousarray(isp, dtype=np.float32).reshape(-1, 3)
+        nrm = np.ascontiguousarray(nrm, dtype=np.float32).reshape(-1, 3)
+        light = np.ascontiguousarray(light, dtype=np.float32).reshape(-1, 3)
+        bc = np.asarray(basecube, dtype=np.float32)
+        p = np.empty(isp.shape[0], dtype=np.float32)
+        check(self._d.qbt_surrogate_predict(self._h, fptr(isp), fptr(nrm), fptr(light), isp.shape[0], fptr(bc),
+                                            fptr(p)), "qbt_surrogate_predict")
+        return p
+
+    def surrogate_grad(self, x, y):
+        """(mean BCE loss, gradient in flat parameter order)."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, _ffi.MLP_INPUTS)
+        y = np.ascontiguousarray(y, dtype=np.float32).ravel()
+        g = np.empty(_ffi.MLP_PARAMS, dtype=np.float32)
+        loss = C.c_float(0.0)
+        check(self._d.qbt_surrogate_grad(self._h, fptr(x), fptr(y), x.shape[0], fptr(g), C.byref(loss)),
+              "qbt_surrogate_grad")
+        return float(loss.value), g
+
+    def surrogate_step(self, x, y, lr):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, _ffi.MLP_INPUTS)
+        y = np.ascontiguousarray(y, dtype=np.float32).ravel()
+        loss = C.c_float(0.0)
+        check(self._d.qbt_surrogate_step(self._h, fptr(x), fptr(y), x.shape[0], float(lr), C.byref(loss)),
+              "qbt_surrogate_step")
+        return float(loss.value)
+
+    def surrogate_train(self, basecube, steps, batch=4096, lr=1e-3, seed=0):
+        bc = np.asarray(basecube, dtype=np.float32)
+        loss = C.c_float(0.0)
+        check(self._d.qbt_surrogate_train(self._h, fptr(bc), int(steps), int(batch), float(lr), int(seed),
+                                          C.byref(loss)), "qbt_surrogate_train")
+        return float(loss.value)
 
     def trace_rays(self, origins, dirs, basecube, maxlevel, semantics=0):
         o = np.ascontiguousarray(origins, dtype=np.float32).reshape(-1, 3)

@@ -72,7 +72,8 @@ class Frame(C.Structure):
 
 class Profile(C.Structure):
     _fields_ = [("launches", C.c_uint32), ("reserved", C.c_uint32), ("total_ms", C.c_double), ("avg_ms", C.c_double),
-                ("min_ms", C.c_double), ("max_ms", C.c_double)]
+                ("min_ms", C.c_double), ("max_ms", C.c_double), ("kind_launches", C.c_uint32 * 4),
+                ("kind_ms", C.c_double * 4)]
 
 
 class Tuning(C.Structure):
@@ -84,7 +85,7 @@ class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("trace_ms", C.c_double), ("primary_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("hits", C.c_uint64), ("expansions", C.c_uint64),
                 ("predicted", C.c_uint64), ("launches", C.c_uint32), ("grid_exact", C.c_uint32),
-                ("dual", C.c_uint32), ("compact", C.c_uint32)]
+                ("dual", C.c_uint32), ("compact", C.c_uint32), ("expansions_primary", C.c_uint64)]
 
 
 BUFFER_STATIC_COLOR, BUFFER_STATIC_NORMAL, BUFFER_STATIC_OCTREE = 0, 1, 2
@@ -98,6 +99,9 @@ OUT_DEFAULT = OUT_HIT | OUT_LEAF | OUT_SHADOW | OUT_RGBA8
 OUT_ALL = 0x1FF
 RENDER_COUNT = 0x10000
 RENDER_SURROGATE = 0x20000
+RENDER_COLLECT = 0x40000
+MLP_INPUTS = 42
+MLP_PARAMS = 11137
 
 PX_DISCARD, PX_LEAF, PX_SHADED, PX_SPRITE, PX_PREDICTED = 1, 2, 4, 8, 16
 
@@ -167,6 +171,9 @@ DEV_SYMBOLS = [
     "qbt_update", "qbt_read_frame", "qbt_frame_device", "qbt_get_stats", "qbt_trace_rays", "qbt_build_octree",
     "qbt_version", "qbt_last_hip_error", "qbt_synchronize", "qbt_profile_begin", "qbt_profile_end",
     "qbt_copy_bandwidth", "qbt_get_tuning", "qbt_set_tuning",
+    "qbt_surrogate_init", "qbt_surrogate_set_params", "qbt_surrogate_get_params", "qbt_surrogate_set_band",
+    "qbt_surrogate_reserve", "qbt_surrogate_samples", "qbt_surrogate_read_samples", "qbt_surrogate_forward",
+    "qbt_surrogate_predict", "qbt_surrogate_grad", "qbt_surrogate_step", "qbt_surrogate_train",
 ]
 
 
@@ -196,6 +203,19 @@ def _declare_dev(d):
     d.qbt_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     d.qbt_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
     d.qbt_copy_bandwidth.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    szp = C.POINTER(C.c_size_t)
+    d.qbt_surrogate_init.argtypes = [vp, C.c_uint64]
+    d.qbt_surrogate_set_params.argtypes = [vp, f32p, C.c_size_t]
+    d.qbt_surrogate_get_params.argtypes = [vp, f32p, C.c_size_t]
+    d.qbt_surrogate_set_band.argtypes = [vp, C.c_float, C.c_float]
+    d.qbt_surrogate_reserve.argtypes = [vp, C.c_size_t, C.c_uint32]
+    d.qbt_surrogate_samples.argtypes = [vp, szp, szp]
+    d.qbt_surrogate_read_samples.argtypes = [vp, C.c_size_t, C.c_size_t, f32p]
+    d.qbt_surrogate_forward.argtypes = [vp, f32p, C.c_size_t, f32p]
+    d.qbt_surrogate_predict.argtypes = [vp, f32p, f32p, f32p, C.c_size_t, f32p, f32p]
+    d.qbt_surrogate_grad.argtypes = [vp, f32p, f32p, C.c_size_t, f32p, f32p]
+    d.qbt_surrogate_step.argtypes = [vp, f32p, f32p, C.c_size_t, C.c_float, f32p]
+    d.qbt_surrogate_train.argtypes = [vp, f32p, C.c_int, C.c_int, C.c_float, C.c_uint64, f32p]
     d.qbt_version.restype = C.c_char_p
     d.qbt_last_hip_error.restype = C.c_char_p
 

@@ -83,6 +83,17 @@ __device__ __forceinline__ void pixel_dir(const ViewDev& v, int px, int py, floa
     quat_rotate(v.qx, c1, csv);
 }
 
+/* Work item -> pixel: items run over 8x8 tiles in row-major tile order,
+ * item = 64 * tile + 8 * (py % 8) + px % 8. False for the padding items of
+ * edge tiles. */
+__device__ __forceinline__ bool item_pixel(uint32_t item, int tiles_x, int w, int h, int& px, int& py)
+{
+    uint32_t tile = item >> 6, sub = item & 63u;
+    px            = (int) (tile % (uint32_t) tiles_x) * 8 + (int) (sub & 7u);
+    py            = (int) (tile / (uint32_t) tiles_x) * 8 + (int) (sub >> 3);
+    return px < w && py < h;
+}
+
 __device__ __forceinline__ unsigned wave_sum(unsigned x)
 {
 #pragma unroll

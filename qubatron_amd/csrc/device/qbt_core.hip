@@ -156,7 +156,7 @@ __global__ void __launch_bounds__(256) k_render(SceneDev s, ViewDev v, const int
     const int px = tx * tw + (threadIdx.x % tw), py = ty * th + (threadIdx.x / tw);
     const bool valid = px < v.w && py < v.h;
 
-    unsigned n_shadow = 0, n_hit = 0, n_exp = 0, n_abort = 0;
+    unsigned n_shadow = 0, n_hit = 0, n_exp = 0, n_exp_p = 0, n_abort = 0;
     if (valid) {
         const size_t o = (size_t) py * v.w + px;
         float        csv[3];
@@ -164,7 +164,7 @@ __global__ void __launch_bounds__(256) k_render(SceneDev s, ViewDev v, const int
         Ray r = {v.cam[0], v.cam[1], v.cam[2], csv[0], csv[1], csv[2]};
         Hit h;
         int stt = trace<false, DUAL, false, GRID>(s, r, st, h); /* fused path: direct layout */
-        if (COUNT) n_exp += h.exps;
+        if (COUNT) n_exp += h.exps, n_exp_p += h.exps;
 
         int32_t  hit = -1, leaf = -1, dyn = -1;
         uint32_t flg = 0, sbits = 0;
@@ -265,9 +265,10 @@ __global__ void __launch_bounds__(256) k_render(SceneDev s, ViewDev v, const int
     n_shadow = wave_sum(n_shadow);
     n_hit    = wave_sum(n_hit);
     n_abort  = wave_sum(n_abort);
-    if (COUNT) n_exp = wave_sum(n_exp);
+    if (COUNT) n_exp = wave_sum(n_exp), n_exp_p = wave_sum(n_exp_p);
     if ((threadIdx.x & 63) == 0) {
         if (n_shadow) atomicAdd(&counters[CNT_SHADOW], (unsigned long long) n_shadow);
+        if (COUNT && n_exp_p) atomicAdd(&counters[CNT_EXP_PRIM], (unsigned long long) n_exp_p);
         if (n_hit) atomicAdd(&counters[CNT_HITS], (unsigned long long) n_hit);
         if (n_abort) atomicAdd(&counters[CNT_ABORT], (unsigned long long) n_abort);
         if (COUNT && n_exp) atomicAdd(&counters[CNT_EXP], (unsigned long long) n_exp);
@@ -790,7 +791,7 @@ static void launch_render_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, con
     const int    th    = bs / 16;
     const int    tiles = ((v.w + 15) / 16) * ((v.h + th - 1) / th);
     const size_t lds   = qbt_stack_bytes(s.maxlevel, DUAL, bs);
-    c->prof_mark_begin();
+    c->prof_mark_begin(QBT_PROF_PRIMARY);
     k_render<DUAL, GRID, COUNT><<<tiles, bs, lds, c->stream>>>(s, v, spr, f, mask, (unsigned long long*) c->counters.p);
     c->prof_mark_end();
 }
@@ -843,7 +844,10 @@ extern "C" int qbt_render(qbt_ctx* c, const qbt_view* views, int nviews, const f
         if (!(q.width >= 1.0f) || !(q.height >= 1.0f) || q.width > 65536.0f || q.height > 65536.0f) return QBT_EINVAL;
         if ((double) (int) q.width * (double) (int) q.height > 2147483647.0) return QBT_EINVAL;
     }
-    if (out_mask & QBT_RENDER_SURROGATE) return c->render_surrogate(views, nviews, lights, nlights, out_mask);
+    const unsigned flags = out_mask & (QBT_RENDER_SURROGATE | QBT_RENDER_COLLECT);
+    /* the surrogate and sample collection are stages of the wavefront path */
+    const int path = flags ? 1 : c->path;
+    if ((flags & QBT_RENDER_SURROGATE) && !qbt_surrogate_ready(c)) return QBT_EINVAL;
 
     c->last = qbt_stats();
     k_zero_u64<<<1, 64, 0, c->stream>>>((unsigned long long*) c->counters.p, CNT_N);
@@ -855,17 +859,17 @@ extern "C" int qbt_render(qbt_ctx* c, const qbt_view* views, int nviews, const f
         SceneDev        s;
         bool            dual, grid, cmp;
         /* the fused kernel walks the direct layout; the wavefront kernels either */
-        int rc = c->scene_for(q.basecube, q.maxlevel, s, dual, grid, cmp, c->path == 1);
+        int rc = c->scene_for(q.basecube, q.maxlevel, s, dual, grid, cmp, path == 1);
         if (rc) return rc;
         ViewDev v;
         qbt_make_view(q, lights, nlights, v);
         if ((rc = c->ensure_frame(i, v.w, v.h, mask))) return rc;
         int* spr = (int*) c->sprite.p + i * QBT_MAX_LIGHTS_RENDER;
         if ((rc = c->launch_sprite(s, cmp, dual, grid, v, spr))) return rc;
-        if (c->path == 0) {
+        if (path == 0) {
             if ((rc = c->launch_render(s, dual, grid, count, v, spr, c->frame_dev(i), mask))) return rc;
             c->last.launches += 1;
-        } else if ((rc = c->render_view_wavefront(s, cmp, dual, grid, count, v, spr, c->frame_dev(i), mask))) {
+        } else if ((rc = c->render_view_wavefront(s, cmp, dual, grid, count, v, spr, c->frame_dev(i), mask, flags))) {
             return rc;
         }
         c->last.compact = cmp;
@@ -926,6 +930,7 @@ extern "C" int qbt_get_stats(qbt_ctx* c, qbt_stats* out)
         c->last.shadow_rays = cnt[CNT_SHADOW];
         c->last.hits        = cnt[CNT_HITS];
         c->last.expansions  = cnt[CNT_EXP];
+        c->last.expansions_primary = cnt[CNT_EXP_PRIM];
         c->last.predicted   = cnt[CNT_PRED];
         c->aborted          = cnt[CNT_ABORT];
         c->pending_stats    = false;
@@ -964,6 +969,9 @@ extern "C" int qbt_profile_end(qbt_ctx* c, qbt_profile* out)
         float ms = 0.0f;
         HIPCHK(hipEventElapsedTime(&ms, c->prof_ev[2 * i], c->prof_ev[2 * i + 1]));
         p.total_ms += ms;
+        const int k = i < c->prof_kind.size() ? c->prof_kind[i] : 0;
+        p.kind_launches[k]++;
+        p.kind_ms[k] += ms;
         if (i == 0 || ms < p.min_ms) p.min_ms = ms;
         if (ms > p.max_ms) p.max_ms = ms;
     }
@@ -1106,7 +1114,7 @@ extern "C" int qbt_trace_rays(qbt_ctx* c, const float* origins, const float* dir
     const size_t lds = qbt_stack_bytes(maxlevel, dual, bs);
     unsigned     nb  = blocks_for(n, bs);
     unsigned long long* cnt = (unsigned long long*) c->counters.p;
-    c->prof_mark_begin();
+    c->prof_mark_begin(QBT_PROF_PRIMARY);
     /* (DUAL, CPU) in {(0,0), (1,0), (0,1)}: the CPU twin walks the static tree only */
     qbt_dispatch4(cmp, dual, semantics == 1, grid, [&](auto C_, auto D_, auto P_, auto G_) {
         constexpr bool kD = decltype(D_)::value && !decltype(P_)::value;

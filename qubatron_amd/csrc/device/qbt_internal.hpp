@@ -15,8 +15,18 @@
 
 #define QBT_OUT_ALL 0x1FFu
 
-enum { CNT_SHADOW = 0, CNT_HITS, CNT_EXP, CNT_ABORT, CNT_PRED, CNT_QUEUE, CNT_Q2, CNT_Q3, CNT_N };
+enum { CNT_SHADOW = 0, CNT_HITS, CNT_EXP, CNT_ABORT, CNT_PRED, CNT_QUEUE, CNT_EXP_PRIM, CNT_Q3, CNT_N };
 enum { FR_HIT = 0, FR_LEAF, FR_SHADOW, FR_RGBA8, FR_RGBAF, FR_ISP, FR_FLAGS, FR_DYN, FR_EXP, FR_N };
+
+/* per-(light, pixel) shadow status of the wavefront path */
+enum : uint8_t {
+    LS_OCCLUDED = 0, /* exact trace: occluded */
+    LS_LIT      = 1, /* exact trace: visible */
+    LS_DISCARD  = 2, /* exact trace rejected at the root: fragment discarded */
+    LS_SKIPPED  = 3, /* an earlier light discarded the fragment */
+    LS_PRED_OCC = 4, /* surrogate: occluded */
+    LS_PRED_LIT = 5  /* surrogate: visible */
+};
 
 struct DevBuf {
     void*  p   = nullptr;
@@ -107,16 +117,22 @@ struct qbt_ctx {
     int          num_cus         = 256;
     /* wavefront render buffers */
     DevBuf       wb_isp, wb_ms, wb_md, wb_leaf, wb_flags, wb_disc, wb_lstat, wb_exp, wb_sel, wb_items, wb_q, wb_tmp;
+    DevBuf       wb_fb; /* surrogate: per-light lists of the rays traced exactly */
     MlpState*    mlp             = nullptr;
     BuildState*  build           = nullptr;
     /* qbt_profile_begin/end: event pairs around traversal-kernel launches */
     bool                    profiling = false;
     std::vector<hipEvent_t> prof_ev;
+    std::vector<uint8_t>    prof_kind;
     uint32_t                prof_n = 0;
 
-    void prof_mark_begin()
+    void prof_mark_begin(int kind)
     {
-        if (profiling && prof_n < 4096 && ensure_prof_events()) (void) hipEventRecord(prof_ev[2 * prof_n], stream);
+        if (profiling && prof_n < 4096 && ensure_prof_events()) {
+            if (prof_kind.size() <= prof_n) prof_kind.resize(prof_n + 1);
+            prof_kind[prof_n] = (uint8_t) (kind & 3);
+            (void) hipEventRecord(prof_ev[2 * prof_n], stream);
+        }
     }
     void prof_mark_end()
     {
@@ -146,13 +162,27 @@ struct qbt_ctx {
     int      launch_render(const qbt::SceneDev& s, bool dual, bool grid, bool count, const ViewDev& v, const int* spr,
                            FrameDev f, unsigned mask);
     int      launch_sprite(const qbt::SceneDev& s, bool cmp, bool dual, bool grid, const ViewDev& v, int* spr);
-    int      render_surrogate(const qbt_view* views, int nviews, const float* lights, int nlights, unsigned out_mask);
     int      ensure_wave_bufs(size_t npix, int nlights);
+    /* flags: QBT_RENDER_SURROGATE, QBT_RENDER_COLLECT */
     int      render_view_wavefront(const qbt::SceneDev& s, bool cmp, bool dual, bool grid, bool count,
-                                   const ViewDev& v, const int* spr, FrameDev f, unsigned mask);
+                                   const ViewDev& v, const int* spr, FrameDev f, unsigned mask, unsigned flags);
     void     mlp_free();
     void     builder_free();
 };
+
+/* surrogate.hip: the MLP decision pass of one view. For each of the nsel
+ * shaded pixels (items, tile order; count at q[qnsel]) and each light it
+ * writes LS_PRED_* where the prediction is outside the band and appends the
+ * item to that light's exact-trace list (fb_items + l * fb_stride, count
+ * fb_count[l]) otherwise. */
+int qbt_surrogate_decide(qbt_ctx* c, const qbt::SceneDev& s, bool dual, const ViewDev& v, const uint32_t* items,
+                         const unsigned* q, int qnsel, size_t npix, uint32_t* fb_items, size_t fb_stride,
+                         unsigned* fb_count);
+/* surrogate.hip: append training samples (every stride-th shaded
+ * pixel x light of the view, exact label) to the sample store */
+int qbt_collect_view(qbt_ctx* c, const qbt::SceneDev& s, bool dual, const ViewDev& v, const uint32_t* items,
+                     const unsigned* q, int qnsel, size_t npix);
+bool qbt_surrogate_ready(const qbt_ctx* c);
 
 void   qbt_set_hip_error(hipError_t e, const char* what, const char* file, int line);
 bool   qbt_grid_exact(const float b[4], int L);

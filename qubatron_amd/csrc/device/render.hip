@@ -47,8 +47,16 @@ namespace {
 
 constexpr int kSeg = 8;
 
-/* per-view device counters (uint32), zeroed before each view */
-enum { Q_PRIM = 0, Q_SHADOW = kSeg, Q_NSEL = kSeg + kSeg * QBT_MAX_LIGHTS_RENDER, Q_N = Q_NSEL + 1 };
+/* per-view device counters (uint32), zeroed before each view: refill
+ * counters of the primary and per-light shadow launches, the number of
+ * shaded pixels, the per-light exact-trace counts of the surrogate mode */
+enum {
+    Q_PRIM   = 0,
+    Q_SHADOW = kSeg,
+    Q_NSEL   = kSeg + kSeg * QBT_MAX_LIGHTS_RENDER,
+    Q_FB     = Q_NSEL + 1,
+    Q_N      = Q_FB + QBT_MAX_LIGHTS_RENDER
+};
 
 struct PrimOut {
     float4*  isp;    /* entry point of the primary leaf, 0 when none */
@@ -90,14 +98,6 @@ struct Refill {
         return item;
     }
 };
-
-__device__ __forceinline__ bool item_pixel(uint32_t item, int tiles_x, int w, int h, int& px, int& py)
-{
-    uint32_t tile = item >> 6, sub = item & 63u;
-    px            = (int) (tile % (uint32_t) tiles_x) * 8 + (int) (sub & 7u);
-    py            = (int) (tile / (uint32_t) tiles_x) * 8 + (int) (sub >> 3);
-    return px < w && py < h;
-}
 
 template <bool CMP, bool DUAL, bool GRID, bool COUNT>
 __global__ void __launch_bounds__(256) k_primary(SceneDev s, ViewDev v, PrimOut o, unsigned* __restrict__ q,
@@ -187,14 +187,13 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, ViewDev v, PrimOut 
         if (n_hit) atomicAdd(&counters[CNT_HITS], (unsigned long long) n_hit);
         if (n_abort) atomicAdd(&counters[CNT_ABORT], (unsigned long long) n_abort);
         if (COUNT && n_exp) atomicAdd(&counters[CNT_EXP], (unsigned long long) n_exp);
+        if (COUNT && n_exp) atomicAdd(&counters[CNT_EXP_PRIM], (unsigned long long) n_exp);
     }
 }
 
-/* lstat values */
-enum : uint8_t { LS_OCCLUDED = 0, LS_LIT = 1, LS_DISCARD = 2, LS_SKIPPED = 3 };
-
 template <bool CMP, bool DUAL, bool GRID, bool COUNT>
 __global__ void __launch_bounds__(256) k_shadow(SceneDev s, ViewDev v, int l, const uint32_t* __restrict__ sel_items,
+                                                const unsigned* __restrict__ nsel,
                                                 const float4* __restrict__ isp, uint8_t* __restrict__ lstat,
                                                 uint8_t* __restrict__ disc, unsigned* __restrict__ q,
                                                 unsigned refill, int* __restrict__ exp,
@@ -204,7 +203,7 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, ViewDev v, int l, co
     const Stack st      = make_stack(lds, s.maxlevel, DUAL);
     const int   lane    = threadIdx.x & 63;
     const int   tiles_x = (v.w + 7) >> 3;
-    Refill      rf(q[Q_NSEL]);
+    Refill      rf(*nsel);
     bool        active = false;
     uint32_t    pix = 0;
     const float Lx = v.light[l][0], Ly = v.light[l][1], Lz = v.light[l][2];
@@ -311,7 +310,8 @@ __global__ void __launch_bounds__(256) k_shade(SceneDev s, ViewDev v, const int*
                     flg |= QBT_PX_DISCARD;
                     break;
                 }
-                float bit = ls == LS_LIT ? 1.0f : 0.0f;
+                if (ls >= LS_PRED_OCC) flg |= QBT_PX_PREDICTED;
+                float bit = (ls == LS_LIT || ls == LS_PRED_LIT) ? 1.0f : 0.0f;
                 if (bit != 0.0f) sbits |= 1u << l;
                 /* normalize(-lghtv) with lghtv = isp - light (octree_fsh.c:431, 436) */
                 float lv[3] = {isp.x - v.light[l][0], isp.y - v.light[l][1], isp.z - v.light[l][2]};
@@ -376,12 +376,6 @@ __global__ void k_zero_u8(uint8_t* p, size_t n)
     if (i < n) p[i] = 0;
 }
 
-__global__ void k_zero_i32(int* p, size_t n)
-{
-    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
-
 inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned) ((n + bs - 1) / bs); }
 
 template <class K>
@@ -409,7 +403,8 @@ int qbt_ctx::ensure_wave_bufs(size_t npix, int nlights)
 }
 
 template <bool CMP, bool DUAL, bool GRID, bool COUNT>
-static int render_view_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const int* spr, FrameDev f, unsigned mask)
+static int render_view_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const int* spr, FrameDev f, unsigned mask,
+                         unsigned flags)
 {
     hipStream_t  st      = c->stream;
     const size_t npix    = (size_t) v.w * v.h;
@@ -417,9 +412,11 @@ static int render_view_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const 
     const size_t items   = (size_t) tiles_x * tiles_y * 64;
     int          rc;
     if ((rc = c->ensure_wave_bufs(npix, v.nlights))) return rc;
+    const bool surrogate = (flags & QBT_RENDER_SURROGATE) && v.nlights > 0;
     if ((rc = c->dev_alloc(c->wb_sel, items)) || (rc = c->dev_alloc(c->wb_items, items * 4)) ||
         (rc = c->dev_alloc(c->wb_q, sizeof(unsigned) * Q_N)))
         return rc;
+    if (surrogate && (rc = c->dev_alloc(c->wb_fb, items * 4 * (size_t) v.nlights))) return rc;
     size_t tmp = 0;
     HIPCHK(rocprim::select(nullptr, tmp, rocprim::counting_iterator<uint32_t>(0u), (const uint8_t*) c->wb_sel.p,
                            (uint32_t*) c->wb_items.p, (unsigned*) c->wb_q.p + Q_NSEL, items, st));
@@ -446,7 +443,7 @@ static int render_view_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const 
     {
         auto kp = k_primary<CMP, DUAL, GRID, COUNT>;
         int  g  = persistent_grid(c, kp, bs, lds);
-        c->prof_mark_begin();
+        c->prof_mark_begin(QBT_PROF_PRIMARY);
         kp<<<g, bs, lds, st>>>(s, v, po, q, refill, cnt);
         c->prof_mark_end();
         HIPCHK(hipGetLastError());
@@ -454,28 +451,37 @@ static int render_view_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const 
     size_t tb = tmp;
     HIPCHK(rocprim::select(c->wb_tmp.p, tb, rocprim::counting_iterator<uint32_t>(0u), (const uint8_t*) c->wb_sel.p,
                            (uint32_t*) c->wb_items.p, q + Q_NSEL, items, st));
+    if (surrogate && (rc = qbt_surrogate_decide(c, s, DUAL, v, (const uint32_t*) c->wb_items.p, q, Q_NSEL, npix,
+                                                (uint32_t*) c->wb_fb.p, items, q + Q_FB)))
+        return rc;
     for (int l = 0; l < v.nlights; l++) {
         auto ks = k_shadow<CMP, DUAL, GRID, COUNT>;
         int  g  = persistent_grid(c, ks, bs, lds);
-        c->prof_mark_begin();
-        ks<<<g, bs, lds, st>>>(s, v, l, (const uint32_t*) c->wb_items.p, (const float4*) c->wb_isp.p,
+        /* exact mode: every shaded pixel; surrogate mode: this light's unsure ones */
+        const uint32_t* li = surrogate ? (const uint32_t*) c->wb_fb.p + (size_t) l * items : (const uint32_t*) c->wb_items.p;
+        const unsigned* ln = surrogate ? q + Q_FB + l : q + Q_NSEL;
+        c->prof_mark_begin(QBT_PROF_SHADOW);
+        ks<<<g, bs, lds, st>>>(s, v, l, li, ln, (const float4*) c->wb_isp.p,
                                (uint8_t*) c->wb_lstat.p + (size_t) l * npix, (uint8_t*) c->wb_disc.p, q, refill,
                                po.exp, cnt);
         c->prof_mark_end();
         HIPCHK(hipGetLastError());
     }
+    if ((flags & QBT_RENDER_COLLECT) && !surrogate && v.nlights > 0 &&
+        (rc = qbt_collect_view(c, s, DUAL, v, (const uint32_t*) c->wb_items.p, q, Q_NSEL, npix)))
+        return rc;
     k_shade<DUAL><<<nblk(npix), 256, 0, st>>>(s, v, spr, po, (const uint8_t*) c->wb_lstat.p, npix, f, mask);
     HIPCHK(hipGetLastError());
-    c->last.launches += 3 + v.nlights;
+    c->last.launches += 3 + v.nlights + (surrogate ? 1 : 0);
     return QBT_OK;
 }
 
 int qbt_ctx::render_view_wavefront(const SceneDev& s, bool cmp, bool dual, bool grid, bool count, const ViewDev& v,
-                                   const int* spr, FrameDev f, unsigned mask)
+                                   const int* spr, FrameDev f, unsigned mask, unsigned flags)
 {
     if (v.w <= 0 || v.h <= 0) return QBT_OK;
     return qbt_dispatch4(cmp, dual, grid, count, [&](auto C_, auto D_, auto G_, auto N_) {
         return render_view_t<decltype(C_)::value, decltype(D_)::value, decltype(G_)::value, decltype(N_)::value>(
-            this, s, v, spr, f, mask);
+            this, s, v, spr, f, mask, flags);
     });
 }

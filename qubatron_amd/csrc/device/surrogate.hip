@@ -1,24 +1,965 @@
 /*
- * surrogate.hip - shadow-visibility MLP (forward, backward, Adam) and the
- * surrogate render path.
+ * surrogate.hip - the learned shadow-visibility surrogate (no reference
+ * counterpart: the reference traces every shadow ray,
+ * src/qubatron/shaders/octree_fsh.c:426-434).
+ *
+ *   training   feature kernel -> four MFMA GEMMs (bias + ReLU + bf16 in the
+ *              epilogue) -> loss / output gradient -> per layer: weight
+ *              gradient GEMM (split-K, fp32 atomics), bias row sums, the
+ *              masked input gradient GEMM -> Adam on fp32 master weights ->
+ *              bf16 repack
+ *   inference  one fused kernel: weights in LDS, 16 queries per wave, the
+ *              four layers chained in registers (mlp.hpp)
+ *   render     the decision pass of QBT_RENDER_SURROGATE: predicted bits
+ *              outside the band, the rest appended to per-light lists that
+ *              the exact shadow kernel traces
+ *   collect    training samples from exact renders (QBT_RENDER_COLLECT)
+ *
+ * Activations are feature-major ([feature][sample], bf16) so that every
+ * layer is C = W H with the batch as the N dimension.
  */
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+#include <math.h>
+
+#include <rocprim/rocprim.hpp>
+
+#include "mlp.hpp"
+#include "pixel.hpp"
 #include "qbt.h"
 #include "qbt_internal.hpp"
+#include "trace.hpp"
+
+#pragma clang fp contract(off)
+
+using namespace qbt;
+
+#define HIPCHK(x) QBT_HIPCHK(x)
+
+static_assert(kMlpParams == QBT_MLP_PARAMS, "parameter count");
+static_assert(kMlpIn == QBT_MLP_INPUTS, "input count");
+
+/* training sample: hit position, normal, light position, label (1 = lit) */
+struct RawSample {
+    float isp[3], nrm[3], light[3], label;
+};
+static_assert(sizeof(RawSample) == 40, "sample record");
+
+/* fp32 gradient layout: packed weight matrices (as kP*), then the biases */
+constexpr int kGB1 = kPackN, kGB2 = kGB1 + 64, kGB3 = kGB2 + 64, kGB4 = kGB3 + 64, kGradN = kGB4 + 1;
+/* biases in LDS for the fused kernels */
+constexpr int kLB1 = 0, kLB2 = 64, kLB3 = 128, kLB4 = 192, kLBN = 196;
 
 struct MlpState {
-    int unused;
+    DevBuf   params, m, v, grads, packed, ctr; /* ctr: collected, lit (u64) */
+    DevBuf   samples, perm, keys, vals, sort_tmp;
+    DevBuf   X, H1, H2, H3, L16, dZ4, dZ3, dZ2, dZ1, lossbuf, io_x, io_p, io_y, io_a, io_c, flat;
+    size_t   cap = 0;
+    uint32_t stride = 1, phase = 0;
+    int      t     = 0; /* Adam steps taken */
+    float    band_lo = 0.1f, band_hi = 0.9f;
+    bool     ready   = false; /* parameters initialised */
+    uint64_t epoch   = 0;
+    size_t   pos     = 0, perm_n = 0; /* next batch in the permutation of perm_n samples */
 };
+
+namespace {
+
+inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned) ((n + bs - 1) / bs); }
+
+__global__ void k_fill_f32(float* p, size_t n, float v)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int n)
+{
+    int i = threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+/* splitmix64 finaliser of a counter */
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* He-uniform weights (bound sqrt(6 / fan_in)), zero biases */
+__global__ void k_mlp_init(float* P, uint64_t seed)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kMlpParams) return;
+    float a = 0.0f;
+    if (i < kB1) a = sqrtf(6.0f / 42.0f);
+    else if ((i >= kW2 && i < kB2) || (i >= kW3 && i < kB3) || (i >= kW4 && i < kB4)) a = sqrtf(6.0f / 64.0f);
+    float u = (float) (mix64(seed * 0x9E3779B97F4A7C15ull + (uint64_t) i) >> 40) * (1.0f / 16777216.0f);
+    P[i]    = a > 0.0f ? (2.0f * u - 1.0f) * a : 0.0f;
+}
+
+/* fp32 parameters -> padded bf16 weight matrices */
+__global__ void k_mlp_pack(const float* __restrict__ P, uint16_t* __restrict__ W)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kPackN) return;
+    float v = 0.0f;
+    if (i < kP2) {
+        int o = i / 48, c = i % 48;
+        v     = c < kMlpIn ? P[kW1 + o * kMlpIn + c] : 0.0f;
+    } else if (i < kP3) {
+        v = P[kW2 + (i - kP2)];
+    } else if (i < kP4) {
+        v = P[kW3 + (i - kP3)];
+    } else {
+        int r = (i - kP4) / 64, c = (i - kP4) % 64;
+        v     = r == 0 ? P[kW4 + c] : 0.0f;
+    }
+    W[i] = f2bf(v);
+}
+
+/* matrix view: element (r, c) at p[r * rs + c * cs] */
+struct MatBF {
+    const uint16_t* p;
+    long            rs, cs;
+    __device__ __forceinline__ short raw(long r, long c) const { return (short) p[r * rs + c * cs]; }
+};
+
+enum { EPI_F32 = 0, EPI_ATOMIC = 1, EPI_BIAS_RELU = 2, EPI_MASK = 3 };
+
+struct Epi {
+    int          mode;
+    float*       of; /* EPI_F32 / EPI_ATOMIC: fp32 output */
+    uint16_t*    ob; /* EPI_BIAS_RELU / EPI_MASK: bf16 output */
+    long         ors, ocs;
+    const float* bias; /* EPI_BIAS_RELU: per row */
+    MatBF        mask; /* EPI_MASK: keep where mask > 0 (the ReLU output) */
+};
+
+/* C (M x N) = A (M x K) B (K x N), M, N, K multiples of 16. One wave per
+ * 16 x 16 output tile and K slice of kchunk; slices combine by atomics. */
+__global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, int kchunk, MatBF A, MatBF B, Epi E)
+{
+    const int  lane  = threadIdx.x & 63;
+    const long wave  = ((long) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int  tm    = M / 16, tn = N / 16, nsplit = (K + kchunk - 1) / kchunk;
+    const long ntile = (long) tm * tn;
+    if (wave >= ntile * nsplit) return;
+    const int t = (int) (wave % ntile), split = (int) (wave / ntile);
+    const int mt = t % tm, nt = t / tm;
+    const int r = lane & 15, g = lane >> 4;
+    f32x4     acc = {0.f, 0.f, 0.f, 0.f};
+    const int k1  = min(K, (split + 1) * kchunk);
+    for (int k0 = split * kchunk; k0 < k1; k0 += 16) {
+        bf16x4 a, b;
+        for (int e = 0; e < 4; e++) {
+            a[e] = A.raw(16 * mt + r, k0 + 4 * g + e);
+            b[e] = B.raw(k0 + 4 * g + e, 16 * nt + r);
+        }
+        acc = mfma16(a, b, acc);
+    }
+    for (int e = 0; e < 4; e++) {
+        const long row = 16 * mt + 4 * g + e, col = 16 * nt + r;
+        const long o   = row * E.ors + col * E.ocs;
+        float      v   = acc[e];
+        switch (E.mode) {
+        case EPI_F32: E.of[o] = v; break;
+        case EPI_ATOMIC: atomicAdd(&E.of[o], v); break;
+        case EPI_BIAS_RELU:
+            v       = v + E.bias[row];
+            E.ob[o] = f2bf(v > 0.0f ? v : 0.0f);
+            break;
+        default: E.ob[o] = E.mask.raw(row, col) > 0 ? f2bf(v) : (uint16_t) 0; break;
+        }
+    }
+}
+
+/* the 48 (padded) features of a query, bf16, feature-major X[48][B] */
+__device__ __forceinline__ void store_features(uint16_t* __restrict__ X, int B, int s, const float u[3],
+                                               const float nrm[3], const float lu[3])
+{
+    for (int f = 0; f < kMlpInPad; f++) X[(size_t) f * B + s] = f2bf(mlp_feature(f, u, nrm, lu));
+}
+
+/* minibatch from the sample store: X, labels Y */
+__global__ void k_batch_features(const RawSample* __restrict__ S, const uint32_t* __restrict__ perm, size_t off,
+                                 int B, float bx, float by, float bz, float bw, uint16_t* __restrict__ X,
+                                 float* __restrict__ Y)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= B) return;
+    const RawSample q = S[perm[off + (size_t) s]];
+    float           u[3], lu[3];
+    mlp_scale(q.isp, bx, by, bz, bw, u);
+    mlp_scale(q.light, bx, by, bz, bw, lu);
+    store_features(X, B, s, u, q.nrm, lu);
+    Y[s] = q.label;
+}
+
+/* supplied feature rows (n x 42 fp32) -> X; columns n..B-1 zero */
+__global__ void k_copy_features(const float* __restrict__ x, int B, int n, uint16_t* __restrict__ X,
+                                const float* __restrict__ yin, float* __restrict__ Y)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= B) return;
+    for (int f = 0; f < kMlpInPad; f++)
+        X[(size_t) f * B + s] = f2bf((s < n && f < kMlpIn) ? x[(size_t) s * kMlpIn + f] : 0.0f);
+    if (Y) Y[s] = (s < n && yin) ? yin[s] : 0.0f;
+}
+
+/* output layer: logit = L16[0][s] + b4, p = sigmoid; summed BCE into *loss;
+ * dZ4 row 0 = bf16((p - y) / n), rows 1..15 zero */
+__global__ void k_mlp_loss(const float* __restrict__ L16, const float* __restrict__ P, const float* __restrict__ Y,
+                           int B, int n, uint16_t* __restrict__ dZ4, float* __restrict__ pout, double* __restrict__ loss)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= B) return;
+    for (int r = 1; r < 16; r++) dZ4[(size_t) r * B + s] = 0;
+    if (s >= n) {
+        dZ4[s] = 0;
+        return;
+    }
+    float logit = L16[s] + P[kB4];
+    float p     = 1.0f / (1.0f + expf(-logit));
+    if (pout) pout[s] = p;
+    float  y  = Y[s];
+    float  pc = fminf(fmaxf(p, 1e-7f), 1.0f - 1e-7f);
+    double l  = y > 0.5f ? -log((double) pc) : -log(1.0 - (double) pc);
+    atomicAdd(loss, l);
+    dZ4[s] = f2bf((p - y) / (float) n);
+}
+
+/* g[r] += sum over s of Z[r][s] (bf16 rows of length B), one block per row */
+__global__ void k_rowsum(const uint16_t* __restrict__ Z, int B, float* __restrict__ g)
+{
+    __shared__ float red[256];
+    const int        r   = blockIdx.x;
+    float            acc = 0.0f;
+    for (int s = threadIdx.x; s < B; s += blockDim.x) acc += bf2f(Z[(size_t) r * B + s]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+        if ((int) threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) g[r] += red[0];
+}
+
+/* flat parameter index -> gradient-layout index */
+__device__ __forceinline__ int grad_index(int i)
+{
+    if (i < kB1) return kP1 + (i / kMlpIn) * 48 + (i % kMlpIn);
+    if (i < kW2) return kGB1 + (i - kB1);
+    if (i < kB2) return kP2 + (i - kW2);
+    if (i < kW3) return kGB2 + (i - kB2);
+    if (i < kB3) return kP3 + (i - kW3);
+    if (i < kW4) return kGB3 + (i - kB3);
+    if (i < kB4) return kP4 + (i - kW4);
+    return kGB4;
+}
+
+/* Adam (Kingma & Ba 2015), bias-corrected moments, fp32 master weights */
+__global__ void k_mlp_adam(float* __restrict__ P, const float* __restrict__ G, float* __restrict__ m,
+                           float* __restrict__ v, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kMlpParams) return;
+    float g  = G[grad_index(i)];
+    m[i]     = b1 * m[i] + (1.0f - b1) * g;
+    v[i]     = b2 * v[i] + (1.0f - b2) * g * g;
+    float mh = m[i] / c1;
+    float vh = v[i] / c2;
+    P[i]     = P[i] - lr * mh / (sqrtf(vh) + eps);
+}
+
+__global__ void k_grads_to_flat(const float* __restrict__ G, float* __restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < kMlpParams) out[i] = G[grad_index(i)];
+}
+
+/* random sort keys for the sample shuffle */
+__global__ void k_perm_keys(uint32_t* keys, uint32_t* vals, size_t n, uint64_t seed)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = (uint32_t) mix64(seed * 0x9E3779B97F4A7C15ull + i);
+    vals[i] = (uint32_t) i;
+}
+
+/* ------------------------------------------------------------------------ */
+/* fused inference                                                          */
+/* ------------------------------------------------------------------------ */
+
+struct InferArgs {
+    const uint16_t* W; /* packed bf16 weights */
+    const float*    P; /* fp32 parameters (the biases are read) */
+    float           bx, by, bz, bw;
+};
+
+__device__ __forceinline__ void load_weights_lds(const InferArgs& ia, uint16_t* Ws, float* Bs)
+{
+    for (int i = threadIdx.x; i < kPackN / 2; i += blockDim.x)
+        reinterpret_cast<uint32_t*>(Ws)[i] = reinterpret_cast<const uint32_t*>(ia.W)[i];
+    for (int i = threadIdx.x; i < kLBN; i += blockDim.x) {
+        float b = 0.0f;
+        if (i < 64) b = ia.P[kB1 + i];
+        else if (i < 128) b = ia.P[kB2 + i - 64];
+        else if (i < 192) b = ia.P[kB3 + i - 128];
+        else if (i == kLB4) b = ia.P[kB4];
+        Bs[i] = b;
+    }
+    __syncthreads();
+}
+
+/* B fragments of the input of 16 queries: lane l holds features 16 kb +
+ * 4 (l / 16) + e of query l % 16 */
+__device__ __forceinline__ void query_frag(const float u[3], const float nrm[3], const float lu[3], int g,
+                                           bf16x4 xin[3])
+{
+    for (int kb = 0; kb < 3; kb++) {
+        float f[4];
+        for (int e = 0; e < 4; e++) f[e] = mlp_feature(16 * kb + 4 * g + e, u, nrm, lu);
+        xin[kb] = pack4(f[0], f[1], f[2], f[3]);
+    }
+}
+
+/* logit of the 16 queries of the wave (valid in lanes 0..15, one per query) */
+__device__ __forceinline__ float mlp_logit(const uint16_t* __restrict__ Ws, const float* __restrict__ Bs,
+                                           const bf16x4 xin[3], int lane)
+{
+    const int r = lane & 15, g = lane >> 4;
+    bf16x4    h[4], h2[4];
+    for (int mt = 0; mt < 4; mt++) { /* layer 1: 64 x 48 */
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int kb = 0; kb < 3; kb++) {
+            bf16x4 a = *reinterpret_cast<const bf16x4*>(Ws + kP1 + (16 * mt + r) * 48 + 16 * kb + 4 * g);
+            acc      = mfma16(a, xin[kb], acc);
+        }
+        float z[4];
+        for (int e = 0; e < 4; e++) {
+            float v = acc[e] + Bs[kLB1 + 16 * mt + 4 * g + e];
+            z[e]    = v > 0.0f ? v : 0.0f;
+        }
+        h[mt] = pack4(z[0], z[1], z[2], z[3]);
+    }
+    for (int layer = 0; layer < 2; layer++) { /* layers 2 and 3: 64 x 64 */
+        const int pw = layer == 0 ? kP2 : kP3, pb = layer == 0 ? kLB2 : kLB3;
+        for (int mt = 0; mt < 4; mt++) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (int kb = 0; kb < 4; kb++) {
+                bf16x4 a = *reinterpret_cast<const bf16x4*>(Ws + pw + (16 * mt + r) * 64 + 16 * kb + 4 * g);
+                acc      = mfma16(a, h[kb], acc);
+            }
+            float z[4];
+            for (int e = 0; e < 4; e++) {
+                float v = acc[e] + Bs[pb + 16 * mt + 4 * g + e];
+                z[e]    = v > 0.0f ? v : 0.0f;
+            }
+            h2[mt] = pack4(z[0], z[1], z[2], z[3]);
+        }
+        for (int mt = 0; mt < 4; mt++) h[mt] = h2[mt];
+    }
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f}; /* layer 4: row 0 of 16 x 64 */
+    for (int kb = 0; kb < 4; kb++) {
+        bf16x4 a = *reinterpret_cast<const bf16x4*>(Ws + kP4 + r * 64 + 16 * kb + 4 * g);
+        acc      = mfma16(a, h[kb], acc);
+    }
+    return acc[0] + Bs[kLB4]; /* D[0][query]: lanes 0..15, element 0 */
+}
+
+/* probabilities of n queries. RAW: inputs from isp/nrm/light (n x 3 each);
+ * else from the feature matrix X[48][B] */
+template <bool RAW>
+__global__ void __launch_bounds__(256) k_mlp_infer(InferArgs ia, const uint16_t* __restrict__ X, int B,
+                                                   const float* __restrict__ isp, const float* __restrict__ nrm,
+                                                   const float* __restrict__ light, int n, float* __restrict__ pout)
+{
+    __shared__ uint16_t Ws[kPackN];
+    __shared__ float    Bs[kLBN];
+    load_weights_lds(ia, Ws, Bs);
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    for (int q0 = wave * 16; q0 < n; q0 += nw * 16) {
+        const int s = q0 + r;
+        bf16x4    xin[3];
+        if (RAW) {
+            float u[3] = {0.f, 0.f, 0.f}, nn[3] = {0.f, 0.f, 0.f}, lu[3] = {0.f, 0.f, 0.f};
+            if (s < n) {
+                const float p3[3] = {isp[3 * s], isp[3 * s + 1], isp[3 * s + 2]};
+                const float l3[3] = {light[3 * s], light[3 * s + 1], light[3 * s + 2]};
+                mlp_scale(p3, ia.bx, ia.by, ia.bz, ia.bw, u);
+                mlp_scale(l3, ia.bx, ia.by, ia.bz, ia.bw, lu);
+                nn[0] = nrm[3 * s], nn[1] = nrm[3 * s + 1], nn[2] = nrm[3 * s + 2];
+            }
+            query_frag(u, nn, lu, g, xin);
+        } else {
+            for (int kb = 0; kb < 3; kb++)
+                for (int e = 0; e < 4; e++)
+                    xin[kb][e] = s < n ? (short) X[(size_t) (16 * kb + 4 * g + e) * B + s] : (short) 0;
+        }
+        const float logit = mlp_logit(Ws, Bs, xin, lane);
+        if (g == 0 && s < n) pout[s] = 1.0f / (1.0f + expf(-logit));
+    }
+}
+
+/* The decision pass: for each (shaded pixel, light) query, 16 per wave,
+ * predicted bits outside the band, the others appended to the light's
+ * exact-trace list (one atomic per light and wave). */
+template <bool DUAL>
+__global__ void __launch_bounds__(256) k_mlp_shadow(InferArgs ia, SceneDev s, ViewDev v,
+                                                    const uint32_t* __restrict__ items, const unsigned* __restrict__ q,
+                                                    int qnsel, const float4* __restrict__ isp, const int* __restrict__ ms,
+                                                    const int* __restrict__ md, uint8_t* __restrict__ lstat,
+                                                    size_t npix, uint32_t* __restrict__ fb_items, size_t fb_stride,
+                                                    unsigned* __restrict__ fb_count, float lo_logit, float hi_logit,
+                                                    unsigned long long* __restrict__ counters)
+{
+    __shared__ uint16_t Ws[kPackN];
+    __shared__ float    Bs[kLBN];
+    load_weights_lds(ia, Ws, Bs);
+    const uint32_t nq      = q[qnsel];
+    const uint32_t total   = nq * (uint32_t) v.nlights;
+    const int      lane    = threadIdx.x & 63;
+    const int      wave    = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int      nw      = (gridDim.x * blockDim.x) >> 6;
+    const int      r       = lane & 15, g = lane >> 4;
+    const int      tiles_x = (v.w + 7) >> 3;
+    unsigned       n_pred  = 0;
+    for (uint32_t q0 = (uint32_t) wave * 16u; q0 < total; q0 += (uint32_t) nw * 16u) {
+        const uint32_t qi    = q0 + (uint32_t) r;
+        const bool     valid = qi < total;
+        int            l     = 0;
+        uint32_t       pix = 0, item = 0;
+        float          u[3] = {0.f, 0.f, 0.f}, nrm[3] = {0.f, 0.f, 0.f}, lu[3] = {0.f, 0.f, 0.f};
+        if (valid) {
+            l    = (int) (qi / nq);
+            item = items[qi - (uint32_t) l * nq];
+            int px, py;
+            item_pixel(item, tiles_x, v.w, v.h, px, py);
+            pix                = (uint32_t) py * (uint32_t) v.w + (uint32_t) px;
+            const float4 ip    = isp[pix];
+            const float  p3[3] = {ip.x, ip.y, ip.z};
+            mlp_scale(p3, ia.bx, ia.by, ia.bz, ia.bw, u);
+            float col[4], n4[4];
+            fetch_attr(s.astat, ms[pix], col, n4);
+            if (DUAL && md[pix] > 0) fetch_attr(s.adyn, md[pix], col, n4);
+            nrm[0] = n4[0], nrm[1] = n4[1], nrm[2] = n4[2];
+            mlp_scale(v.light[l], ia.bx, ia.by, ia.bz, ia.bw, lu);
+        }
+        bf16x4 xin[3];
+        query_frag(u, nrm, lu, g, xin);
+        const float logit  = mlp_logit(Ws, Bs, xin, lane);
+        bool        unsure = false;
+        if (g == 0 && valid) {
+            if (logit < lo_logit) lstat[(size_t) l * npix + pix] = LS_PRED_OCC, n_pred++;
+            else if (logit > hi_logit) lstat[(size_t) l * npix + pix] = LS_PRED_LIT, n_pred++;
+            else unsure = true;
+        }
+        uint64_t pending = __ballot(unsure);
+        while (pending) {
+            const int      leader = __ffsll((unsigned long long) pending) - 1;
+            const int      L      = __shfl(l, leader, 64);
+            const uint64_t mine   = __ballot(unsure && l == L);
+            uint32_t       base   = 0;
+            if (lane == leader) base = atomicAdd(&fb_count[L], (unsigned) __popcll(mine));
+            base = __shfl(base, leader, 64);
+            if (unsure && l == L) {
+                const uint32_t rank = (uint32_t) __popcll(mine & ((1ull << lane) - 1ull));
+                fb_items[(size_t) L * fb_stride + base + rank] = item;
+            }
+            pending &= ~mine;
+        }
+    }
+    n_pred = wave_sum(n_pred);
+    if (lane == 0 && n_pred) atomicAdd(&counters[CNT_PRED], (unsigned long long) n_pred);
+}
+
+/* training samples of an exact render: queries stride * k + phase of the
+ * (light, shaded pixel) sequence, label = the exact visibility */
+template <bool DUAL>
+__global__ void k_collect(SceneDev s, ViewDev v, const uint32_t* __restrict__ items, const unsigned* __restrict__ q,
+                          int qnsel, const float4* __restrict__ isp, const int* __restrict__ ms,
+                          const int* __restrict__ md, const uint8_t* __restrict__ lstat, size_t npix, uint32_t stride,
+                          uint32_t phase, RawSample* __restrict__ out, size_t cap,
+                          unsigned long long* __restrict__ ctr)
+{
+    const uint64_t nq = q[qnsel];
+    const uint64_t qi = ((uint64_t) blockIdx.x * blockDim.x + threadIdx.x) * stride + phase;
+    if (qi >= nq * (uint64_t) v.nlights) return;
+    const int      l       = (int) (qi / nq);
+    const uint32_t item    = items[qi - (uint64_t) l * nq];
+    const int      tiles_x = (v.w + 7) >> 3;
+    int            px, py;
+    item_pixel(item, tiles_x, v.w, v.h, px, py);
+    const uint32_t pix = (uint32_t) py * (uint32_t) v.w + (uint32_t) px;
+    const uint8_t  ls  = lstat[(size_t) l * npix + pix];
+    if (ls != LS_LIT && ls != LS_OCCLUDED) return;
+    const unsigned long long slot = atomicAdd(&ctr[0], 1ull);
+    if (slot >= cap) return;
+    RawSample    r;
+    const float4 ip = isp[pix];
+    r.isp[0] = ip.x, r.isp[1] = ip.y, r.isp[2] = ip.z;
+    float col[4], n4[4];
+    fetch_attr(s.astat, ms[pix], col, n4);
+    if (DUAL && md[pix] > 0) fetch_attr(s.adyn, md[pix], col, n4);
+    r.nrm[0] = n4[0], r.nrm[1] = n4[1], r.nrm[2] = n4[2];
+    r.light[0] = v.light[l][0], r.light[1] = v.light[l][1], r.light[2] = v.light[l][2];
+    r.label    = ls == LS_LIT ? 1.0f : 0.0f;
+    out[slot]  = r;
+    if (ls == LS_LIT) atomicAdd(&ctr[1], 1ull);
+}
+
+__global__ void k_unpack_samples(const RawSample* __restrict__ S, size_t start, size_t n, float* __restrict__ out)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const RawSample r = S[start + i];
+    float*          o = out + 10 * i;
+    for (int k = 0; k < 3; k++) o[k] = r.isp[k], o[3 + k] = r.nrm[k], o[6 + k] = r.light[k];
+    o[9] = r.label;
+}
+
+} // namespace
+
+/* ------------------------------------------------------------------------ */
+/* host side                                                                */
+/* ------------------------------------------------------------------------ */
 
 void qbt_ctx::mlp_free()
 {
+    if (!mlp) return;
+    MlpState* m = mlp;
+    for (DevBuf* b : {&m->params, &m->m, &m->v, &m->grads, &m->packed, &m->ctr, &m->samples, &m->perm, &m->keys,
+                      &m->vals, &m->sort_tmp, &m->X, &m->H1, &m->H2, &m->H3, &m->L16, &m->dZ4, &m->dZ3, &m->dZ2,
+                      &m->dZ1, &m->lossbuf, &m->io_x, &m->io_p, &m->io_y, &m->io_a, &m->io_c, &m->flat})
+        dev_free(*b);
     delete mlp;
     mlp = nullptr;
 }
 
-int qbt_ctx::render_surrogate(const qbt_view* views, int nviews, const float* lights, int nlights, unsigned out_mask)
+bool qbt_surrogate_ready(const qbt_ctx* c) { return c && c->mlp && c->mlp->ready; }
+
+static int mlp_state(qbt_ctx* c, MlpState** out)
 {
-    (void) views, (void) nviews, (void) lights, (void) nlights, (void) out_mask;
-    return QBT_ENOTSUP;
+    if (!c->mlp) {
+        c->mlp = new (std::nothrow) MlpState();
+        if (!c->mlp) return QBT_ENOMEM;
+        MlpState* m = c->mlp;
+        int       rc;
+        if ((rc = c->dev_alloc(m->params, kMlpParams * 4)) || (rc = c->dev_alloc(m->m, kMlpParams * 4)) ||
+            (rc = c->dev_alloc(m->v, kMlpParams * 4)) || (rc = c->dev_alloc(m->grads, kGradN * 4)) ||
+            (rc = c->dev_alloc(m->packed, kPackN * 2)) || (rc = c->dev_alloc(m->lossbuf, 64)) ||
+            (rc = c->dev_alloc(m->ctr, 16)) || (rc = c->dev_alloc(m->flat, kMlpParams * 4))) {
+            c->mlp_free();
+            return rc;
+        }
+        k_fill_u64<<<1, 64, 0, c->stream>>>((unsigned long long*) m->ctr.p, 2);
+        HIPCHK(hipGetLastError());
+    }
+    *out = c->mlp;
+    return QBT_OK;
+}
+
+static int mlp_repack(qbt_ctx* c)
+{
+    k_mlp_pack<<<nblk(kPackN), 256, 0, c->stream>>>((const float*) c->mlp->params.p, (uint16_t*) c->mlp->packed.p);
+    HIPCHK(hipGetLastError());
+    return QBT_OK;
+}
+
+static int mlp_reset_adam(qbt_ctx* c)
+{
+    MlpState* m = c->mlp;
+    k_fill_f32<<<nblk(kMlpParams), 256, 0, c->stream>>>((float*) m->m.p, kMlpParams, 0.0f);
+    k_fill_f32<<<nblk(kMlpParams), 256, 0, c->stream>>>((float*) m->v.p, kMlpParams, 0.0f);
+    HIPCHK(hipGetLastError());
+    m->t = 0;
+    return QBT_OK;
+}
+
+extern "C" int qbt_surrogate_init(qbt_ctx* c, uint64_t seed)
+{
+    if (!c) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    MlpState* m;
+    int       rc = mlp_state(c, &m);
+    if (rc) return rc;
+    k_mlp_init<<<nblk(kMlpParams), 256, 0, c->stream>>>((float*) m->params.p, seed);
+    HIPCHK(hipGetLastError());
+    if ((rc = mlp_reset_adam(c)) || (rc = mlp_repack(c))) return rc;
+    m->ready = true;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return QBT_OK;
+}
+
+extern "C" int qbt_surrogate_set_params(qbt_ctx* c, const float* params, size_t n)
+{
+    if (!c || !params || n != (size_t) kMlpParams) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    MlpState* m;
+    int       rc = mlp_state(c, &m);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(m->params.p, params, kMlpParams * 4, hipMemcpyHostToDevice, c->stream));
+    if ((rc = mlp_reset_adam(c)) || (rc = mlp_repack(c))) return rc;
+    m->ready = true;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return QBT_OK;
+}
+
+extern "C" int qbt_surrogate_get_params(qbt_ctx* c, float* params, size_t n)
+{
+    if (!c || !params || n != (size_t) kMlpParams || !qbt_surrogate_ready(c)) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(params, c->mlp->params.p, kMlpParams * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return QBT_OK;
+}
+
+extern "C" int qbt_surrogate_set_band(qbt_ctx* c, float lo, float hi)
+{
+    if (!c || !(lo > 0.0f) || !(hi < 1.0f) || !(lo <= hi)) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    MlpState* m;
+    int       rc = mlp_state(c, &m);
+    if (rc) return rc;
+    m->band_lo = lo;
+    m->band_hi = hi;
+    return QBT_OK;
+}
+
+extern "C" int qbt_surrogate_reserve(qbt_ctx* c, size_t nsamples, uint32_t stride)
+{
+    if (!c || nsamples == 0 || nsamples > ((size_t) 1 << 31) || stride == 0) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    MlpState* m;
+    int       rc = mlp_state(c, &m);
+    if (rc) return rc;
+    if ((rc = c->dev_alloc(m->samples, nsamples * sizeof(RawSample)))) return rc;
+    m->cap    = nsamples;
+    m->stride = stride;
+    m->phase  = 0;
+    m->perm_n = 0;
+    k_fill_u64<<<1, 64, 0, c->stream>>>((unsigned long long*) m->ctr.p, 2);
+    HIPCHK(hipGetLastError());
+    return QBT_OK;
+}
+
+static int mlp_counts(qbt_ctx* c, size_t* n, size_t* lit)
+{
+    unsigned long long h[2] = {0ull, 0ull};
+    MlpState*          m    = c->mlp;
+    if (m && m->cap) {
+        HIPCHK(hipMemcpyAsync(h, m->ctr.p, 16, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (n) *n = m ? (size_t) (h[0] < m->cap ? h[0] : m->cap) : 0;
+    if (lit) *lit = (size_t) h[1];
+    return QBT_OK;
+}
+
+extern "C" int qbt_surrogate_samples(qbt_ctx* c, size_t* n, size_t* lit)
+{
+    if (!c || !n) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    return mlp_counts(c, n, lit);
+}
+
+extern "C" int qbt_surrogate_read_samples(qbt_ctx* c, size_t start, size_t n, float* out)
+{
+    if (!c || (n && !out)) return QBT_EINVAL;
+    if (n == 0) return QBT_OK;
+    HIPCHK(hipSetDevice(c->device));
+    size_t held = 0;
+    int    rc   = mlp_counts(c, &held, nullptr);
+    if (rc) return rc;
+    if (start > held || n > held - start) return QBT_ERANGE;
+    MlpState* m = c->mlp;
+    if ((rc = c->dev_alloc(m->io_c, n * 40))) return rc;
+    k_unpack_samples<<<nblk(n), 256, 0, c->stream>>>((const RawSample*) m->samples.p, start, n, (float*) m->io_c.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, m->io_c.p, n * 40, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return QBT_OK;
+}
+
+/* activation buffers for batch width B (multiple of 16) */
+static int mlp_batch_bufs(qbt_ctx* c, int B)
+{
+    MlpState*    m = c->mlp;
+    const size_t b = (size_t) B;
+    int          rc;
+    if ((rc = c->dev_alloc(m->X, kMlpInPad * b * 2)) || (rc = c->dev_alloc(m->H1, 64 * b * 2)) ||
+        (rc = c->dev_alloc(m->H2, 64 * b * 2)) || (rc = c->dev_alloc(m->H3, 64 * b * 2)) ||
+        (rc = c->dev_alloc(m->L16, 16 * b * 4)) || (rc = c->dev_alloc(m->dZ4, 16 * b * 2)) ||
+        (rc = c->dev_alloc(m->dZ3, 64 * b * 2)) || (rc = c->dev_alloc(m->dZ2, 64 * b * 2)) ||
+        (rc = c->dev_alloc(m->dZ1, 64 * b * 2)) || (rc = c->dev_alloc(m->io_y, b * 4)) ||
+        (rc = c->dev_alloc(m->io_p, b * 4)))
+        return rc;
+    return QBT_OK;
+}
+
+static void gemm(hipStream_t st, int M, int N, int K, int kchunk, MatBF A, MatBF B, Epi E)
+{
+    const long waves = (long) (M / 16) * (N / 16) * ((K + kchunk - 1) / kchunk);
+    k_gemm<<<(unsigned) ((waves * 64 + 255) / 256), 256, 0, st>>>(M, N, K, kchunk, A, B, E);
+}
+
+/* Forward of the B-column batch in X (n valid columns, labels in io_y):
+ * probabilities into io_p, summed loss into lossbuf; with backward, the
+ * gradient (layout kGradN) into grads. */
+static int mlp_fwd_bwd(qbt_ctx* c, int B, int n, bool backward, double* loss_out)
+{
+    MlpState*       m  = c->mlp;
+    hipStream_t     st = c->stream;
+    const uint16_t* W  = (const uint16_t*) m->packed.p;
+    const float*    P  = (const float*) m->params.p;
+    uint16_t *      X = (uint16_t*) m->X.p, *H1 = (uint16_t*) m->H1.p, *H2 = (uint16_t*) m->H2.p,
+             *H3 = (uint16_t*) m->H3.p;
+    gemm(st, 64, B, 48, 48, {W + kP1, 48, 1}, {X, B, 1}, {EPI_BIAS_RELU, nullptr, H1, B, 1, P + kB1, {}});
+    gemm(st, 64, B, 64, 64, {W + kP2, 64, 1}, {H1, B, 1}, {EPI_BIAS_RELU, nullptr, H2, B, 1, P + kB2, {}});
+    gemm(st, 64, B, 64, 64, {W + kP3, 64, 1}, {H2, B, 1}, {EPI_BIAS_RELU, nullptr, H3, B, 1, P + kB3, {}});
+    gemm(st, 16, B, 64, 64, {W + kP4, 64, 1}, {H3, B, 1}, {EPI_F32, (float*) m->L16.p, nullptr, B, 1, nullptr, {}});
+    HIPCHK(hipGetLastError());
+    double* lossd = (double*) m->lossbuf.p;
+    k_fill_f32<<<1, 64, 0, st>>>((float*) m->lossbuf.p, 2, 0.0f);
+    k_mlp_loss<<<nblk(B), 256, 0, st>>>((const float*) m->L16.p, P, (const float*) m->io_y.p, B, n,
+                                        (uint16_t*) m->dZ4.p, (float*) m->io_p.p, lossd);
+    HIPCHK(hipGetLastError());
+    if (backward) {
+        float*    G   = (float*) m->grads.p;
+        uint16_t *dZ4 = (uint16_t*) m->dZ4.p, *dZ3 = (uint16_t*) m->dZ3.p, *dZ2 = (uint16_t*) m->dZ2.p,
+                 *dZ1 = (uint16_t*) m->dZ1.p;
+        /* K split of the weight-gradient GEMMs (K = batch) */
+        const int kc = B > 512 ? 512 : B;
+        k_fill_f32<<<nblk(kGradN), 256, 0, st>>>(G, kGradN, 0.0f);
+        /* layer 4: dW4 = dZ4 H3^T, db4 = rowsum dZ4, dZ3 = (W4^T dZ4) where H3 > 0 */
+        gemm(st, 16, 64, B, kc, {dZ4, B, 1}, {H3, 1, B}, {EPI_ATOMIC, G + kP4, nullptr, 64, 1, nullptr, {}});
+        k_rowsum<<<1, 256, 0, st>>>(dZ4, B, G + kGB4);
+        gemm(st, 64, B, 16, 16, {W + kP4, 1, 64}, {dZ4, B, 1}, {EPI_MASK, nullptr, dZ3, B, 1, nullptr, {H3, B, 1}});
+        /* layer 3 */
+        gemm(st, 64, 64, B, kc, {dZ3, B, 1}, {H2, 1, B}, {EPI_ATOMIC, G + kP3, nullptr, 64, 1, nullptr, {}});
+        k_rowsum<<<64, 256, 0, st>>>(dZ3, B, G + kGB3);
+        gemm(st, 64, B, 64, 64, {W + kP3, 1, 64}, {dZ3, B, 1}, {EPI_MASK, nullptr, dZ2, B, 1, nullptr, {H2, B, 1}});
+        /* layer 2 */
+        gemm(st, 64, 64, B, kc, {dZ2, B, 1}, {H1, 1, B}, {EPI_ATOMIC, G + kP2, nullptr, 64, 1, nullptr, {}});
+        k_rowsum<<<64, 256, 0, st>>>(dZ2, B, G + kGB2);
+        gemm(st, 64, B, 64, 64, {W + kP2, 1, 64}, {dZ2, B, 1}, {EPI_MASK, nullptr, dZ1, B, 1, nullptr, {H1, B, 1}});
+        /* layer 1 */
+        gemm(st, 64, 48, B, kc, {dZ1, B, 1}, {X, 1, B}, {EPI_ATOMIC, G + kP1, nullptr, 48, 1, nullptr, {}});
+        k_rowsum<<<64, 256, 0, st>>>(dZ1, B, G + kGB1);
+        HIPCHK(hipGetLastError());
+    }
+    if (loss_out) {
+        double l = 0.0;
+        HIPCHK(hipMemcpyAsync(&l, lossd, 8, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        *loss_out = l / (n > 0 ? n : 1);
+    }
+    return QBT_OK;
+}
+
+static int mlp_adam_step(qbt_ctx* c, float lr)
+{
+    MlpState* m = c->mlp;
+    m->t++;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.0f - powf(b1, (float) m->t), c2 = 1.0f - powf(b2, (float) m->t);
+    k_mlp_adam<<<nblk(kMlpParams), 256, 0, c->stream>>>((float*) m->params.p, (const float*) m->grads.p,
+                                                         (float*) m->m.p, (float*) m->v.p, lr, b1, b2, eps, c1, c2);
+    HIPCHK(hipGetLastError());
+    return mlp_repack(c);
+}
+
+/* host feature rows (n x 42) -> X, labels -> io_y; B = n rounded up to 16 */
+static int mlp_load_host(qbt_ctx* c, const float* x, const float* y, size_t n, int* Bout)
+{
+    MlpState* m = c->mlp;
+    const int B = (int) ((n + 15) / 16) * 16;
+    int       rc;
+    if ((rc = mlp_batch_bufs(c, B)) || (rc = c->dev_alloc(m->io_x, n * kMlpIn * 4)) ||
+        (rc = c->dev_alloc(m->io_a, n * 4)))
+        return rc;
+    HIPCHK(hipMemcpyAsync(m->io_x.p, x, n * kMlpIn * 4, hipMemcpyHostToDevice, c->stream));
+    if (y) HIPCHK(hipMemcpyAsync(m->io_a.p, y, n * 4, hipMemcpyHostToDevice, c->stream));
+    k_copy_features<<<nblk(B), 256, 0, c->stream>>>((const float*) m->io_x.p, B, (int) n, (uint16_t*) m->X.p,
+                                                    y ? (const float*) m->io_a.p : nullptr, (float*) m->io_y.p);
+    HIPCHK(hipGetLastError());
+    *Bout = B;
+    return QBT_OK;
+}
+
+static unsigned infer_grid(const qbt_ctx* c, size_t n)
+{
+    const size_t groups = (n + 15) / 16; /* one group of 16 queries per wave, 4 waves per block */
+    const size_t blocks = (groups + 3) / 4;
+    const size_t maxb   = (size_t) c->num_cus * 4;
+    return (unsigned) (blocks < maxb ? (blocks ? blocks : 1) : maxb);
+}
+
+extern "C" int qbt_surrogate_forward(qbt_ctx* c, const float* x, size_t n, float* p)
+{
+    if (!c || (n && (!x || !p)) || n > ((size_t) 1 << 24) || !qbt_surrogate_ready(c)) return QBT_EINVAL;
+    if (n == 0) return QBT_OK;
+    HIPCHK(hipSetDevice(c->device));
+    int B, rc;
+    if ((rc = mlp_load_host(c, x, nullptr, n, &B))) return rc;
+    MlpState* m = c->mlp;
+    InferArgs ia{(const uint16_t*) m->packed.p, (const float*) m->params.p, 0.f, 0.f, 0.f, 1.f};
+    k_mlp_infer<false><<<infer_grid(c, n), 256, 0, c->stream>>>(ia, (const uint16_t*) m->X.p, B, nullptr, nullptr,
+                                                                 nullptr, (int) n, (float*) m->io_p.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(p, m->io_p.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return QBT_OK;
+}
+
+extern "C" int qbt_surrogate_predict(qbt_ctx* c, const float* isp, const float* nrm, const float* light, size_t n,
+                                     const float basecube[4], float* p)
+{
+    if (!c || (n && (!isp || !nrm || !light || !p)) || !basecube || !(basecube[3] > 0.0f) ||
+        n > ((size_t) 1 << 24) || !qbt_surrogate_ready(c))
+        return QBT_EINVAL;
+    if (n == 0) return QBT_OK;
+    HIPCHK(hipSetDevice(c->device));
+    MlpState* m = c->mlp;
+    int       rc;
+    if ((rc = c->dev_alloc(m->io_x, n * 12 * 3)) || (rc = c->dev_alloc(m->io_a, n * 4))) return rc;
+    float* d = (float*) m->io_x.p;
+    HIPCHK(hipMemcpyAsync(d, isp, n * 12, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d + 3 * n, nrm, n * 12, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d + 6 * n, light, n * 12, hipMemcpyHostToDevice, c->stream));
+    InferArgs ia{(const uint16_t*) m->packed.p, (const float*) m->params.p, basecube[0], basecube[1], basecube[2],
+                 basecube[3]};
+    k_mlp_infer<true><<<infer_grid(c, n), 256, 0, c->stream>>>(ia, nullptr, 0, d, d + 3 * n, d + 6 * n, (int) n,
+                                                                (float*) m->io_a.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(p, m->io_a.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return QBT_OK;
+}
+
+extern "C" int qbt_surrogate_grad(qbt_ctx* c, const float* x, const float* y, size_t n, float* grads, float* loss)
+{
+    if (!c || !x || !y || n == 0 || n > ((size_t) 1 << 22) || !qbt_surrogate_ready(c)) return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    int B, rc;
+    if ((rc = mlp_load_host(c, x, y, n, &B))) return rc;
+    double l = 0.0;
+    if ((rc = mlp_fwd_bwd(c, B, (int) n, true, &l))) return rc;
+    if (grads) {
+        k_grads_to_flat<<<nblk(kMlpParams), 256, 0, c->stream>>>((const float*) c->mlp->grads.p,
+                                                                  (float*) c->mlp->flat.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(grads, c->mlp->flat.p, kMlpParams * 4, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    if (loss) *loss = (float) l;
+    return QBT_OK;
+}
+
+extern "C" int qbt_surrogate_step(qbt_ctx* c, const float* x, const float* y, size_t n, float lr, float* loss)
+{
+    if (!(lr >= 0.0f)) return QBT_EINVAL;
+    int rc = qbt_surrogate_grad(c, x, y, n, nullptr, loss);
+    if (rc) return rc;
+    if ((rc = mlp_adam_step(c, lr))) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return QBT_OK;
+}
+
+int qbt_collect_view(qbt_ctx* c, const SceneDev& s, bool dual, const ViewDev& v, const uint32_t* items,
+                     const unsigned* q, int qnsel, size_t npix)
+{
+    MlpState* m = c->mlp;
+    if (!m || !m->cap) return QBT_OK;
+    const uint32_t stride = m->stride, phase = m->phase;
+    m->phase = (m->phase + 1) % stride; /* successive views sample different queries */
+    const size_t threads = (npix * (size_t) v.nlights) / stride + 1;
+    auto         go      = [&](auto kern) {
+        kern<<<nblk(threads), 256, 0, c->stream>>>(s, v, items, q, qnsel, (const float4*) c->wb_isp.p,
+                                                   (const int*) c->wb_ms.p, (const int*) c->wb_md.p,
+                                                   (const uint8_t*) c->wb_lstat.p, npix, stride, phase,
+                                                   (RawSample*) m->samples.p, m->cap, (unsigned long long*) m->ctr.p);
+    };
+    if (dual) go(k_collect<true>);
+    else go(k_collect<false>);
+    HIPCHK(hipGetLastError());
+    return QBT_OK;
+}
+
+extern "C" int qbt_surrogate_train(qbt_ctx* c, const float basecube[4], int steps, int batch, float lr,
+                                   uint64_t seed, float* loss_out)
+{
+    if (!c || !basecube || steps <= 0 || batch < 16 || batch % 16 || batch > (1 << 20) || !(lr >= 0.0f) ||
+        !(basecube[3] > 0.0f) || !qbt_surrogate_ready(c))
+        return QBT_EINVAL;
+    HIPCHK(hipSetDevice(c->device));
+    MlpState* m = c->mlp;
+    size_t    n = 0;
+    int       rc;
+    if ((rc = mlp_counts(c, &n, nullptr))) return rc;
+    if (n < (size_t) batch) return QBT_EINVAL;
+    if ((rc = mlp_batch_bufs(c, batch)) || (rc = c->dev_alloc(m->perm, n * 4)) || (rc = c->dev_alloc(m->keys, n * 8)) ||
+        (rc = c->dev_alloc(m->vals, n * 4)))
+        return rc;
+    size_t tb = 0;
+    HIPCHK(rocprim::radix_sort_pairs(nullptr, tb, (uint32_t*) m->keys.p, (uint32_t*) m->keys.p + n,
+                                     (uint32_t*) m->vals.p, (uint32_t*) m->perm.p, n, 0, 32, c->stream));
+    if ((rc = c->dev_alloc(m->sort_tmp, tb ? tb : 16))) return rc;
+    auto shuffle = [&]() -> int {
+        k_perm_keys<<<nblk(n), 256, 0, c->stream>>>((uint32_t*) m->keys.p, (uint32_t*) m->vals.p, n,
+                                                    seed + 0x51ED270Bull * ++m->epoch);
+        size_t b2 = tb;
+        HIPCHK(rocprim::radix_sort_pairs(m->sort_tmp.p, b2, (uint32_t*) m->keys.p, (uint32_t*) m->keys.p + n,
+                                         (uint32_t*) m->vals.p, (uint32_t*) m->perm.p, n, 0, 32, c->stream));
+        m->pos    = 0;
+        m->perm_n = n;
+        return QBT_OK;
+    };
+    /* a new shuffle when the store changed size or the permutation is used up */
+    if (m->perm_n != n && (rc = shuffle())) return rc;
+    double last = 0.0;
+    for (int it = 0; it < steps; it++) {
+        if (m->pos + (size_t) batch > n && (rc = shuffle())) return rc;
+        k_batch_features<<<nblk(batch), 256, 0, c->stream>>>((const RawSample*) m->samples.p,
+                                                             (const uint32_t*) m->perm.p, m->pos, batch, basecube[0],
+                                                             basecube[1], basecube[2], basecube[3],
+                                                             (uint16_t*) m->X.p, (float*) m->io_y.p);
+        HIPCHK(hipGetLastError());
+        m->pos += (size_t) batch;
+        if ((rc = mlp_fwd_bwd(c, batch, batch, true, it == steps - 1 ? &last : nullptr))) return rc;
+        if ((rc = mlp_adam_step(c, lr))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (loss_out) *loss_out = (float) last;
+    return QBT_OK;
+}
+
+int qbt_surrogate_decide(qbt_ctx* c, const SceneDev& s, bool dual, const ViewDev& v, const uint32_t* items,
+                         const unsigned* q, int qnsel, size_t npix, uint32_t* fb_items, size_t fb_stride,
+                         unsigned* fb_count)
+{
+    MlpState* m = c->mlp;
+    if (!m || !m->ready) return QBT_EINVAL;
+    InferArgs ia{(const uint16_t*) m->packed.p, (const float*) m->params.p, s.bx, s.by, s.bz, s.bw};
+    /* band edges as logits: p < lo  <=>  logit < log(lo / (1 - lo)) */
+    const float lo   = logf(m->band_lo / (1.0f - m->band_lo));
+    const float hi   = logf(m->band_hi / (1.0f - m->band_hi));
+    const int   grid = c->num_cus * 4;
+    auto        go   = [&](auto kern) {
+        kern<<<grid, 256, 0, c->stream>>>(ia, s, v, items, q, qnsel, (const float4*) c->wb_isp.p,
+                                          (const int*) c->wb_ms.p, (const int*) c->wb_md.p, (uint8_t*) c->wb_lstat.p,
+                                          npix, fb_items, fb_stride, fb_count, lo, hi,
+                                          (unsigned long long*) c->counters.p);
+    };
+    c->prof_mark_begin(QBT_PROF_SURROGATE);
+    if (dual) go(k_mlp_shadow<true>);
+    else go(k_mlp_shadow<false>);
+    c->prof_mark_end();
+    HIPCHK(hipGetLastError());
+    return QBT_OK;
 }

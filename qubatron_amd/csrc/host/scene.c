@@ -85,11 +85,17 @@ int qbt_scene_preset(qbt_scene_spec* s, int config, uint64_t seed, uint64_t npoi
     s->npoints = npoints ? npoints : n_default;
     s->inset   = s->size / (float) (1u << s->depth) * 0.5f;
 
-    /* lights */
+    /* lights. Config 1 uses the light of the reference's demo scene,
+     * (900, 1700, 900), which lies on two mid-planes of the root cube: its
+     * shadow rays take the reference's start-on-a-plane path, which enters a
+     * single quarter of the cube. The larger scenes keep every light off the
+     * mid-planes so that their shadow rays are generic traversals. */
+    static const float L1[3]   = {900.0f, 1700.0f, 900.0f};
     static const float L[4][3] = {
-        {900.0f, 1700.0f, 900.0f}, {300.0f, 1500.0f, 1400.0f}, {1500.0f, 1600.0f, 300.0f}, {1450.0f, 900.0f, 1500.0f}};
+        {880.25f, 1700.0f, 915.75f}, {300.0f, 1500.0f, 1400.0f}, {1500.0f, 1600.0f, 300.0f}, {1450.0f, 950.0f, 1500.0f}};
     s->nlights = config == 3 ? 2 : config == 4 ? 4 : 1;
-    memcpy(s->lights, L, sizeof(float) * 3 * s->nlights);
+    if (config == 1) memcpy(s->lights, L1, sizeof(float) * 3);
+    else memcpy(s->lights, L, sizeof(float) * 3 * s->nlights);
 
     /* keep-out points: lights, and the fixed config-1 camera */
     float kp[8][3];
