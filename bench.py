@@ -1,17 +1,31 @@
 """Benchmark: primary+shadow Mrays/s at 1920x1080 on the 90M-point depth-12
 octree (config 3: planes 60%, 200 boxes, 50 spheres, a 10M-point cluster of
-15 ellipsoids; 2 point lights), exact traversal on one MI355X.
+15 ellipsoids; 2 point lights) on one MI355X, exact traversal, and the same
+frames with the learned shadow surrogate.
 
 A step is one frame: one camera pose of a seeded camera path rendered by the
 hot path (primary trace, one shadow trace per light for every shaded pixel,
 shading), inputs resident in HBM. value = (primary + shadow rays) / wall time
-of the K timed frames. The traversal kernel's average launch duration is
-measured with HIP events on the library's stream (qbt_profile_*), and its
-algorithmic bytes (32 B per node expansion + 28 B per primary leaf hit + the
-per-pixel outputs) give the roofline fraction against 8 TB/s. The CPU oracle
-(the C restatement of the reference traversal) is timed beside it on whole
-frames of the same camera path, using the host's cores, and each sampled
-frame is checked bit-exactly against the GPU frame.
+of the K timed exact frames.
+
+Roofline. Every traversal launch is bracketed by HIP events on the library's
+stream (qbt_profile_*), per kernel kind. Algorithmic bytes: 32 B per node
+expansion (one 8 x int32 child block) + 4 B per primary leaf hit (its model
+index) for the primary kernel; 32 B per expansion for a shadow kernel; per
+frame additionally 24 B of colour + normal per primary hit and the outputs
+(hit 4 + leaf 4 + RGBA8 4 + 1 per light per pixel). The roofline object is
+for the dominant kernel (largest share of the frame's kernel time);
+roofline.frame gives the whole frame. Peak: 8 TB/s HBM3E.
+
+Surrogate leg: the MLP is trained online on the GPU (exact renders of a
+separate seeded camera path with sample collection, Adam steps after each
+frame), then the timed frames are rendered again with QBT_RENDER_SURROGATE
+(fallback band [0.1, 0.9]); its Mrays/s counts the same rays (predicted or
+traced), and its shadow-bit mismatch against the exact frames is reported.
+
+CPU baseline: the C oracle (restatement of the reference traversal) on whole
+frames of the timed camera path on the host's cores, each frame checked
+bit-exactly against the GPU frame.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 
@@ -35,8 +49,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 1 << 30
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_k_render_config3_1080p.json")
-RENDER_KERNEL = "k_render<false, true, false>"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traversal_config3_1080p.json")
+KIND_PRIMARY, KIND_SHADOW, KIND_SURROGATE = 0, 1, 2
 
 
 def log(*a):
@@ -57,6 +71,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU-baseline sample budget")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sur-frames", type=int, default=256, help="exact training frames")
+    ap.add_argument("--sur-samples", type=int, default=1 << 24, help="training samples collected")
+    ap.add_argument("--sur-steps", type=int, default=32, help="Adam steps after each training frame")
+    ap.add_argument("--sur-final", type=int, default=2000, help="Adam steps after the last frame")
+    ap.add_argument("--sur-batch", type=int, default=4096)
+    ap.add_argument("--sur-lr", type=float, default=2e-3)
+    ap.add_argument("--sur-band", type=float, nargs=2, default=(0.1, 0.9))
     ap.add_argument("--traffic", default=TRAFFIC_FILE)
     ap.add_argument("--summarize-pmc", nargs="*", default=None, metavar="DIR")
     ap.add_argument("--path", type=int, default=None, help="render path (0 fused, 1 wavefront); default: library's")
@@ -72,8 +94,7 @@ TRAVERSAL_KERNELS = ("k_render", "k_primary", "k_shadow")
 def summarize_pmc(dirs, out_path):
     """Average per-dispatch counters of each traversal kernel (non-counting
     variants) and of the copy calibration kernel; byte counters are scaled by
-    the copy kernel's known traffic. The traversal kernels of one frame are
-    summed into a per-frame figure."""
+    the copy kernel's known traffic."""
     vals = {}
     for d in dirs:
         for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
@@ -91,7 +112,6 @@ def summarize_pmc(dirs, out_path):
                     key = (kern, row["Counter_Name"])
                     vals.setdefault(key, {})[(f, row.get("Dispatch_Id"))] = float(row["Counter_Value"])
     avg = {k: float(np.mean(list(v.values()))) for k, v in vals.items()}
-    tot = {k: float(np.sum(list(v.values()))) for k, v in vals.items()}
     n = {k: len(v) for k, v in vals.items()}
     res = {"copy_bytes": COPY_BYTES}
     fc, wc = avg.get(("copy", "FETCH_SIZE")), avg.get(("copy", "WRITE_SIZE"))
@@ -104,36 +124,135 @@ def summarize_pmc(dirs, out_path):
         if (k, "FETCH_SIZE") in avg and ff:
             e["fetch_bytes_per_launch"] = avg[(k, "FETCH_SIZE")] * 1024.0 * ff
             e["launches"] = n[(k, "FETCH_SIZE")]
-            e["fetch_bytes_total"] = tot[(k, "FETCH_SIZE")] * 1024.0 * ff
         if (k, "WRITE_SIZE") in avg and wf:
             e["write_bytes_per_launch"] = avg[(k, "WRITE_SIZE")] * 1024.0 * wf
-            e["write_bytes_total"] = tot[(k, "WRITE_SIZE")] * 1024.0 * wf
+        if "fetch_bytes_per_launch" in e and "write_bytes_per_launch" in e:
+            e["hbm_bytes_per_launch"] = e["fetch_bytes_per_launch"] + e["write_bytes_per_launch"]
         h, m = avg.get((k, "TCC_HIT_sum")), avg.get((k, "TCC_MISS_sum"))
         if h is not None and m is not None and h + m > 0:
             e["l2_hit_rate"] = h / (h + m)
         if e:
             per_kernel[k] = e
     res["kernels"] = per_kernel
-    # per frame: all traversal launches of the timed frames
-    frames = None
-    for k, e in per_kernel.items():
-        if k in ("k_primary", "k_render") and "launches" in e:
-            frames = e["launches"]
-    if frames:
-        fb = sum(e.get("fetch_bytes_total", 0.0) for e in per_kernel.values()) / frames
-        wb = sum(e.get("write_bytes_total", 0.0) for e in per_kernel.values()) / frames
-        res["hbm_bytes_per_launch"] = round(fb + wb)
-        res["hbm_bytes_per_launch_note"] = "traversal kernels of one frame (one launch of the fused kernel, or " \
-                                           "the primary + per-light shadow launches of the wavefront path)"
-        hits = [e["l2_hit_rate"] for e in per_kernel.values() if "l2_hit_rate" in e]
-        if hits:
-            res["l2_hit_rate"] = float(np.mean(hits))
     res["counters_kib_or_count"] = {"%s:%s" % k: v for k, v in avg.items()}
     res["dispatches"] = {"%s:%s" % k: v for k, v in n.items()}
     os.makedirs(os.path.dirname(out_path), exist_ok=True)
     with open(out_path, "w") as fh:
         json.dump(res, fh, indent=1, sort_keys=True)
     print(json.dumps(res, indent=1, sort_keys=True))
+
+
+def load_traffic(path):
+    if not os.path.exists(path):
+        return {}
+    try:
+        with open(path) as f:
+            return json.load(f).get("kernels", {})
+    except (OSError, ValueError):
+        return {}
+
+
+def roofline(args, prof, K, W, H, nl, exps, exps_p, leafhits, copy_gbps):
+    """Per-kernel and per-frame algorithmic bytes against the measured kernel times."""
+    km, kl = prof["kind_ms"], prof["kind_launches"]
+    e_p = float(np.mean(exps_p))
+    e_s = float(np.mean(exps)) - e_p
+    h = float(np.mean(leafhits))
+    kern = {}
+    if kl[KIND_PRIMARY]:
+        name = "k_primary" if kl[KIND_SHADOW] else "k_render"
+        b = 32.0 * e_p + 4.0 * h if name == "k_primary" else 32.0 * (e_p + e_s) + 28.0 * h + W * H * (12 + nl)
+        kern[name] = {"alg_bytes_per_launch": b, "avg_ms": km[KIND_PRIMARY] / kl[KIND_PRIMARY],
+                      "share": km[KIND_PRIMARY], "launches": kl[KIND_PRIMARY]}
+    if kl[KIND_SHADOW]:
+        kern["k_shadow"] = {"alg_bytes_per_launch": 32.0 * e_s / nl, "avg_ms": km[KIND_SHADOW] / kl[KIND_SHADOW],
+                            "share": km[KIND_SHADOW], "launches": kl[KIND_SHADOW]}
+    traffic = load_traffic(args.traffic)
+    for name, e in kern.items():
+        e["achieved_gbs"] = e["alg_bytes_per_launch"] / (e["avg_ms"] * 1e-3) / 1e9
+        e["frac"] = e["achieved_gbs"] / HBM_PEAK_GBS
+        t = traffic.get(name, {})
+        e["traffic"] = round(t["hbm_bytes_per_launch"]) if "hbm_bytes_per_launch" in t else None
+        e["l2_hit_rate"] = t.get("l2_hit_rate")
+    dom = max(kern, key=lambda k: kern[k]["share"])
+    d = kern[dom]
+    frame_bytes = 32.0 * (e_p + e_s) + 28.0 * h + W * H * (12 + nl)
+    frame_ms = (km[KIND_PRIMARY] + km[KIND_SHADOW]) / K
+    roof = {"bound": "hbm", "achieved": round(d["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(d["frac"], 4), "traffic": d["traffic"], "kernel": dom,
+            "alg_bytes_per_launch": round(d["alg_bytes_per_launch"]), "kernel_avg_ms": round(d["avg_ms"], 4),
+            "l2_hit_rate": d["l2_hit_rate"], "copy_kernel_gbps": round(copy_gbps, 1),
+            "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in e.items()
+                            if kk != "share"} for k, e in kern.items()},
+            "frame": {"alg_bytes": round(frame_bytes), "traversal_ms": round(frame_ms, 4),
+                      "achieved": round(frame_bytes / (frame_ms * 1e-3) / 1e9, 1),
+                      "frac": round(frame_bytes / (frame_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "expansions_per_frame": {"primary": round(e_p), "shadow": round(e_s)}}
+    return roof
+
+
+def surrogate_leg(args, q, ctx, sc, views, Wm, K, mask, exact_wall, exact_rays, queries_per_frame):
+    from qubatron_amd import surrogate as S
+    W, H = args.width, args.height
+    lights = sc.lights
+    tpos, tang = sc.poses(args.sur_frames, seed=args.pose_seed + 7919)
+    tviews = [q.view(tpos[i], tang[i], W, H, sc.basecube, sc.depth) for i in range(args.sur_frames)]
+    stride = max(1, int(round(args.sur_frames * queries_per_frame / args.sur_samples)))
+    t0 = time.time()
+    info = S.train_online(ctx, tviews, lights, sc.basecube, nsamples=args.sur_samples, stride=stride,
+                          steps_per_frame=args.sur_steps, batch=args.sur_batch, lr=args.sur_lr, seed=args.seed,
+                          final_steps=args.sur_final)
+    ctx.synchronize()
+    train_s = time.time() - t0
+    log(f"surrogate: trained on {info['samples']} samples from {args.sur_frames} frames (stride {stride}), "
+        f"{info['steps']} Adam steps in {train_s:.2f}s, last loss {info['loss_last']:.4f}")
+    ctx.surrogate_set_band(*args.sur_band)
+    smask = mask | q.RENDER_SURROGATE
+    for i in range(Wm):
+        ctx.render(views[i], lights, smask)
+    ctx.synchronize()
+    ctx.profile_begin()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(Wm, K + Wm):
+        ctx.render(views[i], lights, smask)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_end()
+    # untimed: mismatch against the exact frames, predicted share
+    bad = tot = pred = traced = 0
+    resolved = 0
+    fields = ("flags", "shadow")
+    for i in range(Wm, K + Wm):
+        ctx.render(views[i], lights, q.OUT_FLAGS | q.OUT_SHADOW)
+        ex = ctx.read_frame(0, fields=fields)
+        ctx.render(views[i], lights, q.OUT_FLAGS | q.OUT_SHADOW | q.RENDER_SURROGATE)
+        su = ctx.read_frame(0, fields=fields)
+        st = ctx.stats()
+        pred += st["predicted"]
+        traced += st["shadow_rays"]
+        resolved += st["primary_rays"] + st["predicted"] + st["shadow_rays"]
+        b, t = S.shadow_mismatch(ex, su, len(lights))
+        bad, tot = bad + b, tot + t
+    value = exact_rays / wall / 1e6
+    km, kl = prof["kind_ms"], prof["kind_launches"]
+    res = {"value": round(value, 3), "unit": "Mrays/s", "speedup_vs_exact": round(exact_wall / wall, 4),
+           "ms_per_step": round(wall * 1e3 / K, 4), "mismatch": round(bad / max(tot, 1), 6),
+           "mismatched_queries": bad, "shadow_queries": tot,
+           "predicted_fraction": round(pred / max(pred + traced, 1), 4),
+           "rays_resolved_per_frame": round(resolved / K), "band": list(args.sur_band),
+           "kernel_ms_per_frame": {"primary": round(km[KIND_PRIMARY] / K, 4),
+                                   "shadow": round(km[KIND_SHADOW] / K, 4),
+                                   "mlp_decision": round(km[KIND_SURROGATE] / K, 4)},
+           "mlp_decision_launches": kl[KIND_SURROGATE],
+           "train": {"frames": args.sur_frames, "samples": info["samples"], "stride": stride,
+                     "lit_fraction": round(info["lit_fraction"], 4), "adam_steps": info["steps"],
+                     "batch": args.sur_batch, "lr": args.sur_lr, "loss_last": round(info["loss_last"], 5),
+                     "seconds": round(train_s, 2), "pose_seed": args.pose_seed + 7919},
+           "workload": f"same scene and timed camera path as the exact form, {W}x{H}"}
+    log(f"surrogate: {value:.1f} Mrays/s ({res['speedup_vs_exact']:.3f}x exact), mismatch {res['mismatch']:.5f}, "
+        f"predicted {res['predicted_fraction']:.3f}")
+    return res
 
 
 def main():
@@ -152,6 +271,7 @@ def main():
     t_all = time.time()
     sc = q.Scene(args.config, seed=args.seed, npoints=args.npoints)
     lights = sc.lights
+    nl = len(lights)
     W, H = args.width, args.height
     nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
 
@@ -182,16 +302,18 @@ def main():
     mask = q.OUT_DEFAULT
 
     # untimed counting pass: per-frame rays, expansions and primary leaf hits
-    rays, exps, leafhits = [], [], []
+    rays, exps, exps_p, leafhits = [], [], [], []
     for i in range(Wm, K + Wm):
         ctx.render(views[i], lights, mask | q.OUT_FLAGS | q.RENDER_COUNT)
         st = ctx.stats()
         fl = ctx.read_frame(0, fields=("flags",))["flags"]
         rays.append(st["primary_rays"] + st["shadow_rays"])
         exps.append(st["expansions"])
+        exps_p.append(st["expansions_primary"])
         tuning["compact_layout_used"] = bool(st["compact"])
         leafhits.append(int(np.count_nonzero(fl & q.PX_LEAF)))
-    log(f"counting pass: {np.mean(rays) / 1e6:.3f} Mrays/frame, {np.mean(exps) / 1e6:.2f} M expansions/frame")
+    log(f"counting pass: {np.mean(rays) / 1e6:.3f} Mrays/frame, {np.mean(exps) / 1e6:.2f} M expansions/frame "
+        f"({np.mean(exps_p) / 1e6:.2f} M primary)")
 
     for i in range(Wm):  # warmup
         ctx.render(views[i], lights, mask)
@@ -209,26 +331,17 @@ def main():
     wall = t1 - t0
     total_rays = float(sum(rays))
     value = total_rays / wall / 1e6
-    log(f"timed {K} frames: {wall * 1e3:.2f} ms wall, kernel avg {prof['avg_ms']:.3f} ms "
-        f"(min {prof['min_ms']:.3f}, max {prof['max_ms']:.3f}) over {prof['launches']} launches")
+    km, kl = prof["kind_ms"], prof["kind_launches"]
+    log(f"timed {K} frames: {wall * 1e3:.2f} ms wall; primary {km[0] / max(kl[0], 1):.3f} ms x {kl[0]}, "
+        f"shadow {km[1] / max(kl[1], 1):.3f} ms x {kl[1]}")
+    roof = roofline(args, prof, K, W, H, nl, exps, exps_p, leafhits, copy_gbps)
+    log(f"roofline: {roof['kernel']} {roof['achieved']} GB/s = {roof['frac']:.4f} of peak; frame "
+        f"{roof['frame']['achieved']} GB/s")
 
-    # roofline of the traversal kernel (one launch = one frame)
-    out_bytes = W * H * (4 + 4 + len(lights) + 4)
-    alg_bytes = [32.0 * e + 28.0 * h + out_bytes for e, h in zip(exps, leafhits)]
-    achieved = (sum(alg_bytes) / len(alg_bytes)) / (prof["avg_ms"] * 1e-3) / 1e9
-    traffic, l2 = None, None
-    if os.path.exists(args.traffic):
-        try:
-            with open(args.traffic) as f:
-                tj = json.load(f)
-            traffic, l2 = tj.get("hbm_bytes_per_launch"), tj.get("l2_hit_rate")
-        except Exception:
-            traffic = None
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "alg_bytes_per_launch": round(float(np.mean(alg_bytes))), "kernel": "k_render",
-            "kernel_avg_ms": round(prof["avg_ms"], 4), "l2_hit_rate": l2,
-            "copy_kernel_gbps": round(copy_gbps, 1)}
+    sur = None
+    if not args.no_surrogate:
+        qpf = (total_rays - K * W * H) / K  # shadow queries per frame
+        sur = surrogate_leg(args, q, ctx, sc, views, Wm, K, mask, wall, total_rays, qpf)
 
     # CPU baseline: the oracle on whole frames of the timed camera path
     cpu = None
@@ -245,7 +358,7 @@ def main():
             ts = time.time()
             o = osc.render(pos[i], ang[i], W, H, lights, nthreads=nthreads)
             tc += time.time() - ts
-            cpu_rays += W * H + int(np.count_nonzero(o["flags"] & po.PX_SHADED)) * len(lights)
+            cpu_rays += W * H + int(np.count_nonzero(o["flags"] & po.PX_SHADED)) * nl
             frames += 1
             same = same and all(np.array_equal(g[k], o[k]) for k in ("hit", "leaf", "shadow", "flags", "isp",
                                                                     "rgba8"))
@@ -265,12 +378,13 @@ def main():
         "ms_per_step": round(wall * 1e3 / K, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator, no assets)",
         "config": {"workload": f"config{args.config} exact primary+shadow traversal, {sc.npoints} points, "
-                               f"depth {sc.depth}, {W}x{H}, {len(lights)} lights, seeded camera path",
-                   "points": sc.npoints, "nodes": nnodes, "width": W, "height": H, "lights": int(len(lights)),
+                               f"depth {sc.depth}, {W}x{H}, {nl} lights, seeded camera path",
+                   "points": sc.npoints, "nodes": nnodes, "width": W, "height": H, "lights": int(nl),
                    "poses": K, "seed": args.seed, "pose_seed": args.pose_seed,
                    "rays_per_frame": round(total_rays / K), "expansions_per_frame": round(float(np.mean(exps)))},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "surrogate": sur,
         "tuning": tuning,
         "build_s": round(t_build, 3),
         "total_s": round(time.time() - t_all, 1),
