@@ -197,24 +197,48 @@ int qbt_profile_end(qbt_ctx* ctx, qbt_profile* out);
 /* Launch parameters of the render path (what the autotuner chooses).
  * path: 0 = one fused kernel per view (every lane runs its pixel's primary
  * and shadow rays back to back), 1 = wavefront stages with persistent,
- * lane-refilling traversal kernels. block_threads: 64..256 (0 = default 256;
- * halved while the LDS stack exceeds 64 KiB). refill: idle lanes that make a
- * wave fetch new rays, 1..64 (0 = 32). layout: 0 = traverse the compact
+ * lane-refilling traversal kernels over a batch of views. block_threads:
+ * 64..256 (0 = default 256; halved while the LDS stack exceeds 64 KiB).
+ * refill: idle lanes that make a wave claim new rays, 1..64 (0 = 32): the
+ * rays a wave takes per refill. layout: 0 = traverse the compact
  * breadth-first node layout (8 bytes per node, children contiguous) whenever
  * the node array is a proper tree, 1 = always the direct 32-byte child
- * blocks. Results never depend on these. */
+ * blocks. stack: 0 = per-lane traversal stacks in LDS, 1 = in global memory.
+ * reorder: 1 = shadow rays in direction-octant order per light. graph: 1 =
+ * capture the batch's launches into a hipGraph and replay it while the
+ * launch configuration is unchanged. Results never depend on these. */
 typedef struct qbt_tuning {
     int path;
     int block_threads;
     int refill;
     int layout;
-    int reserved[4];
+    int stack;
+    int reorder;
+    int graph;
+    int reserved;
 } qbt_tuning;
 
 int qbt_get_tuning(qbt_ctx* ctx, qbt_tuning* out);
 int qbt_set_tuning(qbt_ctx* ctx, const qbt_tuning* in);
 
-/* Diagnostic: copy `bytes` (a multiple of 16) between two scratch buffers
+/* Autotuner: times qbt_render of the given views and lights (QBT_RENDER_*
+ * counting/collection flags are dropped) under candidate tunings of the
+ * wavefront path - coordinate descent over block_threads {64, 128, 256},
+ * refill {8, 16, 32, 48, 64}, stack {LDS, global} and reorder {0, 1}, two
+ * sweeps, a candidate kept only if it is at least 1% faster - and leaves the
+ * fastest set on the context. results (up to max_results) receive every
+ * timed candidate, *nresults how many were timed. reps: timed renders per
+ * candidate (after one untimed render). Outputs of the last render are
+ * those of the last candidate timed. */
+typedef struct qbt_tune_result {
+    qbt_tuning tuning;
+    double     ms; /* mean time of one qbt_render call */
+} qbt_tune_result;
+
+int qbt_autotune(qbt_ctx* ctx, const qbt_view* views, int nviews, const float* lights, int nlights,
+                 unsigned mask, int reps, qbt_tune_result* results, int max_results, int* nresults);
+
+/* Diagnostic: copy `bytes`/* Diagnostic: copy `bytes` (a multiple of 16) between two scratch buffers
  * with a plain streaming kernel (k_copy16) `reps` times and report the
  * average launch time and (read + write) GB/s. Used to calibrate hardware
  * byte counters and as the achievable-bandwidth reference. */

@@ -324,23 +324,39 @@ class Context:
         out["kind_ms"] = list(p.kind_ms)
         return out
 
+    TUNING_FIELDS = ("path", "block_threads", "refill", "layout", "stack", "reorder", "graph")
+
     def tuning(self):
         t = _ffi.Tuning()
         check(self._d.qbt_get_tuning(self._h, C.byref(t)), "qbt_get_tuning")
-        return {"path": t.path, "block_threads": t.block_threads, "refill": t.refill, "layout": t.layout}
+        return {k: getattr(t, k) for k in self.TUNING_FIELDS}
 
-    def set_tuning(self, path=None, block_threads=None, refill=None, layout=None):
+    def set_tuning(self, **kw):
+        """Set any of TUNING_FIELDS (None leaves a field as it is)."""
         t = _ffi.Tuning()
         check(self._d.qbt_get_tuning(self._h, C.byref(t)), "qbt_get_tuning")
-        if layout is not None:
-            t.layout = int(layout)
-        if path is not None:
-            t.path = int(path)
-        if block_threads is not None:
-            t.block_threads = int(block_threads)
-        if refill is not None:
-            t.refill = int(refill)
+        for k, v in kw.items():
+            if k not in self.TUNING_FIELDS:
+                raise TypeError(f"unknown tuning field {k!r}")
+            if v is not None:
+                setattr(t, k, int(v))
         check(self._d.qbt_set_tuning(self._h, C.byref(t)), "qbt_set_tuning")
+
+    def autotune(self, views, lights, mask=OUT_DEFAULT, reps=3):
+        """Run the library's autotuner on these views; returns (best tuning,
+        list of (tuning, ms) for every candidate timed)."""
+        if isinstance(views, _ffi.View):
+            views = [views]
+        arr = (_ffi.View * len(views))(*views)
+        lights = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 3)
+        res = (_ffi.TuneResult * 64)()
+        n = C.c_int(0)
+        check(self._d.qbt_autotune(self._h, arr, len(views), fptr(lights), lights.shape[0], int(mask), int(reps),
+                                   res, 64, C.byref(n)), "qbt_autotune")
+        for i, v in enumerate(views):
+            self._views[i] = (int(v.width), int(v.height), int(mask) & OUT_ALL & ~OUT_EXP)
+        table = [({k: getattr(r.tuning, k) for k in self.TUNING_FIELDS}, r.ms) for r in res[:min(n.value, 64)]]
+        return self.tuning(), table
 
     def copy_bandwidth(self, nbytes=1 << 30, reps=10):
         ms, gbps = C.c_double(0), C.c_double(0)

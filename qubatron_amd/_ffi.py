@@ -78,7 +78,11 @@ class Profile(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("path", C.c_int), ("block_threads", C.c_int), ("refill", C.c_int), ("layout", C.c_int),
-                ("reserved", C.c_int * 4)]
+                ("stack", C.c_int), ("reorder", C.c_int), ("graph", C.c_int), ("reserved", C.c_int)]
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("tuning", Tuning), ("ms", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -173,7 +177,7 @@ DEV_SYMBOLS = [
     "qbt_copy_bandwidth", "qbt_get_tuning", "qbt_set_tuning",
     "qbt_surrogate_init", "qbt_surrogate_set_params", "qbt_surrogate_get_params", "qbt_surrogate_set_band",
     "qbt_surrogate_reserve", "qbt_surrogate_samples", "qbt_surrogate_read_samples", "qbt_surrogate_forward",
-    "qbt_surrogate_predict", "qbt_surrogate_grad", "qbt_surrogate_step", "qbt_surrogate_train",
+    "qbt_surrogate_predict", "qbt_surrogate_grad", "qbt_surrogate_step", "qbt_surrogate_train", "qbt_autotune",
 ]
 
 
@@ -202,6 +206,8 @@ def _declare_dev(d):
     d.qbt_profile_end.argtypes = [vp, C.POINTER(Profile)]
     d.qbt_get_tuning.argtypes = [vp, C.POINTER(Tuning)]
     d.qbt_set_tuning.argtypes = [vp, C.POINTER(Tuning)]
+    d.qbt_autotune.argtypes = [vp, C.POINTER(View), C.c_int, f32p, C.c_int, C.c_uint, C.c_int,
+                               C.POINTER(TuneResult), C.c_int, C.POINTER(C.c_int)]
     d.qbt_copy_bandwidth.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     szp = C.POINTER(C.c_size_t)
     d.qbt_surrogate_init.argtypes = [vp, C.c_uint64]

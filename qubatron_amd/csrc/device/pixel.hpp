@@ -101,6 +101,20 @@ __device__ __forceinline__ unsigned wave_sum(unsigned x)
     return x;
 }
 
+/* Per-lane stacks in global memory (gstack: L * (dual ? 3 : 2) words per
+ * thread of the grid), interleaved across lanes like the LDS stacks. */
+__device__ __forceinline__ Stack make_stack_global(uint32_t* gstack, int L, int dual)
+{
+    Stack          st;
+    const uint32_t nt  = gridDim.x * blockDim.x;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    st.stride          = (int) nt;
+    st.pw              = gstack + tid;
+    st.ns              = reinterpret_cast<int*>(gstack) + (size_t) L * nt + tid;
+    st.nd              = dual ? reinterpret_cast<int*>(gstack) + 2 * (size_t) L * nt + tid : st.ns;
+    return st;
+}
+
 __device__ __forceinline__ Stack make_stack(uint32_t* lds, int L, int dual)
 {
     Stack st;

@@ -123,11 +123,13 @@ __global__ void __launch_bounds__(256) k_copy16(const int4* __restrict__ src, in
 /* per-view, per-light sprite trace (octree_fsh.c:452-459): the ray from the
  * camera to the light is the same for every pixel of a view. */
 template <bool CMP, bool DUAL, bool GRID>
-__global__ void k_sprite(SceneDev s, ViewDev v, int* __restrict__ out)
+__global__ void k_sprite(SceneDev s, BatchDev b, int* __restrict__ sprite)
 {
     extern __shared__ uint32_t lds[];
-    int                        l = threadIdx.x;
-    Stack                      st = make_stack(lds, s.maxlevel, DUAL);
+    const int                  l  = threadIdx.x;
+    const Stack                st = make_stack(lds, s.maxlevel, DUAL);
+    const ViewDev&             v  = b.views[blockIdx.x];
+    int* __restrict__          out = sprite + blockIdx.x * QBT_MAX_LIGHTS_RENDER;
     if (l >= v.nlights) return;
     Ray r = {v.cam[0], v.cam[1], v.cam[2], v.light[l][0] - v.cam[0], v.light[l][1] - v.cam[1],
              v.light[l][2] - v.cam[2]};
@@ -418,9 +420,15 @@ extern "C" void qbt_destroy(qbt_ctx* c)
     c->dev_free(c->counters);
     c->dev_free(c->sprite);
     c->dev_free(c->rays);
+    c->graph_free();
     for (DevBuf* b : {&c->wb_isp, &c->wb_ms, &c->wb_md, &c->wb_leaf, &c->wb_flags, &c->wb_disc, &c->wb_lstat,
-                      &c->wb_exp, &c->wb_sel, &c->wb_items, &c->wb_q, &c->wb_tmp, &c->valflag})
+                      &c->wb_exp, &c->wb_cand, &c->wb_items, &c->wb_q, &c->wb_lists, &c->wb_fb, &c->wb_gstack,
+                      &c->bt_tab, &c->valflag})
         c->dev_free(*b);
+    for (int k = 0; k < qbt_ctx::kTabSlots; k++) {
+        if (c->tab_host[k]) (void) hipHostFree(c->tab_host[k]);
+        if (c->tab_ev[k]) (void) hipEventDestroy(c->tab_ev[k]);
+    }
     c->mlp_free();
     c->builder_free();
     for (hipEvent_t e : c->prof_ev) (void) hipEventDestroy(e);
@@ -818,15 +826,53 @@ int qbt_ctx::launch_render(const SceneDev& s, bool dual, bool grid, bool count, 
     return QBT_OK;
 }
 
-int qbt_ctx::launch_sprite(const SceneDev& s, bool cmp, bool dual, bool grid, const ViewDev& v, int* spr)
+int qbt_ctx::launch_sprite(const SceneDev& s, bool cmp, bool dual, bool grid, const BatchDev& b)
 {
-    if (v.nlights <= 0) return QBT_OK;
     const size_t lds = qbt_stack_bytes(s.maxlevel, dual, 64);
     qbt_dispatch3(cmp, dual, grid, [&](auto C_, auto D_, auto G_) {
-        k_sprite<decltype(C_)::value, decltype(D_)::value, decltype(G_)::value><<<1, 64, lds, stream>>>(s, v, spr);
+        k_sprite<decltype(C_)::value, decltype(D_)::value, decltype(G_)::value><<<b.n, 64, lds, stream>>>(
+            s, b, (int*) sprite.p);
         return 0;
     });
     HIPCHK(hipGetLastError());
+    return QBT_OK;
+}
+
+/* The view tables of a batch go to the device through a ring of pinned
+ * staging slots; a slot is reused only after its previous copy completed. */
+static int upload_batch(qbt_ctx* c, const BatchHost& bh, BatchDev& b)
+{
+    const int    n     = (int) bh.views.size();
+    const size_t o_it  = sizeof(ViewDev) * QBT_MAX_VIEWS;
+    const size_t o_px  = o_it + 4 * (QBT_MAX_VIEWS + 1);
+    const size_t o_fr  = (o_px + 4 * (QBT_MAX_VIEWS + 1) + 15) & ~(size_t) 15;
+    const size_t bytes = o_fr + sizeof(FrameDev) * QBT_MAX_VIEWS;
+    int          rc;
+    if ((rc = c->dev_alloc(c->bt_tab, bytes))) return rc;
+    if (!c->tab_host[0]) {
+        for (int k = 0; k < qbt_ctx::kTabSlots; k++) {
+            HIPCHK(hipHostMalloc(&c->tab_host[k], bytes, hipHostMallocDefault));
+            HIPCHK(hipEventCreateWithFlags(&c->tab_ev[k], hipEventDisableTiming));
+            HIPCHK(hipEventRecord(c->tab_ev[k], c->stream));
+        }
+    }
+    const int k = c->tab_slot;
+    c->tab_slot = (c->tab_slot + 1) % qbt_ctx::kTabSlots;
+    HIPCHK(hipEventSynchronize(c->tab_ev[k]));
+    uint8_t* h = (uint8_t*) c->tab_host[k];
+    memcpy(h, bh.views.data(), sizeof(ViewDev) * n);
+    memcpy(h + o_it, bh.item_off.data(), 4 * (size_t) (n + 1));
+    memcpy(h + o_px, bh.pix_off.data(), 4 * (size_t) (n + 1));
+    memcpy(h + o_fr, bh.frames.data(), sizeof(FrameDev) * n);
+    HIPCHK(hipMemcpyAsync(c->bt_tab.p, h, bytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->tab_ev[k], c->stream));
+    uint8_t* d = (uint8_t*) c->bt_tab.p;
+    b.views    = (const ViewDev*) d;
+    b.item_off = (const uint32_t*) (d + o_it);
+    b.pix_off  = (const uint32_t*) (d + o_px);
+    b.frames   = (const FrameDev*) (d + o_fr);
+    b.sprite   = (const int*) c->sprite.p;
+    b.n        = n;
     return QBT_OK;
 }
 
@@ -853,30 +899,60 @@ extern "C" int qbt_render(qbt_ctx* c, const qbt_view* views, int nviews, const f
     k_zero_u64<<<1, 64, 0, c->stream>>>((unsigned long long*) c->counters.p, CNT_N);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    uint64_t prim = 0;
-    for (int i = 0; i < nviews; i++) {
-        const qbt_view& q = views[i];
-        SceneDev        s;
-        bool            dual, grid, cmp;
+    uint64_t  prim = 0;
+    LightsDev L;
+    memset(&L, 0, sizeof(L));
+    L.n = nlights;
+    if (nlights > 0) memcpy(L.p, lights, sizeof(float) * 3 * (size_t) nlights);
+    /* batches: runs of consecutive views with the same basecube and maxlevel */
+    for (int i = 0; i < nviews;) {
+        int j = i + 1;
+        while (j < nviews && memcmp(views[j].basecube, views[i].basecube, sizeof(views[i].basecube)) == 0 &&
+               views[j].maxlevel == views[i].maxlevel)
+            j++;
+        SceneDev s;
+        bool     dual, grid, cmp;
         /* the fused kernel walks the direct layout; the wavefront kernels either */
-        int rc = c->scene_for(q.basecube, q.maxlevel, s, dual, grid, cmp, path == 1);
+        int rc = c->scene_for(views[i].basecube, views[i].maxlevel, s, dual, grid, cmp, path == 1);
         if (rc) return rc;
-        ViewDev v;
-        qbt_make_view(q, lights, nlights, v);
-        if ((rc = c->ensure_frame(i, v.w, v.h, mask))) return rc;
-        int* spr = (int*) c->sprite.p + i * QBT_MAX_LIGHTS_RENDER;
-        if ((rc = c->launch_sprite(s, cmp, dual, grid, v, spr))) return rc;
+        BatchHost bh;
+        bh.first       = i;
+        uint64_t items = 0, pix = 0;
+        for (int k = i; k < j; k++) {
+            ViewDev v;
+            qbt_make_view(views[k], lights, nlights, v);
+            if ((rc = c->ensure_frame(k, v.w, v.h, mask))) return rc;
+            bh.item_off.push_back((uint32_t) items);
+            bh.pix_off.push_back((uint32_t) pix);
+            items += (uint64_t) ((v.w + 7) >> 3) * (uint64_t) ((v.h + 7) >> 3) * 64u;
+            pix += (uint64_t) v.w * (uint64_t) v.h;
+            if (items > 0xFFFFFFF0ull) return QBT_ERANGE; /* batch indices are 32-bit */
+            bh.views.push_back(v);
+            bh.frames.push_back(c->frame_dev(k));
+        }
+        bh.item_off.push_back((uint32_t) items);
+        bh.pix_off.push_back((uint32_t) pix);
+        bh.nitems = (uint32_t) items;
+        bh.npix   = (uint32_t) pix;
+        BatchDev b;
+        if ((rc = upload_batch(c, bh, b))) return rc;
+        if (nlights > 0 && (rc = c->launch_sprite(s, cmp, dual, grid, b))) return rc;
         if (path == 0) {
-            if ((rc = c->launch_render(s, dual, grid, count, v, spr, c->frame_dev(i), mask))) return rc;
-            c->last.launches += 1;
-        } else if ((rc = c->render_view_wavefront(s, cmp, dual, grid, count, v, spr, c->frame_dev(i), mask, flags))) {
+            for (int k = i; k < j; k++) {
+                const int* spr = (const int*) c->sprite.p + (k - i) * QBT_MAX_LIGHTS_RENDER;
+                if ((rc = c->launch_render(s, dual, grid, count, bh.views[k - i], spr, c->frame_dev(k), mask)))
+                    return rc;
+                c->last.launches += 1;
+            }
+        } else if ((rc = c->render_batch_wavefront(s, cmp, dual, grid, count, bh, b, L, mask, flags))) {
             return rc;
         }
-        c->last.compact = cmp;
-        prim += (uint64_t) v.w * (uint64_t) v.h;
+        c->last.compact    = cmp;
         c->last.grid_exact = grid;
         c->last.dual       = dual;
-        c->last.launches += 1;
+        c->last.launches += 2;
+        prim += pix;
+        i = j;
     }
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     c->last.primary_rays = prim;
@@ -989,6 +1065,9 @@ extern "C" int qbt_get_tuning(qbt_ctx* c, qbt_tuning* out)
     out->block_threads = c->tune_block;
     out->refill        = c->tune_refill;
     out->layout        = c->use_compact ? 0 : 1;
+    out->stack         = c->tune_stack;
+    out->reorder       = c->tune_reorder;
+    out->graph         = c->tune_graph;
     return QBT_OK;
 }
 
@@ -1000,11 +1079,97 @@ extern "C" int qbt_set_tuning(qbt_ctx* c, const qbt_tuning* in)
         return QBT_EINVAL;
     if (in->refill < 0 || in->refill > 64) return QBT_EINVAL;
     if (in->layout < 0 || in->layout > 1) return QBT_EINVAL;
-    c->path        = in->path;
+    if (in->stack < 0 || in->stack > 1 || in->reorder < 0 || in->reorder > 1 || in->graph < 0 || in->graph > 1)
+        return QBT_EINVAL;
+    c->path         = in->path;
+    c->tune_stack   = in->stack;
+    c->tune_reorder = in->reorder;
+    c->tune_graph   = in->graph;
     c->tune_block  = in->block_threads;
     c->tune_refill = in->refill;
     c->use_compact = in->layout == 0;
     return QBT_OK;
+}
+
+/* time `reps` renders of the batch under the context's current tuning (one
+ * untimed render first: relayout, allocation, graph capture) */
+static int time_renders(qbt_ctx* c, const qbt_view* views, int nviews, const float* lights, int nlights,
+                        unsigned mask, int reps, double* ms)
+{
+    int rc = qbt_render(c, views, nviews, lights, nlights, mask);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(c->ev2, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev2));
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void) hipEventDestroy(a);
+        return QBT_EDEVICE;
+    }
+    (void) hipEventRecord(a, c->stream);
+    for (int r = 0; r < reps && !rc; r++) rc = qbt_render(c, views, nviews, lights, nlights, mask);
+    (void) hipEventRecord(b, c->stream);
+    hipError_t e  = hipEventSynchronize(b);
+    float      el = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&el, a, b);
+    (void) hipEventDestroy(a);
+    (void) hipEventDestroy(b);
+    if (rc) return rc;
+    if (e != hipSuccess) return QBT_EDEVICE;
+    *ms = el / reps;
+    c->pending_stats = false; /* the stats events were overwritten by the timing renders */
+    return QBT_OK;
+}
+
+extern "C" int qbt_autotune(qbt_ctx* c, const qbt_view* views, int nviews, const float* lights, int nlights,
+                            unsigned mask, int reps, qbt_tune_result* results, int max_results, int* nresults)
+{
+    if (!c || !views || nviews <= 0 || reps <= 0 || reps > 100 || (max_results > 0 && !results) || !nresults)
+        return QBT_EINVAL;
+    mask &= ~(QBT_RENDER_COUNT | QBT_RENDER_COLLECT); /* tune the production render */
+    qbt_tuning best;
+    int        rc = qbt_get_tuning(c, &best);
+    if (rc) return rc;
+    best.path   = 1;
+    double best_ms = 0.0;
+    int    n       = 0;
+    auto   run     = [&](const qbt_tuning& t, double* ms) -> int {
+        int e = qbt_set_tuning(c, &t);
+        if (!e) e = time_renders(c, views, nviews, lights, nlights, mask, reps, ms);
+        if (!e && n < max_results) results[n].tuning = t, results[n].ms = *ms;
+        if (!e) n++;
+        return e;
+    };
+    if ((rc = run(best, &best_ms))) return rc;
+    /* coordinate descent over the knobs, two sweeps */
+    static const int blocks[]  = {64, 128, 256};
+    static const int refills[] = {8, 16, 32, 48, 64};
+    for (int sweep = 0; sweep < 2; sweep++) {
+        bool improved = false;
+        for (int knob = 0; knob < 4; knob++) {
+            const int* vals = knob == 0 ? blocks : knob == 1 ? refills : nullptr;
+            const int  nv   = knob == 0 ? 3 : knob == 1 ? 5 : 2;
+            for (int k = 0; k < nv; k++) {
+                qbt_tuning t = best;
+                int*       f = knob == 0 ? &t.block_threads : knob == 1 ? &t.refill : knob == 2 ? &t.stack : &t.reorder;
+                const int  v = vals ? vals[k] : k;
+                const int  cur = knob == 0 ? (best.block_threads ? best.block_threads : 256)
+                                           : knob == 1 ? (best.refill ? best.refill : 32) : *f;
+                if (v == cur) continue;
+                *f = v;
+                double ms = 0.0;
+                if ((rc = run(t, &ms))) return rc;
+                if (ms < best_ms * 0.99) { /* keep only a clear gain */
+                    best     = t;
+                    best_ms  = ms;
+                    improved = true;
+                }
+            }
+        }
+        if (!improved) break;
+    }
+    *nresults = n;
+    return qbt_set_tuning(c, &best);
 }
 
 extern "C" int qbt_copy_bandwidth(qbt_ctx* c, size_t bytes, int reps, double* avg_ms, double* gbps)

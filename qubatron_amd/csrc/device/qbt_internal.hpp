@@ -74,6 +74,12 @@ struct ViewDev {
     float cos_thr;
 };
 
+/* the light positions of a render call (shared by all its views) */
+struct LightsDev {
+    float p[QBT_MAX_LIGHTS_RENDER][3];
+    int   n;
+};
+
 struct FrameDev {
     int*      hit;
     int*      leaf;
@@ -84,6 +90,39 @@ struct FrameDev {
     uint8_t*  flags;
     int*      dyn;
     int*      exp;
+};
+
+/* A batch of views rendered by one set of launches (device tables).
+ * Work items: 64 per 8x8 pixel tile, views in order, item_off[v] the first
+ * item of view v; pixels: view v's pixel (px, py) at pix_off[v] + py*w + px. */
+struct BatchDev {
+    const ViewDev*  views;
+    const uint32_t* item_off; /* n + 1 */
+    const uint32_t* pix_off;  /* n + 1 */
+    const FrameDev* frames;
+    const int*      sprite; /* 8 per view: bit 0 discard, bit 1 paint white */
+    int             n;
+};
+
+/* view of a batch item / pixel: the largest v with off[v] <= x */
+__device__ __forceinline__ int batch_view(const uint32_t* __restrict__ off, int n, uint32_t x)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+/* host side of a batch: the views [first, first + n) of a qbt_render call */
+struct BatchHost {
+    std::vector<ViewDev>  views;
+    std::vector<uint32_t> item_off, pix_off;
+    std::vector<FrameDev> frames;
+    uint32_t              nitems = 0, npix = 0;
+    int                   first = 0;
 };
 
 struct Frame {
@@ -116,8 +155,23 @@ struct qbt_ctx {
     int          path            = 1;  /* 0 = fused per-pixel kernel, 1 = wavefront stages */
     int          num_cus         = 256;
     /* wavefront render buffers */
-    DevBuf       wb_isp, wb_ms, wb_md, wb_leaf, wb_flags, wb_disc, wb_lstat, wb_exp, wb_sel, wb_items, wb_q, wb_tmp;
-    DevBuf       wb_fb; /* surrogate: per-light lists of the rays traced exactly */
+    DevBuf       wb_isp, wb_ms, wb_md, wb_leaf, wb_flags, wb_disc, wb_lstat, wb_exp, wb_cand, wb_items, wb_q;
+    DevBuf       wb_lists; /* per-light shadow lists in direction-octant order */
+    DevBuf       wb_fb;    /* surrogate: per-light lists of the rays traced exactly */
+    DevBuf       wb_gstack; /* traversal stacks in global memory (tuning stack = 1) */
+    DevBuf       bt_tab; /* batch tables: views, item and pixel offsets, frames */
+    static constexpr int kTabSlots = 4;
+    void*        tab_host[kTabSlots] = {};      /* pinned staging of the batch tables */
+    hipEvent_t   tab_ev[kTabSlots]   = {};      /* the last copy out of each slot */
+    int          tab_slot            = 0;
+    int          tune_stack   = 0; /* 0 = LDS, 1 = global memory */
+    int          tune_reorder = 1; /* shadow rays in direction-octant order */
+    int          tune_graph   = 1; /* replay batched renders from a captured hipGraph */
+    /* captured render graph and the launch configuration it was captured for */
+    hipGraph_t           graph      = nullptr;
+    hipGraphExec_t       graph_exec = nullptr;
+    std::vector<uint8_t> graph_key;
+    uint64_t             graph_replays = 0;
     MlpState*    mlp             = nullptr;
     BuildState*  build           = nullptr;
     /* qbt_profile_begin/end: event pairs around traversal-kernel launches */
@@ -161,28 +215,31 @@ struct qbt_ctx {
     int      block_threads(int L, bool dual) const;
     int      launch_render(const qbt::SceneDev& s, bool dual, bool grid, bool count, const ViewDev& v, const int* spr,
                            FrameDev f, unsigned mask);
-    int      launch_sprite(const qbt::SceneDev& s, bool cmp, bool dual, bool grid, const ViewDev& v, int* spr);
+    int      launch_sprite(const qbt::SceneDev& s, bool cmp, bool dual, bool grid, const BatchDev& b);
     int      ensure_wave_bufs(size_t npix, int nlights);
     /* flags: QBT_RENDER_SURROGATE, QBT_RENDER_COLLECT */
-    int      render_view_wavefront(const qbt::SceneDev& s, bool cmp, bool dual, bool grid, bool count,
-                                   const ViewDev& v, const int* spr, FrameDev f, unsigned mask, unsigned flags);
+    int      render_batch_wavefront(const qbt::SceneDev& s, bool cmp, bool dual, bool grid, bool count,
+                                    const BatchHost& bh, const BatchDev& b, const LightsDev& L, unsigned mask,
+                                    unsigned flags);
+    void     graph_free();
     void     mlp_free();
     void     builder_free();
 };
 
-/* surrogate.hip: the MLP decision pass of one view. For each of the nsel
- * shaded pixels (items, tile order; count at q[qnsel]) and each light it
- * writes LS_PRED_* where the prediction is outside the band and appends the
- * item to that light's exact-trace list (fb_items + l * fb_stride, count
- * fb_count[l]) otherwise. */
-int qbt_surrogate_decide(qbt_ctx* c, const qbt::SceneDev& s, bool dual, const ViewDev& v, const uint32_t* items,
-                         const unsigned* q, int qnsel, size_t npix, uint32_t* fb_items, size_t fb_stride,
+/* surrogate.hip: the MLP decision pass of a batch. The shaded pixels of
+ * light l are lists[l * list_stride + i], i < *nsel (batch pixel indices).
+ * For every (pixel, light) query it writes LS_PRED_* where the prediction is
+ * outside the band and appends the pixel to that light's exact-trace list
+ * (fb + l * fb_stride, count fb_count[l]) otherwise. */
+int qbt_surrogate_decide(qbt_ctx* c, const qbt::SceneDev& s, bool dual, const LightsDev& L, const uint32_t* lists,
+                         size_t list_stride, const unsigned* nsel, size_t npix, uint32_t* fb, size_t fb_stride,
                          unsigned* fb_count);
-/* surrogate.hip: append training samples (every stride-th shaded
- * pixel x light of the view, exact label) to the sample store */
-int qbt_collect_view(qbt_ctx* c, const qbt::SceneDev& s, bool dual, const ViewDev& v, const uint32_t* items,
-                     const unsigned* q, int qnsel, size_t npix);
+/* surrogate.hip: append training samples (every stride-th shaded pixel x
+ * light of the batch, exact label) to the sample store */
+int qbt_collect_batch(qbt_ctx* c, const qbt::SceneDev& s, bool dual, const LightsDev& L, const uint32_t* lists,
+                      size_t list_stride, const unsigned* nsel, size_t npix);
 bool qbt_surrogate_ready(const qbt_ctx* c);
+void qbt_surrogate_band(const qbt_ctx* c, float* lo, float* hi);
 
 void   qbt_set_hip_error(hipError_t e, const char* what, const char* file, int line);
 bool   qbt_grid_exact(const float b[4], int L);

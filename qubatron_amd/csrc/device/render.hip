@@ -1,36 +1,49 @@
 /*
- * render.hip - the wavefront render path of qbt_render.
+ * render.hip - the wavefront render path of qbt_render, one batch of views
+ * (same scene, basecube, maxlevel and lights) per set of launches.
  *
  * The pixel program of octree_fsh.c:399-464 is split into stages so that
  * every traversal kernel runs one kind of ray and keeps its SIMD lanes busy:
  *
- *   k_primary  camera rays, one per pixel, pixels in 8x8-tile order;
- *              writes the hit record (entry point, model indices, leaf, flags)
- *   select     stable compaction (rocprim) of the shaded pixels' work items
+ *   k_sprite_batch  (qbt_core.hip) per view and light, the sprite trace of
+ *              :452-459, the same for every pixel of a view
+ *   k_primary  camera rays of every view of the batch, one per pixel, pixels
+ *              in 8x8-tile order, views in order; writes the hit record
+ *              (entry point, model indices, leaf, flags) and, per work item,
+ *              the batch pixel index when the pixel is shaded
+ *   k_compact  list of the shaded pixels (one atomic per wave; a wave covers
+ *              one tile, so tiles stay contiguous)
+ *   k_oct_*    per light, the list reordered by the direction octant of the
+ *              shadow ray (tuning reorder = 1)
+ *   (k_mlp_shadow, surrogate.hip: QBT_RENDER_SURROGATE decisions)
  *   k_shadow   one launch per light: the ray from the light to each shaded
- *              pixel's entry point (octree_fsh.c:431-434); writes the per-light
- *              status (lit / occluded / discarded); pixels discarded by an
- *              earlier light are skipped, as the shader stops there
- *   k_shade    per pixel: attributes, the shading formula
- *              (octree_fsh.c:436-449), the light sprite (:452-459), outputs
+ *              pixel's entry point (:431-434); writes the per-light status;
+ *              pixels discarded by an earlier light are skipped, as the
+ *              shader stops there
+ *   k_shade    per pixel of the batch: attributes, shading (:436-449),
+ *              sprite (:452-459), outputs into each view's frame
  *
  * k_primary and k_shadow are persistent: each wave keeps up to 64 rays in
  * flight and refills idle lanes from a work counter with one atomic per
  * refill, so a lane that finishes early starts the next ray instead of
- * idling until the slowest ray of its wave ends. Work items are split into
- * eight contiguous segments, one per XCD (workgroups are dispatched to the
- * eight XCDs round-robin, so block b starts on segment b % 8 and moves to
- * the next segment when its own runs dry): neighbouring pixels, which share
- * most of their node path, stay on one XCD's L2.
+ * idling until the slowest ray of its wave ends. Work is split into eight
+ * contiguous segments, one per XCD (workgroups are dispatched to the eight
+ * XCDs round-robin, so block b starts on segment b % 8 and moves on when its
+ * own runs dry): neighbouring pixels, which share most of their node path,
+ * stay on one XCD's L2.
+ *
+ * The whole sequence contains kernels only (no memset, copy or
+ * allocation), so it is captured once into a hipGraph and replayed for later
+ * batches of the same shape; the view tables it reads are copied in before
+ * each replay.
  *
  * All traversal arithmetic is trace.hpp's; results are identical to the
- * fused kernel and to the reference algorithm.
+ * fused kernel and to the reference algorithm, whatever the batch
+ * composition, ray order or stack placement.
  */
 #include <hip/hip_runtime.h>
 
 #include <cstring>
-
-#include <rocprim/rocprim.hpp>
 
 #include "pixel.hpp"
 #include "qbt.h"
@@ -46,26 +59,31 @@ using namespace qbt;
 namespace {
 
 constexpr int kSeg = 8;
+constexpr int kNL  = QBT_MAX_LIGHTS_RENDER;
 
-/* per-view device counters (uint32), zeroed before each view: refill
- * counters of the primary and per-light shadow launches, the number of
- * shaded pixels, the per-light exact-trace counts of the surrogate mode */
+/* batch counters (uint32), zeroed before each batch: refill counters of the
+ * primary and per-light shadow launches, the number of shaded pixels, the
+ * per-light exact-trace counts of the surrogate mode, the per-(light,
+ * octant) counts and cursors of the reordering */
 enum {
     Q_PRIM   = 0,
     Q_SHADOW = kSeg,
-    Q_NSEL   = kSeg + kSeg * QBT_MAX_LIGHTS_RENDER,
+    Q_NSEL   = kSeg + kSeg * kNL,
     Q_FB     = Q_NSEL + 1,
-    Q_N      = Q_FB + QBT_MAX_LIGHTS_RENDER
+    Q_OCT    = Q_FB + kNL,
+    Q_CUR    = Q_OCT + 8 * kNL,
+    Q_N      = Q_CUR + 8 * kNL
 };
+static_assert(Q_N <= 256, "counter block");
 
 struct PrimOut {
-    float4*  isp;    /* entry point of the primary leaf, 0 when none */
-    int*     ms;     /* static model index, -1 none */
-    int*     md;     /* dynamic model index, -1 none */
-    int*     leaf;   /* static leaf node, -1 none */
-    uint8_t* flags;  /* QBT_PX_LEAF | QBT_PX_SHADED | QBT_PX_DISCARD */
-    uint8_t* sel;    /* per work item (tile order): shaded */
-    int*     exp;    /* per-pixel expansions (counting renders) */
+    float4*   isp;   /* entry point of the primary leaf, 0 when none */
+    int*      ms;    /* static model index, -1 none */
+    int*      md;    /* dynamic model index, -1 none */
+    int*      leaf;  /* static leaf node (reference index), -1 none */
+    uint8_t*  flags; /* QBT_PX_LEAF | QBT_PX_SHADED | QBT_PX_DISCARD */
+    uint32_t* cand;  /* per work item: batch pixel index if shaded, else ~0 */
+    int*      exp;   /* per pixel expansions (counting renders) */
 };
 
 __device__ __forceinline__ uint32_t seg_bound(uint32_t n, int s) { return (uint32_t) (((uint64_t) n * s) / kSeg); }
@@ -84,7 +102,7 @@ struct Refill {
         const uint32_t lo = seg_bound(n, seg), hi = seg_bound(n, seg + 1);
         uint32_t       base = 0;
         if (lane == leader) base = atomicAdd(&ctr[seg], nidle);
-        base = __shfl(base, leader, 64);
+        base          = __shfl(base, leader, 64);
         uint32_t item = UINT32_MAX;
         if ((idle >> lane) & 1ull) {
             uint32_t rank = (uint32_t) __popcll(idle & ((1ull << lane) - 1ull));
@@ -99,25 +117,33 @@ struct Refill {
     }
 };
 
-template <bool CMP, bool DUAL, bool GRID, bool COUNT>
-__global__ void __launch_bounds__(256) k_primary(SceneDev s, ViewDev v, PrimOut o, unsigned* __restrict__ q,
-                                                 unsigned refill, unsigned long long* __restrict__ counters)
+/* the calling lane's traversal stack: LDS, or global memory (GSTK) */
+template <bool DUAL, bool GSTK>
+__device__ __forceinline__ Stack lane_stack(uint32_t* lds, uint32_t* gstack, int L)
+{
+    return GSTK ? make_stack_global(gstack, L, DUAL) : make_stack(lds, L, DUAL);
+}
+
+template <bool CMP, bool DUAL, bool GRID, bool COUNT, bool GSTK>
+__global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut o, uint32_t nitems,
+                                                 unsigned* __restrict__ q, unsigned refill, uint32_t* gstack,
+                                                 unsigned long long* __restrict__ counters)
 {
     extern __shared__ uint32_t lds[];
-    const Stack    st      = make_stack(lds, s.maxlevel, DUAL);
-    const int      lane    = threadIdx.x & 63;
-    const int      tiles_x = (v.w + 7) >> 3, tiles_y = (v.h + 7) >> 3;
-    Refill         rf((uint32_t) tiles_x * (uint32_t) tiles_y * 64u);
-    bool           active = false;
-    uint32_t       item = 0, pix = 0;
-    Ray            r = {v.cam[0], v.cam[1], v.cam[2], 0.f, 0.f, 0.f};
-    TraceState     t;
-    unsigned       n_exp = 0, n_abort = 0, n_hit = 0;
+    const Stack st   = lane_stack<DUAL, GSTK>(lds, gstack, s.maxlevel);
+    const int   lane = threadIdx.x & 63;
+    Refill      rf(nitems);
+    bool        active = false;
+    uint32_t    item = 0, gp = 0;
+    Ray         r = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    TraceState  t;
+    unsigned    n_exp = 0, n_abort = 0, n_hit = 0;
 
     auto finish = [&](int stt) {
-        float4  isp = make_float4(0.f, 0.f, 0.f, 0.f);
-        int     ms = -1, md = -1, leaf = -1;
-        uint8_t f = 0, sel = 0;
+        float4   isp = make_float4(0.f, 0.f, 0.f, 0.f);
+        int      ms = -1, md = -1, leaf = -1;
+        uint8_t  f    = 0;
+        uint32_t cand = UINT32_MAX;
         if (stt == ST_DISCARD) f = QBT_PX_DISCARD;
         if (stt == ST_ABORT) n_abort++;
         if (stt == ST_HIT) {
@@ -130,18 +156,18 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, ViewDev v, PrimOut 
             f    = QBT_PX_LEAF;
             if (isp.w > 0.0f) {
                 f |= QBT_PX_SHADED;
-                sel = 1;
+                cand = gp;
                 n_hit++;
             }
         }
-        o.isp[pix]   = isp;
-        o.ms[pix]    = ms;
-        o.md[pix]    = md;
-        o.leaf[pix]  = leaf;
-        o.flags[pix] = f;
-        o.sel[item]  = sel;
+        o.isp[gp]    = isp;
+        o.ms[gp]     = ms;
+        o.md[gp]     = md;
+        o.leaf[gp]   = leaf;
+        o.flags[gp]  = f;
+        o.cand[item] = cand;
         if (COUNT) {
-            o.exp[pix] = (int) t.exps;
+            o.exp[gp] = (int) t.exps;
             n_exp += t.exps;
         }
     };
@@ -149,14 +175,19 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, ViewDev v, PrimOut 
     for (;;) {
         const uint64_t idle = __ballot(!active);
         if (rf.left > 0 && (__popcll(idle) >= (int) refill || idle == ~0ull)) {
-            uint32_t it = rf.fetch(&q[Q_PRIM], idle, lane);
+            const uint32_t it = rf.fetch(&q[Q_PRIM], idle, lane);
             if (it != UINT32_MAX) {
-                int px, py;
+                const int      v = batch_view(b.item_off, b.n, it);
+                const ViewDev& V = b.views[v];
+                int            px, py;
                 item = it;
-                if (item_pixel(it, tiles_x, v.w, v.h, px, py)) {
-                    pix = (uint32_t) py * (uint32_t) v.w + (uint32_t) px;
+                if (item_pixel(it - b.item_off[v], (V.w + 7) >> 3, V.w, V.h, px, py)) {
+                    gp = b.pix_off[v] + (uint32_t) py * (uint32_t) V.w + (uint32_t) px;
                     float csv[3];
-                    pixel_dir(v, px, py, csv);
+                    pixel_dir(V, px, py, csv);
+                    r.px    = V.cam[0];
+                    r.py    = V.cam[1];
+                    r.pz    = V.cam[2];
                     r.dx    = csv[0];
                     r.dy    = csv[1];
                     r.dz    = csv[2];
@@ -164,7 +195,7 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, ViewDev v, PrimOut 
                     if (stt == ST_CONT) active = true;
                     else finish(stt);
                 } else {
-                    o.sel[it] = 0;
+                    o.cand[it] = UINT32_MAX; /* padding item of an edge tile */
                 }
             }
         }
@@ -191,32 +222,98 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, ViewDev v, PrimOut 
     }
 }
 
-template <bool CMP, bool DUAL, bool GRID, bool COUNT>
-__global__ void __launch_bounds__(256) k_shadow(SceneDev s, ViewDev v, int l, const uint32_t* __restrict__ sel_items,
-                                                const unsigned* __restrict__ nsel,
+/* shaded pixels -> list; one atomic per wave, the wave's lanes in order */
+__global__ void k_compact(const uint32_t* __restrict__ cand, uint32_t n, uint32_t* __restrict__ out,
+                          unsigned* __restrict__ count)
+{
+    const uint32_t i    = blockIdx.x * blockDim.x + threadIdx.x;
+    const int      lane = threadIdx.x & 63;
+    const uint32_t v    = i < n ? cand[i] : UINT32_MAX;
+    const bool     keep = v != UINT32_MAX;
+    const uint64_t m    = __ballot(keep);
+    if (m == 0ull) return; /* uniform over the wave */
+    const int leader = __ffsll((unsigned long long) m) - 1;
+    uint32_t  base   = 0;
+    if (lane == leader) base = atomicAdd(count, (unsigned) __popcll(m));
+    base = __shfl(base, leader, 64);
+    if (keep) out[base + (uint32_t) __popcll(m & ((1ull << lane) - 1ull))] = v;
+}
+
+/* direction octant of the shadow ray light -> isp */
+__device__ __forceinline__ int shadow_octant(const float4& ip, const float* Lp)
+{
+    return (ip.x - Lp[0] < 0.0f ? 1 : 0) | (ip.y - Lp[1] < 0.0f ? 2 : 0) | (ip.z - Lp[2] < 0.0f ? 4 : 0);
+}
+
+/* Octant reordering, pass 1 (SCATTER = false): per (light, octant) counts;
+ * pass 2 (SCATTER = true): each pixel appended to its bin of that light's
+ * list. One atomic per (wave, light, octant present). */
+template <bool SCATTER>
+__global__ void k_oct_bin(const uint32_t* __restrict__ list, const unsigned* __restrict__ nsel,
+                          const float4* __restrict__ isp, LightsDev L, unsigned* __restrict__ cnt,
+                          uint32_t* __restrict__ lists, size_t stride)
+{
+    const uint32_t i     = blockIdx.x * blockDim.x + threadIdx.x;
+    const int      lane  = threadIdx.x & 63;
+    const bool     valid = i < *nsel;
+    const uint32_t gp    = valid ? list[i] : 0u;
+    const float4   ip    = valid ? isp[gp] : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (__ballot(valid) == 0ull) return; /* uniform over the wave */
+    for (int l = 0; l < L.n; l++) {
+        const int oct     = shadow_octant(ip, L.p[l]);
+        uint64_t  pending = __ballot(valid);
+        while (pending) {
+            const int      leader = __ffsll((unsigned long long) pending) - 1;
+            const int      k      = __shfl(oct, leader, 64);
+            const uint64_t mine   = __ballot(valid && oct == k);
+            uint32_t       base   = 0;
+            if (lane == leader) base = atomicAdd(&cnt[8 * l + k], (unsigned) __popcll(mine));
+            if (SCATTER) {
+                base = __shfl(base, leader, 64);
+                if (valid && oct == k)
+                    lists[(size_t) l * stride + base + (uint32_t) __popcll(mine & ((1ull << lane) - 1ull))] = gp;
+            }
+            pending &= ~mine;
+        }
+    }
+}
+
+/* per light: exclusive scan of the 8 octant counts into the cursors */
+__global__ void k_oct_scan(const unsigned* __restrict__ cnt, unsigned* __restrict__ cur, int nl)
+{
+    const int l = threadIdx.x;
+    if (l >= nl) return;
+    unsigned acc = 0;
+    for (int k = 0; k < 8; k++) {
+        cur[8 * l + k] = acc;
+        acc += cnt[8 * l + k];
+    }
+}
+
+template <bool CMP, bool DUAL, bool GRID, bool COUNT, bool GSTK>
+__global__ void __launch_bounds__(256) k_shadow(SceneDev s, float Lx, float Ly, float Lz, int l,
+                                                const uint32_t* __restrict__ list, const unsigned* __restrict__ nsel,
                                                 const float4* __restrict__ isp, uint8_t* __restrict__ lstat,
                                                 uint8_t* __restrict__ disc, unsigned* __restrict__ q,
-                                                unsigned refill, int* __restrict__ exp,
+                                                unsigned refill, uint32_t* gstack, int* __restrict__ exp,
                                                 unsigned long long* __restrict__ counters)
 {
     extern __shared__ uint32_t lds[];
-    const Stack st      = make_stack(lds, s.maxlevel, DUAL);
-    const int   lane    = threadIdx.x & 63;
-    const int   tiles_x = (v.w + 7) >> 3;
+    const Stack st   = lane_stack<DUAL, GSTK>(lds, gstack, s.maxlevel);
+    const int   lane = threadIdx.x & 63;
     Refill      rf(*nsel);
     bool        active = false;
-    uint32_t    pix = 0;
-    const float Lx = v.light[l][0], Ly = v.light[l][1], Lz = v.light[l][2];
-    Ray         r = {Lx, Ly, Lz, 0.f, 0.f, 0.f};
-    float4      ip = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t    gp     = 0;
+    Ray         r      = {Lx, Ly, Lz, 0.f, 0.f, 0.f};
+    float4      ip     = make_float4(0.f, 0.f, 0.f, 0.f);
     TraceState  t;
     unsigned    n_exp = 0, n_abort = 0, n_ray = 0;
 
     auto finish = [&](int stt) {
         uint8_t ls;
         if (stt == ST_DISCARD) {
-            ls        = LS_DISCARD;
-            disc[pix] = 1;
+            ls       = LS_DISCARD;
+            disc[gp] = 1;
         } else {
             if (stt == ST_ABORT) n_abort++;
             float4 li  = stt == ST_HIT ? t.entry : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -224,9 +321,9 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, ViewDev v, int l, co
             float  sqr = ix * ix + iy * iy + iz * iz;
             ls         = (15.0f < sqr) ? LS_OCCLUDED : LS_LIT; /* step(sqr, 15.0) */
         }
-        lstat[pix] = ls;
+        lstat[gp] = ls;
         if (COUNT) {
-            atomicAdd(&exp[pix], (int) t.exps);
+            atomicAdd(&exp[gp], (int) t.exps);
             n_exp += t.exps;
         }
     };
@@ -234,15 +331,13 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, ViewDev v, int l, co
     for (;;) {
         const uint64_t idle = __ballot(!active);
         if (rf.left > 0 && (__popcll(idle) >= (int) refill || idle == ~0ull)) {
-            uint32_t k = rf.fetch(&q[Q_SHADOW + kSeg * l], idle, lane);
+            const uint32_t k = rf.fetch(&q[Q_SHADOW + kSeg * l], idle, lane);
             if (k != UINT32_MAX) {
-                int px, py;
-                item_pixel(sel_items[k], tiles_x, v.w, v.h, px, py);
-                pix = (uint32_t) py * (uint32_t) v.w + (uint32_t) px;
-                if (l > 0 && disc[pix]) {
-                    lstat[pix] = LS_SKIPPED; /* an earlier light discarded the fragment */
+                gp = list[k];
+                if (l > 0 && disc[gp]) {
+                    lstat[gp] = LS_SKIPPED; /* an earlier light discarded the fragment */
                 } else {
-                    ip      = isp[pix];
+                    ip      = isp[gp];
                     r.dx    = ip.x - Lx;
                     r.dy    = ip.y - Ly;
                     r.dz    = ip.z - Lz;
@@ -275,22 +370,24 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, ViewDev v, int l, co
     }
 }
 
-/* per pixel: attributes, shading, sprite, outputs (octree_fsh.c:421-463) */
+/* per batch pixel: attributes, shading, sprite, outputs (octree_fsh.c:421-463) */
 template <bool DUAL>
-__global__ void __launch_bounds__(256) k_shade(SceneDev s, ViewDev v, const int* __restrict__ sprite, PrimOut in,
-                                               const uint8_t* __restrict__ lstat, size_t lstride, FrameDev f,
-                                               unsigned mask)
+__global__ void __launch_bounds__(256) k_shade(SceneDev s, BatchDev b, PrimOut in, const uint8_t* __restrict__ lstat,
+                                               uint32_t npix, unsigned mask)
 {
-    const size_t npix = (size_t) v.w * v.h;
-    const size_t o    = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= npix) return;
-    const int px = (int) (o % (size_t) v.w), py = (int) (o / (size_t) v.w);
-    uint32_t  flg   = in.flags[o];
-    uint32_t  sbits = 0;
-    float     col[4] = {0.f, 0.f, 0.f, 0.f};
-    float     nrm[4] = {0.f, 0.f, 0.f, 0.f};
-    const int ms = in.ms[o], md = in.md[o];
-    float4    isp = in.isp[o];
+    const int      vi     = batch_view(b.pix_off, b.n, o);
+    const ViewDev& v      = b.views[vi];
+    const uint32_t lo     = o - b.pix_off[vi];
+    const int      px     = (int) (lo % (uint32_t) v.w), py = (int) (lo / (uint32_t) v.w);
+    const int*     spr    = b.sprite + vi * kNL;
+    uint32_t       flg    = in.flags[o];
+    uint32_t       sbits  = 0;
+    float          col[4] = {0.f, 0.f, 0.f, 0.f};
+    float          nrm[4] = {0.f, 0.f, 0.f, 0.f};
+    const int      ms = in.ms[o], md = in.md[o];
+    const float4   isp = in.isp[o];
     if (!(flg & QBT_PX_DISCARD)) {
         float csv[3];
         pixel_dir(v, px, py, csv);
@@ -305,7 +402,7 @@ __global__ void __launch_bounds__(256) k_shade(SceneDev s, ViewDev v, const int*
             float camv = gmax(dot3(nmc, nn), 0.0f);
             float fac  = 0.1f + 0.2f * camv;
             for (int l = 0; l < v.nlights; l++) {
-                uint8_t ls = lstat[l * lstride + o];
+                uint8_t ls = lstat[(size_t) l * npix + o];
                 if (ls == LS_DISCARD) {
                     flg |= QBT_PX_DISCARD;
                     break;
@@ -337,7 +434,7 @@ __global__ void __launch_bounds__(256) k_shade(SceneDev s, ViewDev v, const int*
             normalize3(csv, ncsv);
             for (int l = 0; l < v.nlights; l++) {
                 if (dot3(v.cl[l], ncsv) > v.cos_thr) {
-                    int sf = sprite[l];
+                    int sf = spr[l];
                     if (sf & 1) {
                         flg |= QBT_PX_DISCARD;
                         break;
@@ -354,14 +451,16 @@ __global__ void __launch_bounds__(256) k_shade(SceneDev s, ViewDev v, const int*
         col[0] = col[1] = col[2] = col[3] = 0.0f;
         sbits                             = 0;
     }
-    if (mask & QBT_OUT_HIT) f.hit[o] = ms;
-    if (mask & QBT_OUT_LEAF) f.leaf[o] = in.leaf[o];
-    if (mask & QBT_OUT_DYN) f.dyn[o] = md;
-    if (mask & QBT_OUT_SHADOW) f.shadow[o] = (uint8_t) sbits;
-    if (mask & QBT_OUT_FLAGS) f.flags[o] = (uint8_t) flg;
-    if (mask & QBT_OUT_RGBA8) f.rgba8[o] = pack_rgba8(col[0], col[1], col[2], col[3]);
-    if (mask & QBT_OUT_RGBAF) f.rgbaf[o] = make_float4(col[0], col[1], col[2], col[3]);
-    if (mask & QBT_OUT_ISP) f.isp[o] = isp;
+    const FrameDev& f = b.frames[vi];
+    if (mask & QBT_OUT_HIT) f.hit[lo] = ms;
+    if (mask & QBT_OUT_LEAF) f.leaf[lo] = in.leaf[o];
+    if (mask & QBT_OUT_DYN) f.dyn[lo] = md;
+    if (mask & QBT_OUT_SHADOW) f.shadow[lo] = (uint8_t) sbits;
+    if (mask & QBT_OUT_FLAGS) f.flags[lo] = (uint8_t) flg;
+    if (mask & QBT_OUT_RGBA8) f.rgba8[lo] = pack_rgba8(col[0], col[1], col[2], col[3]);
+    if (mask & QBT_OUT_RGBAF) f.rgbaf[lo] = make_float4(col[0], col[1], col[2], col[3]);
+    if (mask & QBT_OUT_ISP) f.isp[lo] = isp;
+    if (mask & QBT_OUT_EXP) f.exp[lo] = in.exp[o];
 }
 
 __global__ void k_zero_u32(unsigned* p, int n)
@@ -389,38 +488,68 @@ int persistent_grid(qbt_ctx* c, K kernel, int bs, size_t lds)
     return per * c->num_cus;
 }
 
+/* the launch configuration a captured graph depends on (kernel arguments) */
+struct GraphKey {
+    SceneDev s;
+    void*    bufs[16];
+    uint32_t nitems, npix, nviews, nlights;
+    unsigned mask, flags;
+    int      bs, refill, gp, gs, stack, reorder;
+    float    band[2];
+    float    lights[kNL][3];
+    uint32_t tmpl;
+};
+
 } // namespace
 
-/* internal per-view buffers, sized for the largest frame so far */
+void qbt_ctx::graph_free()
+{
+    if (graph_exec) (void) hipGraphExecDestroy(graph_exec);
+    if (graph) (void) hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph      = nullptr;
+    graph_key.clear();
+}
+
+/* internal per-batch buffers, sized for the largest batch so far */
 int qbt_ctx::ensure_wave_bufs(size_t npix, int nlights)
 {
-    int rc;
+    int          rc;
+    const size_t nl = (size_t) (nlights > 0 ? nlights : 1);
     if ((rc = dev_alloc(wb_isp, npix * 16)) || (rc = dev_alloc(wb_ms, npix * 4)) || (rc = dev_alloc(wb_md, npix * 4)) ||
         (rc = dev_alloc(wb_leaf, npix * 4)) || (rc = dev_alloc(wb_flags, npix)) || (rc = dev_alloc(wb_disc, npix)) ||
-        (rc = dev_alloc(wb_lstat, npix * (size_t) (nlights > 0 ? nlights : 1))) || (rc = dev_alloc(wb_exp, npix * 4)))
+        (rc = dev_alloc(wb_lstat, npix * nl)) || (rc = dev_alloc(wb_exp, npix * 4)))
         return rc;
     return QBT_OK;
 }
 
-template <bool CMP, bool DUAL, bool GRID, bool COUNT>
-static int render_view_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const int* spr, FrameDev f, unsigned mask,
-                         unsigned flags)
+template <bool CMP, bool DUAL, bool GRID, bool COUNT, bool GSTK>
+static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, const BatchDev& b, const LightsDev& L,
+                          unsigned mask, unsigned flags)
 {
-    hipStream_t  st      = c->stream;
-    const size_t npix    = (size_t) v.w * v.h;
-    const int    tiles_x = (v.w + 7) >> 3, tiles_y = (v.h + 7) >> 3;
-    const size_t items   = (size_t) tiles_x * tiles_y * 64;
-    int          rc;
-    if ((rc = c->ensure_wave_bufs(npix, v.nlights))) return rc;
-    const bool surrogate = (flags & QBT_RENDER_SURROGATE) && v.nlights > 0;
-    if ((rc = c->dev_alloc(c->wb_sel, items)) || (rc = c->dev_alloc(c->wb_items, items * 4)) ||
-        (rc = c->dev_alloc(c->wb_q, sizeof(unsigned) * Q_N)))
+    hipStream_t    st        = c->stream;
+    const uint32_t npix      = bh.npix, nitems = bh.nitems;
+    const int      nl        = L.n;
+    const bool     surrogate = (flags & QBT_RENDER_SURROGATE) && nl > 0;
+    const bool     collect   = (flags & QBT_RENDER_COLLECT) && nl > 0 && !surrogate;
+    const bool     reorder   = c->tune_reorder && nl > 0;
+    int            rc;
+    /* every buffer exists before a capture starts */
+    if ((rc = c->ensure_wave_bufs(npix, nl)) || (rc = c->dev_alloc(c->wb_cand, (size_t) nitems * 4)) ||
+        (rc = c->dev_alloc(c->wb_items, (size_t) npix * 4)) || (rc = c->dev_alloc(c->wb_q, sizeof(unsigned) * Q_N)))
         return rc;
-    if (surrogate && (rc = c->dev_alloc(c->wb_fb, items * 4 * (size_t) v.nlights))) return rc;
-    size_t tmp = 0;
-    HIPCHK(rocprim::select(nullptr, tmp, rocprim::counting_iterator<uint32_t>(0u), (const uint8_t*) c->wb_sel.p,
-                           (uint32_t*) c->wb_items.p, (unsigned*) c->wb_q.p + Q_NSEL, items, st));
-    if ((rc = c->dev_alloc(c->wb_tmp, tmp))) return rc;
+    if (reorder && (rc = c->dev_alloc(c->wb_lists, (size_t) npix * 4 * nl))) return rc;
+    if (surrogate && (rc = c->dev_alloc(c->wb_fb, (size_t) npix * 4 * nl))) return rc;
+
+    const int      bs     = c->block_threads(s.maxlevel, DUAL);
+    const size_t   lds    = GSTK ? 0 : qbt_stack_bytes(s.maxlevel, DUAL, bs);
+    const unsigned refill = (unsigned) (c->tune_refill > 0 && c->tune_refill <= 64 ? c->tune_refill : 32);
+    auto           kp     = k_primary<CMP, DUAL, GRID, COUNT, GSTK>;
+    auto           ks     = k_shadow<CMP, DUAL, GRID, COUNT, GSTK>;
+    const int      gp     = persistent_grid(c, kp, bs, lds);
+    const int      gs     = persistent_grid(c, ks, bs, lds);
+    if (GSTK && (rc = c->dev_alloc(c->wb_gstack, qbt_stack_bytes(s.maxlevel, DUAL, (gp > gs ? gp : gs) * bs))))
+        return rc;
 
     PrimOut po;
     po.isp   = (float4*) c->wb_isp.p;
@@ -428,60 +557,108 @@ static int render_view_t(qbt_ctx* c, const SceneDev& s, const ViewDev& v, const 
     po.md    = (int*) c->wb_md.p;
     po.leaf  = (int*) c->wb_leaf.p;
     po.flags = (uint8_t*) c->wb_flags.p;
-    po.sel   = (uint8_t*) c->wb_sel.p;
-    po.exp   = (COUNT && (mask & QBT_OUT_EXP)) ? f.exp : (int*) c->wb_exp.p;
-    unsigned*           q   = (unsigned*) c->wb_q.p;
-    unsigned long long* cnt = (unsigned long long*) c->counters.p;
+    po.cand  = (uint32_t*) c->wb_cand.p;
+    po.exp   = (int*) c->wb_exp.p;
+    unsigned*           q     = (unsigned*) c->wb_q.p;
+    unsigned long long* cnt   = (unsigned long long*) c->counters.p;
+    uint32_t*           items = (uint32_t*) c->wb_items.p;
+    uint32_t*           gstk  = (uint32_t*) c->wb_gstack.p;
+    /* shadow lists: per light (octant order) or one shared list */
+    const uint32_t* lists   = reorder ? (const uint32_t*) c->wb_lists.p : items;
+    const size_t    lstride = reorder ? npix : 0;
 
-    k_zero_u32<<<1, 128, 0, st>>>(q, Q_N);
-    k_zero_u8<<<nblk(npix), 256, 0, st>>>((uint8_t*) c->wb_disc.p, npix);
-    HIPCHK(hipGetLastError());
-
-    const int    bs  = c->block_threads(s.maxlevel, DUAL);
-    const size_t lds = qbt_stack_bytes(s.maxlevel, DUAL, bs);
-    const unsigned refill = (unsigned) (c->tune_refill > 0 && c->tune_refill <= 64 ? c->tune_refill : 32);
-    {
-        auto kp = k_primary<CMP, DUAL, GRID, COUNT>;
-        int  g  = persistent_grid(c, kp, bs, lds);
+    /* the launches of one batch; traversal kernels bracketed for qbt_profile_*
+     * (no-ops unless profiling, which is never on while capturing) */
+    auto enqueue = [&]() -> int {
+        k_zero_u32<<<1, 256, 0, st>>>(q, Q_N);
+        k_zero_u8<<<nblk(npix), 256, 0, st>>>((uint8_t*) c->wb_disc.p, npix);
         c->prof_mark_begin(QBT_PROF_PRIMARY);
-        kp<<<g, bs, lds, st>>>(s, v, po, q, refill, cnt);
+        kp<<<gp, bs, lds, st>>>(s, b, po, nitems, q, refill, gstk, cnt);
         c->prof_mark_end();
+        k_compact<<<nblk(nitems), 256, 0, st>>>(po.cand, nitems, items, q + Q_NSEL);
+        if (reorder) {
+            k_oct_bin<false><<<nblk(npix), 256, 0, st>>>(items, q + Q_NSEL, po.isp, L, q + Q_OCT, nullptr, 0);
+            k_oct_scan<<<1, 64, 0, st>>>(q + Q_OCT, q + Q_CUR, nl);
+            k_oct_bin<true><<<nblk(npix), 256, 0, st>>>(items, q + Q_NSEL, po.isp, L, q + Q_CUR,
+                                                        (uint32_t*) c->wb_lists.p, npix);
+        }
+        int err;
+        if (surrogate && (err = qbt_surrogate_decide(c, s, DUAL, L, lists, lstride, q + Q_NSEL, npix,
+                                                     (uint32_t*) c->wb_fb.p, npix, q + Q_FB)))
+            return err;
+        for (int l = 0; l < nl; l++) {
+            /* exact mode: every shaded pixel; surrogate mode: this light's undecided ones */
+            const uint32_t* li = surrogate ? (const uint32_t*) c->wb_fb.p + (size_t) l * npix : lists + l * lstride;
+            const unsigned* ln = surrogate ? q + Q_FB + l : q + Q_NSEL;
+            c->prof_mark_begin(QBT_PROF_SHADOW);
+            ks<<<gs, bs, lds, st>>>(s, L.p[l][0], L.p[l][1], L.p[l][2], l, li, ln, po.isp,
+                                    (uint8_t*) c->wb_lstat.p + (size_t) l * npix, (uint8_t*) c->wb_disc.p, q,
+                                    refill, gstk, po.exp, cnt);
+            c->prof_mark_end();
+        }
+        if (collect && (err = qbt_collect_batch(c, s, DUAL, L, lists, lstride, q + Q_NSEL, npix))) return err;
+        k_shade<DUAL><<<nblk(npix), 256, 0, st>>>(s, b, po, (const uint8_t*) c->wb_lstat.p, npix, mask);
+        return QBT_OK;
+    };
+
+    const bool use_graph = c->tune_graph && !COUNT && !collect && !c->profiling;
+    if (!use_graph) {
+        if ((rc = enqueue())) return rc;
         HIPCHK(hipGetLastError());
+        c->last.launches += 5 + nl + (reorder ? 3 : 0) + (surrogate ? 1 : 0);
+        return QBT_OK;
     }
-    size_t tb = tmp;
-    HIPCHK(rocprim::select(c->wb_tmp.p, tb, rocprim::counting_iterator<uint32_t>(0u), (const uint8_t*) c->wb_sel.p,
-                           (uint32_t*) c->wb_items.p, q + Q_NSEL, items, st));
-    if (surrogate && (rc = qbt_surrogate_decide(c, s, DUAL, v, (const uint32_t*) c->wb_items.p, q, Q_NSEL, npix,
-                                                (uint32_t*) c->wb_fb.p, items, q + Q_FB)))
-        return rc;
-    for (int l = 0; l < v.nlights; l++) {
-        auto ks = k_shadow<CMP, DUAL, GRID, COUNT>;
-        int  g  = persistent_grid(c, ks, bs, lds);
-        /* exact mode: every shaded pixel; surrogate mode: this light's unsure ones */
-        const uint32_t* li = surrogate ? (const uint32_t*) c->wb_fb.p + (size_t) l * items : (const uint32_t*) c->wb_items.p;
-        const unsigned* ln = surrogate ? q + Q_FB + l : q + Q_NSEL;
-        c->prof_mark_begin(QBT_PROF_SHADOW);
-        ks<<<g, bs, lds, st>>>(s, v, l, li, ln, (const float4*) c->wb_isp.p,
-                               (uint8_t*) c->wb_lstat.p + (size_t) l * npix, (uint8_t*) c->wb_disc.p, q, refill,
-                               po.exp, cnt);
-        c->prof_mark_end();
-        HIPCHK(hipGetLastError());
+
+    /* graph replay when the launch configuration is unchanged, else capture */
+    GraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.s      = s;
+    void* bp[] = {c->wb_isp.p,   c->wb_ms.p,    c->wb_md.p,   c->wb_leaf.p,  c->wb_flags.p, c->wb_disc.p,
+                  c->wb_lstat.p, c->wb_exp.p,   c->wb_cand.p, c->wb_items.p, c->wb_q.p,     c->wb_lists.p,
+                  c->wb_fb.p,    c->wb_gstack.p, c->bt_tab.p, c->sprite.p};
+    static_assert(sizeof(bp) == sizeof(key.bufs), "graph key buffers");
+    memcpy(key.bufs, bp, sizeof(bp));
+    key.nitems = nitems, key.npix = npix, key.nviews = (uint32_t) b.n, key.nlights = (uint32_t) nl;
+    key.mask = mask, key.flags = flags, key.bs = bs, key.refill = (int) refill, key.gp = gp, key.gs = gs;
+    key.stack = GSTK, key.reorder = reorder;
+    memcpy(key.lights, L.p, sizeof(key.lights));
+    key.tmpl = (CMP ? 1u : 0u) | (DUAL ? 2u : 0u) | (GRID ? 4u : 0u);
+    if (surrogate) qbt_surrogate_band(c, &key.band[0], &key.band[1]);
+    const uint8_t* kb = reinterpret_cast<const uint8_t*>(&key);
+    if (c->graph_exec && c->graph_key.size() == sizeof(key) && memcmp(c->graph_key.data(), kb, sizeof(key)) == 0) {
+        c->graph_replays++;
+    } else {
+        c->graph_free();
+        (void) hipGetLastError();
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int        err = enqueue();
+        const hipError_t le  = hipGetLastError();
+        hipGraph_t       g   = nullptr;
+        const hipError_t ce  = hipStreamEndCapture(st, &g);
+        if (err || le != hipSuccess || ce != hipSuccess) {
+            if (g) (void) hipGraphDestroy(g);
+            if (err) return err;
+            HIPCHK(le != hipSuccess ? le : ce);
+        }
+        c->graph = g;
+        HIPCHK(hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+        c->graph_key.assign(kb, kb + sizeof(key));
     }
-    if ((flags & QBT_RENDER_COLLECT) && !surrogate && v.nlights > 0 &&
-        (rc = qbt_collect_view(c, s, DUAL, v, (const uint32_t*) c->wb_items.p, q, Q_NSEL, npix)))
-        return rc;
-    k_shade<DUAL><<<nblk(npix), 256, 0, st>>>(s, v, spr, po, (const uint8_t*) c->wb_lstat.p, npix, f, mask);
-    HIPCHK(hipGetLastError());
-    c->last.launches += 3 + v.nlights + (surrogate ? 1 : 0);
+    HIPCHK(hipGraphLaunch(c->graph_exec, st));
+    c->last.launches += 1;
     return QBT_OK;
 }
 
-int qbt_ctx::render_view_wavefront(const SceneDev& s, bool cmp, bool dual, bool grid, bool count, const ViewDev& v,
-                                   const int* spr, FrameDev f, unsigned mask, unsigned flags)
+int qbt_ctx::render_batch_wavefront(const SceneDev& s, bool cmp, bool dual, bool grid, bool count,
+                                    const BatchHost& bh, const BatchDev& b, const LightsDev& L, unsigned mask,
+                                    unsigned flags)
 {
-    if (v.w <= 0 || v.h <= 0) return QBT_OK;
+    if (bh.npix == 0) return QBT_OK;
+    const bool gstk = tune_stack == 1;
     return qbt_dispatch4(cmp, dual, grid, count, [&](auto C_, auto D_, auto G_, auto N_) {
-        return render_view_t<decltype(C_)::value, decltype(D_)::value, decltype(G_)::value, decltype(N_)::value>(
-            this, s, v, spr, f, mask, flags);
+        return qbt_dispatch1(gstk, [&](auto S_) {
+            return render_batch_t<decltype(C_)::value, decltype(D_)::value, decltype(G_)::value, decltype(N_)::value,
+                                  decltype(S_)::value>(this, s, bh, b, L, mask, flags);
+        });
     });
 }

@@ -408,68 +408,66 @@ __global__ void __launch_bounds__(256) k_mlp_infer(InferArgs ia, const uint16_t*
 
 /* The decision pass: for each (shaded pixel, light) query, 16 per wave,
  * predicted bits outside the band, the others appended to the light's
- * exact-trace list (one atomic per light and wave). */
+ * exact-trace list (one atomic per light and wave). Pixels of light l:
+ * lists[l * lstride + i], i < *nsel. */
 template <bool DUAL>
-__global__ void __launch_bounds__(256) k_mlp_shadow(InferArgs ia, SceneDev s, ViewDev v,
-                                                    const uint32_t* __restrict__ items, const unsigned* __restrict__ q,
-                                                    int qnsel, const float4* __restrict__ isp, const int* __restrict__ ms,
-                                                    const int* __restrict__ md, uint8_t* __restrict__ lstat,
-                                                    size_t npix, uint32_t* __restrict__ fb_items, size_t fb_stride,
+__global__ void __launch_bounds__(256) k_mlp_shadow(InferArgs ia, SceneDev s, LightsDev L,
+                                                    const uint32_t* __restrict__ lists, size_t lstride,
+                                                    const unsigned* __restrict__ nsel, const float4* __restrict__ isp,
+                                                    const int* __restrict__ ms, const int* __restrict__ md,
+                                                    uint8_t* __restrict__ lstat, size_t npix,
+                                                    uint32_t* __restrict__ fb, size_t fb_stride,
                                                     unsigned* __restrict__ fb_count, float lo_logit, float hi_logit,
                                                     unsigned long long* __restrict__ counters)
 {
     __shared__ uint16_t Ws[kPackN];
     __shared__ float    Bs[kLBN];
     load_weights_lds(ia, Ws, Bs);
-    const uint32_t nq      = q[qnsel];
-    const uint32_t total   = nq * (uint32_t) v.nlights;
-    const int      lane    = threadIdx.x & 63;
-    const int      wave    = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int      nw      = (gridDim.x * blockDim.x) >> 6;
-    const int      r       = lane & 15, g = lane >> 4;
-    const int      tiles_x = (v.w + 7) >> 3;
-    unsigned       n_pred  = 0;
+    const uint32_t nq     = *nsel;
+    const uint32_t total  = nq * (uint32_t) L.n;
+    const int      lane   = threadIdx.x & 63;
+    const int      wave   = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int      nw     = (gridDim.x * blockDim.x) >> 6;
+    const int      r      = lane & 15, g = lane >> 4;
+    unsigned       n_pred = 0;
     for (uint32_t q0 = (uint32_t) wave * 16u; q0 < total; q0 += (uint32_t) nw * 16u) {
         const uint32_t qi    = q0 + (uint32_t) r;
         const bool     valid = qi < total;
         int            l     = 0;
-        uint32_t       pix = 0, item = 0;
+        uint32_t       gp    = 0;
         float          u[3] = {0.f, 0.f, 0.f}, nrm[3] = {0.f, 0.f, 0.f}, lu[3] = {0.f, 0.f, 0.f};
         if (valid) {
-            l    = (int) (qi / nq);
-            item = items[qi - (uint32_t) l * nq];
-            int px, py;
-            item_pixel(item, tiles_x, v.w, v.h, px, py);
-            pix                = (uint32_t) py * (uint32_t) v.w + (uint32_t) px;
-            const float4 ip    = isp[pix];
+            l                  = (int) (qi / nq);
+            gp                 = lists[(size_t) l * lstride + (qi - (uint32_t) l * nq)];
+            const float4 ip    = isp[gp];
             const float  p3[3] = {ip.x, ip.y, ip.z};
             mlp_scale(p3, ia.bx, ia.by, ia.bz, ia.bw, u);
             float col[4], n4[4];
-            fetch_attr(s.astat, ms[pix], col, n4);
-            if (DUAL && md[pix] > 0) fetch_attr(s.adyn, md[pix], col, n4);
+            fetch_attr(s.astat, ms[gp], col, n4);
+            if (DUAL && md[gp] > 0) fetch_attr(s.adyn, md[gp], col, n4);
             nrm[0] = n4[0], nrm[1] = n4[1], nrm[2] = n4[2];
-            mlp_scale(v.light[l], ia.bx, ia.by, ia.bz, ia.bw, lu);
+            mlp_scale(L.p[l], ia.bx, ia.by, ia.bz, ia.bw, lu);
         }
         bf16x4 xin[3];
         query_frag(u, nrm, lu, g, xin);
         const float logit  = mlp_logit(Ws, Bs, xin, lane);
         bool        unsure = false;
         if (g == 0 && valid) {
-            if (logit < lo_logit) lstat[(size_t) l * npix + pix] = LS_PRED_OCC, n_pred++;
-            else if (logit > hi_logit) lstat[(size_t) l * npix + pix] = LS_PRED_LIT, n_pred++;
+            if (logit < lo_logit) lstat[(size_t) l * npix + gp] = LS_PRED_OCC, n_pred++;
+            else if (logit > hi_logit) lstat[(size_t) l * npix + gp] = LS_PRED_LIT, n_pred++;
             else unsure = true;
         }
         uint64_t pending = __ballot(unsure);
         while (pending) {
             const int      leader = __ffsll((unsigned long long) pending) - 1;
-            const int      L      = __shfl(l, leader, 64);
-            const uint64_t mine   = __ballot(unsure && l == L);
+            const int      Lq     = __shfl(l, leader, 64);
+            const uint64_t mine   = __ballot(unsure && l == Lq);
             uint32_t       base   = 0;
-            if (lane == leader) base = atomicAdd(&fb_count[L], (unsigned) __popcll(mine));
+            if (lane == leader) base = atomicAdd(&fb_count[Lq], (unsigned) __popcll(mine));
             base = __shfl(base, leader, 64);
-            if (unsure && l == L) {
+            if (unsure && l == Lq) {
                 const uint32_t rank = (uint32_t) __popcll(mine & ((1ull << lane) - 1ull));
-                fb_items[(size_t) L * fb_stride + base + rank] = item;
+                fb[(size_t) Lq * fb_stride + base + rank] = gp;
             }
             pending &= ~mine;
         }
@@ -481,35 +479,31 @@ __global__ void __launch_bounds__(256) k_mlp_shadow(InferArgs ia, SceneDev s, Vi
 /* training samples of an exact render: queries stride * k + phase of the
  * (light, shaded pixel) sequence, label = the exact visibility */
 template <bool DUAL>
-__global__ void k_collect(SceneDev s, ViewDev v, const uint32_t* __restrict__ items, const unsigned* __restrict__ q,
-                          int qnsel, const float4* __restrict__ isp, const int* __restrict__ ms,
-                          const int* __restrict__ md, const uint8_t* __restrict__ lstat, size_t npix, uint32_t stride,
-                          uint32_t phase, RawSample* __restrict__ out, size_t cap,
+__global__ void k_collect(SceneDev s, LightsDev L, const uint32_t* __restrict__ lists, size_t lstride,
+                          const unsigned* __restrict__ nsel, const float4* __restrict__ isp,
+                          const int* __restrict__ ms, const int* __restrict__ md, const uint8_t* __restrict__ lstat,
+                          size_t npix, uint32_t stride, uint32_t phase, RawSample* __restrict__ out, size_t cap,
                           unsigned long long* __restrict__ ctr)
 {
-    const uint64_t nq = q[qnsel];
+    const uint64_t nq = *nsel;
     const uint64_t qi = ((uint64_t) blockIdx.x * blockDim.x + threadIdx.x) * stride + phase;
-    if (qi >= nq * (uint64_t) v.nlights) return;
-    const int      l       = (int) (qi / nq);
-    const uint32_t item    = items[qi - (uint64_t) l * nq];
-    const int      tiles_x = (v.w + 7) >> 3;
-    int            px, py;
-    item_pixel(item, tiles_x, v.w, v.h, px, py);
-    const uint32_t pix = (uint32_t) py * (uint32_t) v.w + (uint32_t) px;
-    const uint8_t  ls  = lstat[(size_t) l * npix + pix];
+    if (qi >= nq * (uint64_t) L.n) return;
+    const int      l  = (int) (qi / nq);
+    const uint32_t gp = lists[(size_t) l * lstride + (qi - (uint64_t) l * nq)];
+    const uint8_t  ls = lstat[(size_t) l * npix + gp];
     if (ls != LS_LIT && ls != LS_OCCLUDED) return;
     const unsigned long long slot = atomicAdd(&ctr[0], 1ull);
     if (slot >= cap) return;
-    RawSample    r;
-    const float4 ip = isp[pix];
-    r.isp[0] = ip.x, r.isp[1] = ip.y, r.isp[2] = ip.z;
+    RawSample    rs;
+    const float4 ip = isp[gp];
+    rs.isp[0] = ip.x, rs.isp[1] = ip.y, rs.isp[2] = ip.z;
     float col[4], n4[4];
-    fetch_attr(s.astat, ms[pix], col, n4);
-    if (DUAL && md[pix] > 0) fetch_attr(s.adyn, md[pix], col, n4);
-    r.nrm[0] = n4[0], r.nrm[1] = n4[1], r.nrm[2] = n4[2];
-    r.light[0] = v.light[l][0], r.light[1] = v.light[l][1], r.light[2] = v.light[l][2];
-    r.label    = ls == LS_LIT ? 1.0f : 0.0f;
-    out[slot]  = r;
+    fetch_attr(s.astat, ms[gp], col, n4);
+    if (DUAL && md[gp] > 0) fetch_attr(s.adyn, md[gp], col, n4);
+    rs.nrm[0] = n4[0], rs.nrm[1] = n4[1], rs.nrm[2] = n4[2];
+    rs.light[0] = L.p[l][0], rs.light[1] = L.p[l][1], rs.light[2] = L.p[l][2];
+    rs.label    = ls == LS_LIT ? 1.0f : 0.0f;
+    out[slot]   = rs;
     if (ls == LS_LIT) atomicAdd(&ctr[1], 1ull);
 }
 
@@ -871,16 +865,16 @@ extern "C" int qbt_surrogate_step(qbt_ctx* c, const float* x, const float* y, si
     return QBT_OK;
 }
 
-int qbt_collect_view(qbt_ctx* c, const SceneDev& s, bool dual, const ViewDev& v, const uint32_t* items,
-                     const unsigned* q, int qnsel, size_t npix)
+int qbt_collect_batch(qbt_ctx* c, const SceneDev& s, bool dual, const LightsDev& L, const uint32_t* lists,
+                      size_t list_stride, const unsigned* nsel, size_t npix)
 {
     MlpState* m = c->mlp;
     if (!m || !m->cap) return QBT_OK;
     const uint32_t stride = m->stride, phase = m->phase;
-    m->phase = (m->phase + 1) % stride; /* successive views sample different queries */
-    const size_t threads = (npix * (size_t) v.nlights) / stride + 1;
+    m->phase = (m->phase + 1) % stride; /* successive renders sample different queries */
+    const size_t threads = (npix * (size_t) L.n) / stride + 1;
     auto         go      = [&](auto kern) {
-        kern<<<nblk(threads), 256, 0, c->stream>>>(s, v, items, q, qnsel, (const float4*) c->wb_isp.p,
+        kern<<<nblk(threads), 256, 0, c->stream>>>(s, L, lists, list_stride, nsel, (const float4*) c->wb_isp.p,
                                                    (const int*) c->wb_ms.p, (const int*) c->wb_md.p,
                                                    (const uint8_t*) c->wb_lstat.p, npix, stride, phase,
                                                    (RawSample*) m->samples.p, m->cap, (unsigned long long*) m->ctr.p);
@@ -939,8 +933,14 @@ extern "C" int qbt_surrogate_train(qbt_ctx* c, const float basecube[4], int step
     return QBT_OK;
 }
 
-int qbt_surrogate_decide(qbt_ctx* c, const SceneDev& s, bool dual, const ViewDev& v, const uint32_t* items,
-                         const unsigned* q, int qnsel, size_t npix, uint32_t* fb_items, size_t fb_stride,
+void qbt_surrogate_band(const qbt_ctx* c, float* lo, float* hi)
+{
+    *lo = c->mlp ? c->mlp->band_lo : 0.0f;
+    *hi = c->mlp ? c->mlp->band_hi : 0.0f;
+}
+
+int qbt_surrogate_decide(qbt_ctx* c, const SceneDev& s, bool dual, const LightsDev& L, const uint32_t* lists,
+                         size_t list_stride, const unsigned* nsel, size_t npix, uint32_t* fb, size_t fb_stride,
                          unsigned* fb_count)
 {
     MlpState* m = c->mlp;
@@ -951,15 +951,13 @@ int qbt_surrogate_decide(qbt_ctx* c, const SceneDev& s, bool dual, const ViewDev
     const float hi   = logf(m->band_hi / (1.0f - m->band_hi));
     const int   grid = c->num_cus * 4;
     auto        go   = [&](auto kern) {
-        kern<<<grid, 256, 0, c->stream>>>(ia, s, v, items, q, qnsel, (const float4*) c->wb_isp.p,
+        kern<<<grid, 256, 0, c->stream>>>(ia, s, L, lists, list_stride, nsel, (const float4*) c->wb_isp.p,
                                           (const int*) c->wb_ms.p, (const int*) c->wb_md.p, (uint8_t*) c->wb_lstat.p,
-                                          npix, fb_items, fb_stride, fb_count, lo, hi,
-                                          (unsigned long long*) c->counters.p);
+                                          npix, fb, fb_stride, fb_count, lo, hi, (unsigned long long*) c->counters.p);
     };
     c->prof_mark_begin(QBT_PROF_SURROGATE);
     if (dual) go(k_mlp_shadow<true>);
     else go(k_mlp_shadow<false>);
     c->prof_mark_end();
-    HIPCHK(hipGetLastError());
     return QBT_OK;
 }

@@ -19,14 +19,16 @@ pytestmark = pytest.mark.gpu
 ALL = q.OUT_ALL | q.RENDER_COUNT
 
 
-@pytest.fixture(params=[(1, 0), (1, 1), (0, 1)], ids=["wavefront-compact", "wavefront-direct", "fused"])
+@pytest.fixture(params=[dict(path=1, layout=0), dict(path=1, layout=1), dict(path=0, layout=1),
+                        dict(path=1, layout=0, stack=1, reorder=0)],
+                ids=["wavefront-compact", "wavefront-direct", "fused", "wavefront-gstack"])
 def ctx(gpu_ctx, request):
     """Every render path: wavefront stages on the compact node layout
-    (default), wavefront on the direct layout, and the fused kernel."""
-    path, layout = request.param
-    gpu_ctx.set_tuning(path=path, layout=layout)
+    (default), wavefront on the direct layout, the fused kernel, and the
+    wavefront stages with global-memory stacks and unordered shadow rays."""
+    gpu_ctx.set_tuning(**request.param)
     yield gpu_ctx
-    gpu_ctx.set_tuning(path=1, layout=0)
+    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=1, graph=1)
 
 
 def test_shared_child_falls_back_to_direct(ctx):
