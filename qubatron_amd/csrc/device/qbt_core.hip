@@ -1130,8 +1130,10 @@ extern "C" int qbt_autotune(qbt_ctx* c, const qbt_view* views, int nviews, const
     qbt_tuning best;
     int        rc = qbt_get_tuning(c, &best);
     if (rc) return rc;
-    best.path   = 1;
-    double best_ms = 0.0;
+    best.path          = 1;
+    best.block_threads = best.block_threads ? best.block_threads : 256; /* the defaults, made explicit */
+    best.refill        = best.refill ? best.refill : 32;
+    double best_ms     = 0.0;
     int    n       = 0;
     auto   run     = [&](const qbt_tuning& t, double* ms) -> int {
         int e = qbt_set_tuning(c, &t);
@@ -1153,9 +1155,7 @@ extern "C" int qbt_autotune(qbt_ctx* c, const qbt_view* views, int nviews, const
                 qbt_tuning t = best;
                 int*       f = knob == 0 ? &t.block_threads : knob == 1 ? &t.refill : knob == 2 ? &t.stack : &t.reorder;
                 const int  v = vals ? vals[k] : k;
-                const int  cur = knob == 0 ? (best.block_threads ? best.block_threads : 256)
-                                           : knob == 1 ? (best.refill ? best.refill : 32) : *f;
-                if (v == cur) continue;
+                if (v == *f) continue;
                 *f = v;
                 double ms = 0.0;
                 if ((rc = run(t, &ms))) return rc;
