@@ -158,7 +158,8 @@ int qbt_ctx::relayout(int t)
     HIPCHK(hipMemcpyAsync(&nbad, bad.p, 4, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     cleanup();
-    cm.ok = ok && nbad == 0;
+    /* the single-tree traversal keeps first-child indices in 30 bits */
+    cm.ok = ok && nbad == 0 && hi < (1u << 30);
     cm.n  = cm.ok ? hi : 0;
     return QBT_OK;
 }

@@ -303,6 +303,158 @@ __device__ __forceinline__ int trace_begin(const SceneDev& s, const Ray& r, Trac
 }
 
 /*
+ * Single-tree steps on the compact layout (the hot configuration) keep a
+ * different per-level encoding, so that backtracking never re-reads a node:
+ *   ns word  first child index of the level's node (bits 0-29) | entries
+ *            still pending (bits 30-31, 0..3)
+ *   pw word  the entry being visited (bits 0-7), then up to three pending
+ *            entries (bits 8-15, 16-23, 24-31); an entry is octet (3 bits) |
+ *            source << 3 (2 bits) | rank << 5 (3 bits), rank = the child's
+ *            position among the node's children, so the child is
+ *            first + rank.
+ * Candidates are sorted as (t, key) pairs; each key carries the candidate's
+ * octet and its tie-flipped octet, computed before the sort from the same
+ * comparisons (octree_fsh.c:292-311), and its source; the entry point of the
+ * first child is selected by source afterwards. Same results, fewer moves.
+ */
+template <bool CPU, bool GRID>
+__device__ __forceinline__ float4 entry_of_c1(const SceneDev& s, const Ray& r, const Stack& st, int level, int ix,
+                                           int iy, int iz, float4 start)
+{
+    for (int m = level; m > 0; m--) {
+        const int src = (int) ((st.pw[(m - 1) * st.stride] >> 3) & 3u);
+        if (src != 0) {
+            const int sh = level - (m - 1);
+            float     x, y, z, ww;
+            cube_at<GRID>(s, m - 1, ix >> sh, iy >> sh, iz >> sh, x, y, z, ww);
+            return midplane<CPU>(src, r, x, y, z, ww);
+        }
+    }
+    return start;
+}
+
+__device__ __forceinline__ void kswap(float& wa, int& ka, float& wb, int& kb)
+{
+    const bool sw = wb < wa; /* strict: ties keep order (octree_fsh.c:283); NaN never moves */
+    const float tw = sw ? wb : wa;
+    const int   tk = sw ? kb : ka;
+    wb             = sw ? wa : wb;
+    kb             = sw ? ka : kb;
+    wa             = tw;
+    ka             = tk;
+}
+
+template <bool CPU, bool GRID>
+__device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, const Stack& st, TraceState& t)
+{
+    if (++t.it > s.iter_cap) return ST_ABORT;
+    const int   level = t.level;
+    const uint2 rec   = s.stat.rec[t.ns];
+    t.exps++;
+    const uint32_t mask = rec.y;
+    if (mask) {
+        float tx, ty, tz, tw;
+        cube_at<GRID>(s, level, t.ix, t.iy, t.iz, tx, ty, tz, tw);
+        const float bxx = tx + tw, byy = ty - tw, bzz = tz - tw;
+        const float hx = bxx + (tx - bxx) * 0.5f;
+        const float hy = byy + (ty - byy) * 0.5f;
+        const float hz = bzz + (tz - bzz) * 0.5f;
+        const float qnan = __builtin_nanf("");
+        auto okey = [&](float x, float y, float z, int src) -> int {
+            const int o = (x > hx ? 1 : 0) | (y < hy ? 2 : 0) | (z < hz ? 4 : 0);
+            const int f = x == hx ? o ^ 1 : y == hy ? o ^ 2 : z == hz ? o ^ 4 : o;
+            return o | (f << 3) | (src << 6);
+        };
+        const float4 az = plane_z<CPU>(hz, r);
+        const float4 ax = plane_x<CPU>(hx, r);
+        const float4 ay = plane_y<CPU>(hy, r);
+        float        w0 = t.entry.w;
+        float        w1 = (az.w > 0.0f && tx < az.x && az.x <= bxx && ty > az.y && az.y >= byy) ? az.w : qnan;
+        float        w2 = (ax.w > 0.0f && ty > ax.y && ax.y >= byy && tz > ax.z && ax.z >= bzz) ? ax.w : qnan;
+        float        w3 = (ay.w > 0.0f && tx < ay.x && ay.x <= bxx && tz > ay.z && ay.z >= bzz) ? ay.w : qnan;
+        int          k0 = okey(t.entry.x, t.entry.y, t.entry.z, 0);
+        int          k1 = okey(az.x, az.y, az.z, 1);
+        int          k2 = okey(ax.x, ax.y, ax.z, 2);
+        int          k3 = okey(ay.x, ay.y, ay.z, 3);
+        kswap(w0, k0, w1, k1);
+        kswap(w0, k0, w2, k2);
+        kswap(w0, k0, w3, k3);
+        kswap(w1, k1, w2, k2);
+        kswap(w1, k1, w3, k3);
+        kswap(w2, k2, w3, k3);
+
+        int      pre = -1, n = 0, src0 = 0;
+        uint32_t word = 0, rank0 = 0;
+        int      oct0 = 0;
+#define QBT_CLASSIFY(w, k)                                                       \
+    if ((w) == (w)) {                                                            \
+        int o = (k) & 7;                                                         \
+        if (o == pre) o = ((k) >> 3) & 7;                                        \
+        pre = o;                                                                 \
+        if ((mask >> o) & 1u) {                                                  \
+            const uint32_t rk = (uint32_t) __popc(mask & ((1u << o) - 1u));      \
+            word |= ((uint32_t) o | ((uint32_t) ((k) >> 6) << 3) | (rk << 5)) << (8 * n); \
+            if (n == 0) {                                                        \
+                src0  = (k) >> 6;                                                \
+                oct0  = o;                                                       \
+                rank0 = rk;                                                      \
+            }                                                                    \
+            n++;                                                                 \
+        }                                                                        \
+    }
+        QBT_CLASSIFY(w0, k0)
+        QBT_CLASSIFY(w1, k1)
+        QBT_CLASSIFY(w2, k2)
+        QBT_CLASSIFY(w3, k3)
+#undef QBT_CLASSIFY
+
+        if (n > 0) {
+            st.pw[level * st.stride] = word;
+            st.ns[level * st.stride] = rec.x | ((uint32_t) (n - 1) << 30);
+            t.entry = src0 == 0 ? t.entry : src0 == 1 ? az : src0 == 2 ? ax : ay;
+            t.ns    = (int) (rec.x + rank0);
+            t.ix    = (t.ix << 1) | (oct0 & 1);
+            t.iy    = (t.iy << 1) | ((oct0 >> 1) & 1);
+            t.iz    = (t.iz << 1) | (oct0 >> 2);
+            t.level = level + 1;
+            return t.level == s.maxlevel ? ST_HIT : ST_CONT;
+        }
+    }
+
+    /* backtrack (octree_fsh.c:368-375) and pop the next pending entry */
+    int lv = level, ix = t.ix, iy = t.iy, iz = t.iz;
+    for (;;) {
+        lv--;
+        if (lv < 0) return ST_MISS;
+        ix >>= 1;
+        iy >>= 1;
+        iz >>= 1;
+        const uint32_t a = (uint32_t) st.ns[lv * st.stride];
+        if (a >> 30) {
+            const uint32_t w = st.pw[lv * st.stride] >> 8; /* the next entry becomes the current one */
+            st.pw[lv * st.stride] = w;
+            st.ns[lv * st.stride] = (int) (a - (1u << 30));
+            const int o = (int) (w & 7u), src = (int) ((w >> 3) & 3u);
+            float4    isp;
+            if (src == 0) {
+                isp = entry_of_c1<CPU, GRID>(s, r, st, lv, ix, iy, iz, t.start);
+            } else {
+                float x, y, z, ww;
+                cube_at<GRID>(s, lv, ix, iy, iz, x, y, z, ww);
+                isp = midplane<CPU>(src, r, x, y, z, ww);
+            }
+            t.ns    = (int) ((a & 0x3FFFFFFFu) + ((w >> 5) & 7u));
+            t.ix    = (ix << 1) | (o & 1);
+            t.iy    = (iy << 1) | ((o >> 1) & 1);
+            t.iz    = (iz << 1) | (o >> 2);
+            t.entry = isp;
+            t.level = lv + 1;
+            return t.level == s.maxlevel ? ST_HIT : ST_CONT;
+        }
+    }
+}
+
+/*
  * One step: expand the node just entered and descend into its nearest
  * pending child, or backtrack and descend into the next pending child of an
  * ancestor. Returns ST_CONT, ST_HIT (t.level == maxlevel; the leaf is
@@ -311,6 +463,7 @@ __device__ __forceinline__ int trace_begin(const SceneDev& s, const Ray& r, Trac
 template <bool CMP, bool DUAL, bool CPU, bool GRID>
 __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const Stack& st, TraceState& t)
 {
+    if constexpr (CMP && !DUAL) return trace_step_c1<CPU, GRID>(s, r, st, t);
     if (++t.it > s.iter_cap) return ST_ABORT;
     const int level = t.level;
 
