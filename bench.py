@@ -85,6 +85,9 @@ def parse():
     ap.add_argument("--block", type=int, default=None, help="threads per block (64/128/256)")
     ap.add_argument("--refill", type=int, default=None, help="idle lanes that trigger a wave refill (1..64)")
     ap.add_argument("--layout", type=int, default=None, help="0 compact node layout, 1 direct")
+    ap.add_argument("--stack", type=int, default=None, help="0 LDS traversal stacks, 1 global memory")
+    ap.add_argument("--reorder", type=int, default=None, help="1 shadow rays in direction-octant order")
+    ap.add_argument("--graph", type=int, default=None, help="1 replay captured hipGraphs")
     return ap.parse_args()
 
 
@@ -280,7 +283,8 @@ def main():
     log(f"generated {sc.npoints} points in {time.time() - t0:.2f}s (config {args.config}, seed {args.seed})")
 
     ctx = q.Context(0)
-    ctx.set_tuning(path=args.path, block_threads=args.block, refill=args.refill, layout=args.layout)
+    ctx.set_tuning(path=args.path, block_threads=args.block, refill=args.refill, layout=args.layout,
+                   stack=args.stack, reorder=args.reorder, graph=args.graph)
     tuning = ctx.tuning()
     copy_ms, copy_gbps = ctx.copy_bandwidth(COPY_BYTES, 5)
     log(f"copy kernel: {copy_gbps:.0f} GB/s (read+write) on {COPY_BYTES >> 20} MiB")

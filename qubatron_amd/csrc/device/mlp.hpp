@@ -66,15 +66,23 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
 /* Feature f (0..47) of one query: f = 6k + 2a + {0 sin, 1 cos} for band k
  * (frequency 2^k pi) and axis a of the hit position scaled to [0,1]; 36..38
  * normal; 39..41 light scaled to [0,1]; 42..47 zero padding. */
+/* element a (0..2) of a 3-vector, selected by value (an indexed access with
+ * a lane-dependent index would put the vector in scratch memory) */
+__device__ __forceinline__ float pick3(const float v[3], int a)
+{
+    const float x = v[0], y = v[1], z = v[2];
+    return a == 0 ? x : (a == 1 ? y : z);
+}
+
 __device__ __forceinline__ float mlp_feature(int f, const float u[3], const float n[3], const float lu[3])
 {
     if (f < 36) {
         int   k = f / 6, a = (f % 6) >> 1;
-        float arg = ((float) (1 << k) * 3.14159265358979f) * u[a];
+        float arg = ((float) (1 << k) * 3.14159265358979f) * pick3(u, a);
         return (f & 1) ? cosf(arg) : sinf(arg);
     }
-    if (f < 39) return n[f - 36];
-    if (f < 42) return lu[f - 39];
+    if (f < 39) return pick3(n, f - 36);
+    if (f < 42) return pick3(lu, f - 39);
     return 0.0f;
 }
 

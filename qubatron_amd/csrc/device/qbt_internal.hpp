@@ -165,7 +165,7 @@ struct qbt_ctx {
     hipEvent_t   tab_ev[kTabSlots]   = {};      /* the last copy out of each slot */
     int          tab_slot            = 0;
     int          tune_stack   = 0; /* 0 = LDS, 1 = global memory */
-    int          tune_reorder = 1; /* shadow rays in direction-octant order */
+    int          tune_reorder = 0; /* shadow rays in direction-octant order (measured: no gain) */
     int          tune_graph   = 1; /* replay batched renders from a captured hipGraph */
     /* captured render graph and the launch configuration it was captured for */
     hipGraph_t           graph      = nullptr;

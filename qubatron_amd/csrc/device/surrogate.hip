@@ -299,10 +299,9 @@ struct InferArgs {
     float           bx, by, bz, bw;
 };
 
-__device__ __forceinline__ void load_weights_lds(const InferArgs& ia, uint16_t* Ws, float* Bs)
+/* the biases in LDS (every lane of a row group reads the same four) */
+__device__ __forceinline__ void load_biases_lds(const InferArgs& ia, float* Bs)
 {
-    for (int i = threadIdx.x; i < kPackN / 2; i += blockDim.x)
-        reinterpret_cast<uint32_t*>(Ws)[i] = reinterpret_cast<const uint32_t*>(ia.W)[i];
     for (int i = threadIdx.x; i < kLBN; i += blockDim.x) {
         float b = 0.0f;
         if (i < 64) b = ia.P[kB1 + i];
@@ -314,78 +313,124 @@ __device__ __forceinline__ void load_weights_lds(const InferArgs& ia, uint16_t* 
     __syncthreads();
 }
 
+/* All weights as matrix-core A fragments, held in registers for the whole
+ * kernel (48 fragments, 96 VGPRs): lane l holds W[16 mt + l % 16][16 kb +
+ * 4 (l / 16) + e]. */
+struct MlpFrags {
+    bf16x4 w1[4][3], w2[4][4], w3[4][4], w4[4];
+};
+
+__device__ __forceinline__ void load_frags(const uint16_t* __restrict__ W, int lane, MlpFrags& F)
+{
+    const int r = lane & 15, g = lane >> 4;
+    for (int mt = 0; mt < 4; mt++) {
+        for (int kb = 0; kb < 3; kb++)
+            F.w1[mt][kb] = *reinterpret_cast<const bf16x4*>(W + kP1 + (16 * mt + r) * 48 + 16 * kb + 4 * g);
+        for (int kb = 0; kb < 4; kb++) {
+            F.w2[mt][kb] = *reinterpret_cast<const bf16x4*>(W + kP2 + (16 * mt + r) * 64 + 16 * kb + 4 * g);
+            F.w3[mt][kb] = *reinterpret_cast<const bf16x4*>(W + kP3 + (16 * mt + r) * 64 + 16 * kb + 4 * g);
+        }
+    }
+    for (int kb = 0; kb < 4; kb++) F.w4[kb] = *reinterpret_cast<const bf16x4*>(W + kP4 + r * 64 + 16 * kb + 4 * g);
+}
+
 /* B fragments of the input of 16 queries: lane l holds features 16 kb +
- * 4 (l / 16) + e of query l % 16 */
+ * 4 (l / 16) + e of query l % 16; a sin/cos pair shares one sincosf */
 __device__ __forceinline__ void query_frag(const float u[3], const float nrm[3], const float lu[3], int g,
                                            bf16x4 xin[3])
 {
     for (int kb = 0; kb < 3; kb++) {
         float f[4];
-        for (int e = 0; e < 4; e++) f[e] = mlp_feature(16 * kb + 4 * g + e, u, nrm, lu);
+        for (int p = 0; p < 2; p++) {
+            const int fi = 16 * kb + 4 * g + 2 * p; /* even */
+            if (fi < 36) {
+                const int   k = fi / 6, a = (fi % 6) >> 1;
+                const float arg = ((float) (1 << k) * 3.14159265358979f) * pick3(u, a);
+                sincosf(arg, &f[2 * p], &f[2 * p + 1]);
+            } else {
+                f[2 * p]     = mlp_feature(fi, u, nrm, lu);
+                f[2 * p + 1] = mlp_feature(fi + 1, u, nrm, lu);
+            }
+        }
         xin[kb] = pack4(f[0], f[1], f[2], f[3]);
     }
 }
 
-/* logit of the 16 queries of the wave (valid in lanes 0..15, one per query) */
-__device__ __forceinline__ float mlp_logit(const uint16_t* __restrict__ Ws, const float* __restrict__ Bs,
-                                           const bf16x4 xin[3], int lane)
+__device__ __forceinline__ float relu(float v) { return v > 0.0f ? v : 0.0f; }
+
+/* logit of the 16 queries of the fragments (valid in lanes 0..15) */
+__device__ __forceinline__ float mlp_logit(const MlpFrags& F, const float* __restrict__ Bs, const bf16x4 xin[3],
+                                           int lane)
 {
-    const int r = lane & 15, g = lane >> 4;
+    const int g = lane >> 4;
     bf16x4    h[4], h2[4];
     for (int mt = 0; mt < 4; mt++) { /* layer 1: 64 x 48 */
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-        for (int kb = 0; kb < 3; kb++) {
-            bf16x4 a = *reinterpret_cast<const bf16x4*>(Ws + kP1 + (16 * mt + r) * 48 + 16 * kb + 4 * g);
-            acc      = mfma16(a, xin[kb], acc);
-        }
-        float z[4];
-        for (int e = 0; e < 4; e++) {
-            float v = acc[e] + Bs[kLB1 + 16 * mt + 4 * g + e];
-            z[e]    = v > 0.0f ? v : 0.0f;
-        }
-        h[mt] = pack4(z[0], z[1], z[2], z[3]);
+        for (int kb = 0; kb < 3; kb++) acc = mfma16(F.w1[mt][kb], xin[kb], acc);
+        const float4 b = *reinterpret_cast<const float4*>(Bs + kLB1 + 16 * mt + 4 * g);
+        h[mt] = pack4(relu(acc[0] + b.x), relu(acc[1] + b.y), relu(acc[2] + b.z),
+                      relu(acc[3] + b.w));
     }
-    for (int layer = 0; layer < 2; layer++) { /* layers 2 and 3: 64 x 64 */
-        const int pw = layer == 0 ? kP2 : kP3, pb = layer == 0 ? kLB2 : kLB3;
-        for (int mt = 0; mt < 4; mt++) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (int kb = 0; kb < 4; kb++) {
-                bf16x4 a = *reinterpret_cast<const bf16x4*>(Ws + pw + (16 * mt + r) * 64 + 16 * kb + 4 * g);
-                acc      = mfma16(a, h[kb], acc);
-            }
-            float z[4];
-            for (int e = 0; e < 4; e++) {
-                float v = acc[e] + Bs[pb + 16 * mt + 4 * g + e];
-                z[e]    = v > 0.0f ? v : 0.0f;
-            }
-            h2[mt] = pack4(z[0], z[1], z[2], z[3]);
-        }
-        for (int mt = 0; mt < 4; mt++) h[mt] = h2[mt];
+    for (int mt = 0; mt < 4; mt++) { /* layer 2 */
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int kb = 0; kb < 4; kb++) acc = mfma16(F.w2[mt][kb], h[kb], acc);
+        const float4 b = *reinterpret_cast<const float4*>(Bs + kLB2 + 16 * mt + 4 * g);
+        h2[mt] = pack4(relu(acc[0] + b.x), relu(acc[1] + b.y), relu(acc[2] + b.z),
+                       relu(acc[3] + b.w));
+    }
+    for (int mt = 0; mt < 4; mt++) { /* layer 3 */
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int kb = 0; kb < 4; kb++) acc = mfma16(F.w3[mt][kb], h2[kb], acc);
+        const float4 b = *reinterpret_cast<const float4*>(Bs + kLB3 + 16 * mt + 4 * g);
+        h[mt] = pack4(relu(acc[0] + b.x), relu(acc[1] + b.y), relu(acc[2] + b.z),
+                      relu(acc[3] + b.w));
     }
     f32x4 acc = {0.f, 0.f, 0.f, 0.f}; /* layer 4: row 0 of 16 x 64 */
-    for (int kb = 0; kb < 4; kb++) {
-        bf16x4 a = *reinterpret_cast<const bf16x4*>(Ws + kP4 + r * 64 + 16 * kb + 4 * g);
-        acc      = mfma16(a, h[kb], acc);
-    }
+    for (int kb = 0; kb < 4; kb++) acc = mfma16(F.w4[kb], h[kb], acc);
     return acc[0] + Bs[kLB4]; /* D[0][query]: lanes 0..15, element 0 */
 }
 
-/* probabilities of n queries. RAW: inputs from isp/nrm/light (n x 3 each);
- * else from the feature matrix X[48][B] */
+/* Logits of the 64 queries of a wave, one per lane: four fragment groups of
+ * 16, each lane's inputs handed to its group's fragment lanes by shuffles. */
+__device__ __forceinline__ float mlp_logit64(const MlpFrags& F, const float* __restrict__ Bs, const float u[3],
+                                             const float nrm[3], const float lu[3], int lane)
+{
+    const int r = lane & 15, g = lane >> 4;
+    float     res = 0.0f;
+    for (int j = 0; j < 4; j++) {
+        const int src = 16 * j + r;
+        float     uj[3], nj[3], lj[3];
+        for (int a = 0; a < 3; a++) {
+            uj[a] = __shfl(u[a], src, 64);
+            nj[a] = __shfl(nrm[a], src, 64);
+            lj[a] = __shfl(lu[a], src, 64);
+        }
+        bf16x4 xin[3];
+        query_frag(uj, nj, lj, g, xin);
+        const float lg  = mlp_logit(F, Bs, xin, lane);
+        const float got = __shfl(lg, r, 64); /* query 16 j + r's logit sits in lane r */
+        if (g == j) res = got;
+    }
+    return res;
+}
+
+/* probabilities of n queries, 64 per wave. RAW: inputs from isp/nrm/light
+ * (n x 3 each); else from the feature matrix X[48][B] */
 template <bool RAW>
 __global__ void __launch_bounds__(256) k_mlp_infer(InferArgs ia, const uint16_t* __restrict__ X, int B,
                                                    const float* __restrict__ isp, const float* __restrict__ nrm,
                                                    const float* __restrict__ light, int n, float* __restrict__ pout)
 {
-    __shared__ uint16_t Ws[kPackN];
-    __shared__ float    Bs[kLBN];
-    load_weights_lds(ia, Ws, Bs);
+    __shared__ float Bs[kLBN];
+    load_biases_lds(ia, Bs);
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
     const int r = lane & 15, g = lane >> 4;
-    for (int q0 = wave * 16; q0 < n; q0 += nw * 16) {
-        const int s = q0 + r;
-        bf16x4    xin[3];
+    MlpFrags  F;
+    load_frags(ia.W, lane, F);
+    for (int q0 = wave * 64; q0 < n; q0 += nw * 64) {
+        const int s = q0 + lane;
+        float     logit;
         if (RAW) {
             float u[3] = {0.f, 0.f, 0.f}, nn[3] = {0.f, 0.f, 0.f}, lu[3] = {0.f, 0.f, 0.f};
             if (s < n) {
@@ -395,18 +440,25 @@ __global__ void __launch_bounds__(256) k_mlp_infer(InferArgs ia, const uint16_t*
                 mlp_scale(l3, ia.bx, ia.by, ia.bz, ia.bw, lu);
                 nn[0] = nrm[3 * s], nn[1] = nrm[3 * s + 1], nn[2] = nrm[3 * s + 2];
             }
-            query_frag(u, nn, lu, g, xin);
+            logit = mlp_logit64(F, Bs, u, nn, lu, lane);
         } else {
-            for (int kb = 0; kb < 3; kb++)
-                for (int e = 0; e < 4; e++)
-                    xin[kb][e] = s < n ? (short) X[(size_t) (16 * kb + 4 * g + e) * B + s] : (short) 0;
+            logit = 0.0f;
+            for (int j = 0; j < 4; j++) {
+                const int sj = q0 + 16 * j + r;
+                bf16x4    xin[3];
+                for (int kb = 0; kb < 3; kb++)
+                    for (int e = 0; e < 4; e++)
+                        xin[kb][e] = sj < n ? (short) X[(size_t) (16 * kb + 4 * g + e) * B + sj] : (short) 0;
+                const float lg  = mlp_logit(F, Bs, xin, lane);
+                const float got = __shfl(lg, r, 64);
+                if (g == j) logit = got;
+            }
         }
-        const float logit = mlp_logit(Ws, Bs, xin, lane);
-        if (g == 0 && s < n) pout[s] = 1.0f / (1.0f + expf(-logit));
+        if (s < n) pout[s] = 1.0f / (1.0f + expf(-logit));
     }
 }
 
-/* The decision pass: for each (shaded pixel, light) query, 16 per wave,
+/* The decision pass: for each (shaded pixel, light) query, 64 per wave,
  * predicted bits outside the band, the others appended to the light's
  * exact-trace list (one atomic per light and wave). Pixels of light l:
  * lists[l * lstride + i], i < *nsel. */
@@ -420,18 +472,19 @@ __global__ void __launch_bounds__(256) k_mlp_shadow(InferArgs ia, SceneDev s, Li
                                                     unsigned* __restrict__ fb_count, float lo_logit, float hi_logit,
                                                     unsigned long long* __restrict__ counters)
 {
-    __shared__ uint16_t Ws[kPackN];
-    __shared__ float    Bs[kLBN];
-    load_weights_lds(ia, Ws, Bs);
+    __shared__ float Bs[kLBN];
+    load_biases_lds(ia, Bs);
     const uint32_t nq     = *nsel;
     const uint32_t total  = nq * (uint32_t) L.n;
     const int      lane   = threadIdx.x & 63;
     const int      wave   = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int      nw     = (gridDim.x * blockDim.x) >> 6;
-    const int      r      = lane & 15, g = lane >> 4;
     unsigned       n_pred = 0;
-    for (uint32_t q0 = (uint32_t) wave * 16u; q0 < total; q0 += (uint32_t) nw * 16u) {
-        const uint32_t qi    = q0 + (uint32_t) r;
+    if ((uint32_t) wave * 64u >= total) return; /* uniform over the wave; no barrier follows */
+    MlpFrags F;
+    load_frags(ia.W, lane, F);
+    for (uint32_t q0 = (uint32_t) wave * 64u; q0 < total; q0 += (uint32_t) nw * 64u) {
+        const uint32_t qi    = q0 + (uint32_t) lane;
         const bool     valid = qi < total;
         int            l     = 0;
         uint32_t       gp    = 0;
@@ -448,11 +501,9 @@ __global__ void __launch_bounds__(256) k_mlp_shadow(InferArgs ia, SceneDev s, Li
             nrm[0] = n4[0], nrm[1] = n4[1], nrm[2] = n4[2];
             mlp_scale(L.p[l], ia.bx, ia.by, ia.bz, ia.bw, lu);
         }
-        bf16x4 xin[3];
-        query_frag(u, nrm, lu, g, xin);
-        const float logit  = mlp_logit(Ws, Bs, xin, lane);
+        const float logit  = mlp_logit64(F, Bs, u, nrm, lu, lane);
         bool        unsure = false;
-        if (g == 0 && valid) {
+        if (valid) {
             if (logit < lo_logit) lstat[(size_t) l * npix + gp] = LS_PRED_OCC, n_pred++;
             else if (logit > hi_logit) lstat[(size_t) l * npix + gp] = LS_PRED_LIT, n_pred++;
             else unsure = true;
@@ -949,7 +1000,7 @@ int qbt_surrogate_decide(qbt_ctx* c, const SceneDev& s, bool dual, const LightsD
     /* band edges as logits: p < lo  <=>  logit < log(lo / (1 - lo)) */
     const float lo   = logf(m->band_lo / (1.0f - m->band_lo));
     const float hi   = logf(m->band_hi / (1.0f - m->band_hi));
-    const int   grid = c->num_cus * 4;
+    const int   grid = c->num_cus * 2; /* two 4-wave blocks per CU: the occupancy at ~220 VGPRs */
     auto        go   = [&](auto kern) {
         kern<<<grid, 256, 0, c->stream>>>(ia, s, L, lists, list_stride, nsel, (const float4*) c->wb_isp.p,
                                           (const int*) c->wb_ms.p, (const int*) c->wb_md.p, (uint8_t*) c->wb_lstat.p,

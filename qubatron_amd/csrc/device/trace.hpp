@@ -153,8 +153,18 @@ __device__ __forceinline__ float4 midplane(int src, const Ray& r, float x, float
     return src == 1 ? plane_z<CPU>(hz, r) : src == 2 ? plane_x<CPU>(hx, r) : plane_y<CPU>(hy, r);
 }
 
-__device__ __forceinline__ int pick4(int4 v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
-__device__ __forceinline__ int pick8(int4 lo, int4 hi, int o) { return pick4((o & 4) ? hi : lo, o & 3); }
+/* component selects by value (a conditional over lvalues selects an address
+ * and forces the operands into scratch memory) */
+__device__ __forceinline__ int sel(bool c, int a, int b) { return c ? a : b; }
+__device__ __forceinline__ int pick4(int x, int y, int z, int w, int k)
+{
+    return sel(k == 0, x, sel(k == 1, y, sel(k == 2, z, w)));
+}
+__device__ __forceinline__ int pick8(int4 lo, int4 hi, int o)
+{
+    const bool h = (o & 4) != 0;
+    return pick4(sel(h, hi.x, lo.x), sel(h, hi.y, lo.y), sel(h, hi.z, lo.z), sel(h, hi.w, lo.w), o & 3);
+}
 __device__ __forceinline__ unsigned exist_mask(int4 lo, int4 hi)
 {
     return (lo.x > 0 ? 1u : 0u) | (lo.y > 0 ? 2u : 0u) | (lo.z > 0 ? 4u : 0u) | (lo.w > 0 ? 8u : 0u) |
@@ -333,6 +343,17 @@ __device__ __forceinline__ float4 entry_of_c1(const SceneDev& s, const Ray& r, c
     return start;
 }
 
+/* candidate src (0 entry, 1 z, 2 x, 3 y) by value: a conditional over the
+ * lvalues would select an address and keep the candidates in scratch memory */
+__device__ __forceinline__ float4 pick_entry(int src, float4 e, float4 z, float4 x, float4 y)
+{
+    float4 r = e;
+    if (src == 1) r = z;
+    if (src == 2) r = x;
+    if (src == 3) r = y;
+    return r;
+}
+
 __device__ __forceinline__ void kswap(float& wa, int& ka, float& wb, int& kb)
 {
     const bool sw = wb < wa; /* strict: ties keep order (octree_fsh.c:283); NaN never moves */
@@ -411,7 +432,7 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
         if (n > 0) {
             st.pw[level * st.stride] = word;
             st.ns[level * st.stride] = rec.x | ((uint32_t) (n - 1) << 30);
-            t.entry = src0 == 0 ? t.entry : src0 == 1 ? az : src0 == 2 ? ax : ay;
+            t.entry = pick_entry(src0, t.entry, az, ax, ay);
             t.ns    = (int) (rec.x + rank0);
             t.ix    = (t.ix << 1) | (oct0 & 1);
             t.iy    = (t.iy << 1) | ((oct0 >> 1) & 1);

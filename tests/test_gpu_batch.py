@@ -26,7 +26,7 @@ def scene_ctx(gpu_ctx):
     gpu_ctx.upload_model(col, nrm)
     gpu_ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
     yield gpu_ctx, sc, po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth)
-    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=1, graph=1)
+    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=0, graph=1)
 
 
 def _views(sc, seed, sizes=SIZES, maxlevel=None):
@@ -96,7 +96,7 @@ def test_mixed_maxlevel_views_split_into_batches(scene_ctx):
     ctx.render(views, LIGHTS4[:2], MASK)
     for i in range(4):
         osc = H.oracle_scene(1, npoints=300000, maxlevel=sc.depth - (i % 2))
-        _same(ctx.read_frame(i), osc.render(pos[i], ang[i], 48, 32, LIGHTS4[:2], shoot=i % 2), f"view {i}")
+        _same(ctx.read_frame(i), osc.render(pos[i], ang[i], 48, 32, LIGHTS4[:2]), f"view {i}")
 
 
 def test_fused_path_batch(scene_ctx):

@@ -28,7 +28,7 @@ def ctx(gpu_ctx, request):
     wavefront stages with global-memory stacks and unordered shadow rays."""
     gpu_ctx.set_tuning(**request.param)
     yield gpu_ctx
-    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=1, graph=1)
+    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=0, graph=1)
 
 
 def test_shared_child_falls_back_to_direct(ctx):
