@@ -9,7 +9,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBDIR = os.path.join(HERE, "lib")
+# QBT_LIBDIR selects another build of the two libraries (A/B measurements)
+LIBDIR = os.environ.get("QBT_LIBDIR") or os.path.join(HERE, "lib")
 HOST_LIB = os.path.join(LIBDIR, "libqbt_host.so")
 DEV_LIB = os.path.join(LIBDIR, "libqbt.so")
 

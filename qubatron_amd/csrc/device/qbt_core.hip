@@ -423,6 +423,7 @@ extern "C" void qbt_destroy(qbt_ctx* c)
     c->graph_free();
     for (DevBuf* b : {&c->wb_isp, &c->wb_ms, &c->wb_md, &c->wb_leaf, &c->wb_flags, &c->wb_disc, &c->wb_lstat,
                       &c->wb_exp, &c->wb_cand, &c->wb_items, &c->wb_q, &c->wb_lists, &c->wb_fb, &c->wb_gstack,
+                      &c->wb_bcnt,
                       &c->bt_tab, &c->valflag})
         c->dev_free(*b);
     for (int k = 0; k < qbt_ctx::kTabSlots; k++) {
@@ -826,11 +827,11 @@ int qbt_ctx::launch_render(const SceneDev& s, bool dual, bool grid, bool count, 
     return QBT_OK;
 }
 
-int qbt_ctx::launch_sprite(const SceneDev& s, bool cmp, bool dual, bool grid, const BatchDev& b)
+int qbt_ctx::launch_sprite(const SceneDev& s, bool cmp, bool dual, bool grid, const BatchDev& b, hipStream_t st)
 {
     const size_t lds = qbt_stack_bytes(s.maxlevel, dual, 64);
     qbt_dispatch3(cmp, dual, grid, [&](auto C_, auto D_, auto G_) {
-        k_sprite<decltype(C_)::value, decltype(D_)::value, decltype(G_)::value><<<b.n, 64, lds, stream>>>(
+        k_sprite<decltype(C_)::value, decltype(D_)::value, decltype(G_)::value><<<b.n, 64, lds, st>>>(
             s, b, (int*) sprite.p);
         return 0;
     });
@@ -926,7 +927,7 @@ extern "C" int qbt_render(qbt_ctx* c, const qbt_view* views, int nviews, const f
             bh.pix_off.push_back((uint32_t) pix);
             items += (uint64_t) ((v.w + 7) >> 3) * (uint64_t) ((v.h + 7) >> 3) * 64u;
             pix += (uint64_t) v.w * (uint64_t) v.h;
-            if (items > 0xFFFFFFF0ull) return QBT_ERANGE; /* batch indices are 32-bit */
+            if (items > 0xFFFF0000ull) return QBT_ERANGE; /* batch indices are 32-bit */
             bh.views.push_back(v);
             bh.frames.push_back(c->frame_dev(k));
         }
@@ -936,8 +937,8 @@ extern "C" int qbt_render(qbt_ctx* c, const qbt_view* views, int nviews, const f
         bh.npix   = (uint32_t) pix;
         BatchDev b;
         if ((rc = upload_batch(c, bh, b))) return rc;
-        if (nlights > 0 && (rc = c->launch_sprite(s, cmp, dual, grid, b))) return rc;
         if (path == 0) {
+            if (nlights > 0 && (rc = c->launch_sprite(s, cmp, dual, grid, b, c->stream))) return rc;
             for (int k = i; k < j; k++) {
                 const int* spr = (const int*) c->sprite.p + (k - i) * QBT_MAX_LIGHTS_RENDER;
                 if ((rc = c->launch_render(s, dual, grid, count, bh.views[k - i], spr, c->frame_dev(k), mask)))

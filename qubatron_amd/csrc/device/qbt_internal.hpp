@@ -25,7 +25,8 @@ enum : uint8_t {
     LS_DISCARD  = 2, /* exact trace rejected at the root: fragment discarded */
     LS_SKIPPED  = 3, /* an earlier light discarded the fragment */
     LS_PRED_OCC = 4, /* surrogate: occluded */
-    LS_PRED_LIT = 5  /* surrogate: visible */
+    LS_PRED_LIT = 5, /* surrogate: visible */
+    LS_UNSURE   = 6  /* surrogate: inside the band, to be traced exactly */
 };
 
 struct DevBuf {
@@ -159,6 +160,7 @@ struct qbt_ctx {
     DevBuf       wb_lists; /* per-light shadow lists in direction-octant order */
     DevBuf       wb_fb;    /* surrogate: per-light lists of the rays traced exactly */
     DevBuf       wb_gstack; /* traversal stacks in global memory (tuning stack = 1) */
+    DevBuf       wb_bcnt;   /* per-block counts of the stream compactions */
     DevBuf       bt_tab; /* batch tables: views, item and pixel offsets, frames */
     static constexpr int kTabSlots = 4;
     void*        tab_host[kTabSlots] = {};      /* pinned staging of the batch tables */
@@ -215,7 +217,8 @@ struct qbt_ctx {
     int      block_threads(int L, bool dual) const;
     int      launch_render(const qbt::SceneDev& s, bool dual, bool grid, bool count, const ViewDev& v, const int* spr,
                            FrameDev f, unsigned mask);
-    int      launch_sprite(const qbt::SceneDev& s, bool cmp, bool dual, bool grid, const BatchDev& b);
+    int      launch_sprite(const qbt::SceneDev& s, bool cmp, bool dual, bool grid, const BatchDev& b,
+                           hipStream_t st);
     int      ensure_wave_bufs(size_t npix, int nlights);
     /* flags: QBT_RENDER_SURROGATE, QBT_RENDER_COLLECT */
     int      render_batch_wavefront(const qbt::SceneDev& s, bool cmp, bool dual, bool grid, bool count,
@@ -229,17 +232,23 @@ struct qbt_ctx {
 /* surrogate.hip: the MLP decision pass of a batch. The shaded pixels of
  * light l are lists[l * list_stride + i], i < *nsel (batch pixel indices).
  * For every (pixel, light) query it writes LS_PRED_* where the prediction is
- * outside the band and appends the pixel to that light's exact-trace list
- * (fb + l * fb_stride, count fb_count[l]) otherwise. */
+ * outside the band and LS_UNSURE otherwise (to be traced exactly). */
 int qbt_surrogate_decide(qbt_ctx* c, const qbt::SceneDev& s, bool dual, const LightsDev& L, const uint32_t* lists,
-                         size_t list_stride, const unsigned* nsel, size_t npix, uint32_t* fb, size_t fb_stride,
-                         unsigned* fb_count);
+                         size_t list_stride, const unsigned* nsel, size_t npix);
 /* surrogate.hip: append training samples (every stride-th shaded pixel x
  * light of the batch, exact label) to the sample store */
 int qbt_collect_batch(qbt_ctx* c, const qbt::SceneDev& s, bool dual, const LightsDev& L, const uint32_t* lists,
                       size_t list_stride, const unsigned* nsel, size_t npix);
 bool qbt_surrogate_ready(const qbt_ctx* c);
 void qbt_surrogate_band(const qbt_ctx* c, float* lo, float* hi);
+
+/* Stream compaction without same-address atomics (render.hip): the indices
+ * i < n for which pred(i, &value) holds, their values written to out in index
+ * order, the count to *count. Blocks of kSelItems items: per-block counts, one
+ * scan block, scatter. bcnt: (n / kSelItems + 1) * rows words of scratch;
+ * rows = gridDim.y of the launches (one independent compaction per row, the
+ * predicate gets the row). */
+constexpr int kSelItems = 1024;
 
 void   qbt_set_hip_error(hipError_t e, const char* what, const char* file, int line);
 bool   qbt_grid_exact(const float b[4], int L);

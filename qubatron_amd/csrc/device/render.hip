@@ -124,22 +124,37 @@ __device__ __forceinline__ Stack lane_stack(uint32_t* lds, uint32_t* gstack, int
     return GSTK ? make_stack_global(gstack, L, DUAL) : make_stack(lds, L, DUAL);
 }
 
+/* Camera rays of the batch. Work items 0 .. nspr-1 are the sprite traces
+ * (view i / nl, light i % nl: the ray from the camera to the light,
+ * octree_fsh.c:452-459, one per view and light; result flags to
+ * b.sprite), then nitems pixel items. */
 template <bool CMP, bool DUAL, bool GRID, bool COUNT, bool GSTK>
-__global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut o, uint32_t nitems,
-                                                 unsigned* __restrict__ q, unsigned refill, uint32_t* gstack,
+__global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut o, uint32_t nitems, uint32_t nspr,
+                                                 int nl, int* __restrict__ sprite, unsigned* __restrict__ q,
+                                                 unsigned refill, uint32_t* gstack,
                                                  unsigned long long* __restrict__ counters)
 {
     extern __shared__ uint32_t lds[];
     const Stack st   = lane_stack<DUAL, GSTK>(lds, gstack, s.maxlevel);
     const int   lane = threadIdx.x & 63;
-    Refill      rf(nitems);
+    Refill      rf(nspr + nitems);
     bool        active = false;
     uint32_t    item = 0, gp = 0;
+    int         spr  = -1; /* sprite slot of a sprite lane */
     Ray         r = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     TraceState  t;
     unsigned    n_exp = 0, n_abort = 0, n_hit = 0;
 
     auto finish = [&](int stt) {
+        if (spr >= 0) { /* white if the light is in front of the first surface (resv / lghtv.x > 1) */
+            const float ix = stt == ST_HIT ? t.entry.x : 0.0f;
+            int         f  = 0;
+            if (stt == ST_DISCARD) f |= 1;
+            if ((ix - r.px) / r.dx > 1.0f) f |= 2;
+            if (stt == ST_ABORT) f |= 4;
+            sprite[spr] = f;
+            return;
+        }
         float4   isp = make_float4(0.f, 0.f, 0.f, 0.f);
         int      ms = -1, md = -1, leaf = -1;
         uint8_t  f    = 0;
@@ -175,12 +190,27 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut
     for (;;) {
         const uint64_t idle = __ballot(!active);
         if (rf.left > 0 && (__popcll(idle) >= (int) refill || idle == ~0ull)) {
-            const uint32_t it = rf.fetch(&q[Q_PRIM], idle, lane);
-            if (it != UINT32_MAX) {
-                const int      v = batch_view(b.item_off, b.n, it);
+            const uint32_t fetched = rf.fetch(&q[Q_PRIM], idle, lane);
+            if (fetched != UINT32_MAX && fetched < nspr) {
+                const int      v = (int) (fetched / (uint32_t) nl), l = (int) (fetched % (uint32_t) nl);
                 const ViewDev& V = b.views[v];
+                spr              = v * kNL + l;
+                r.px             = V.cam[0];
+                r.py             = V.cam[1];
+                r.pz             = V.cam[2];
+                r.dx             = V.light[l][0] - V.cam[0];
+                r.dy             = V.light[l][1] - V.cam[1];
+                r.dz             = V.light[l][2] - V.cam[2];
+                int stt          = trace_begin<false>(s, r, t);
+                if (stt == ST_CONT) active = true;
+                else finish(stt);
+            } else if (fetched != UINT32_MAX) {
+                const uint32_t it = fetched - nspr;
+                const int      v  = batch_view(b.item_off, b.n, it);
+                const ViewDev& V  = b.views[v];
                 int            px, py;
                 item = it;
+                spr  = -1;
                 if (item_pixel(it - b.item_off[v], (V.w + 7) >> 3, V.w, V.h, px, py)) {
                     gp = b.pix_off[v] + (uint32_t) py * (uint32_t) V.w + (uint32_t) px;
                     float csv[3];
@@ -222,21 +252,120 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut
     }
 }
 
-/* shaded pixels -> list; one atomic per wave, the wave's lanes in order */
-__global__ void k_compact(const uint32_t* __restrict__ cand, uint32_t n, uint32_t* __restrict__ out,
-                          unsigned* __restrict__ count)
+/* ---- stream compaction: per-block counts, a scan of the counts, scatter ---- */
+
+/* the shaded-pixel list: work items whose candidate is a pixel index */
+struct PredCand {
+    const uint32_t* cand;
+    uint32_t        n;
+    __device__ __forceinline__ bool operator()(uint32_t i, int, uint32_t& v) const
+    {
+        if (i >= n) return false;
+        v = cand[i];
+        return v != UINT32_MAX;
+    }
+};
+
+/* the surrogate's undecided queries of light `row` */
+struct PredUnsure {
+    const uint32_t* lists;
+    size_t          lstride;
+    const unsigned* nsel;
+    const uint8_t*  lstat;
+    size_t          npix;
+    __device__ __forceinline__ bool operator()(uint32_t i, int row, uint32_t& v) const
+    {
+        if (i >= *nsel) return false;
+        v = lists[(size_t) row * lstride + i];
+        return lstat[(size_t) row * npix + v] == LS_UNSURE;
+    }
+};
+
+template <class P>
+__global__ void __launch_bounds__(256) k_sel_count(P p, uint32_t* __restrict__ bcnt, uint32_t nb)
 {
-    const uint32_t i    = blockIdx.x * blockDim.x + threadIdx.x;
-    const int      lane = threadIdx.x & 63;
-    const uint32_t v    = i < n ? cand[i] : UINT32_MAX;
-    const bool     keep = v != UINT32_MAX;
-    const uint64_t m    = __ballot(keep);
-    if (m == 0ull) return; /* uniform over the wave */
-    const int leader = __ffsll((unsigned long long) m) - 1;
-    uint32_t  base   = 0;
-    if (lane == leader) base = atomicAdd(count, (unsigned) __popcll(m));
-    base = __shfl(base, leader, 64);
-    if (keep) out[base + (uint32_t) __popcll(m & ((1ull << lane) - 1ull))] = v;
+    __shared__ uint32_t ws[4];
+    const int row = blockIdx.y;
+    uint32_t  c   = 0;
+    for (int k = 0; k < kSelItems / 256; k++) {
+        uint32_t v;
+        c += p(blockIdx.x * kSelItems + k * 256 + threadIdx.x, row, v) ? 1u : 0u;
+    }
+    c = wave_sum(c);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) bcnt[(size_t) row * nb + blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+/* in place: exclusive scan of the nb counts of row blockIdx.x; the row's total
+ * to total[row * tstride] */
+__global__ void __launch_bounds__(1024) k_sel_scan(uint32_t* __restrict__ bcnt, uint32_t nb,
+                                                   unsigned* __restrict__ total, int tstride)
+{
+    __shared__ uint32_t ws[16];
+    __shared__ uint32_t carry;
+    uint32_t*           b    = bcnt + (size_t) blockIdx.x * nb;
+    const int           lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < nb; base += 1024) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nb ? b[i] : 0u;
+        uint32_t       x = v; /* inclusive scan within the wave */
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane == 63) ws[w] = x;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t a = carry;
+            for (int k = 0; k < 16; k++) {
+                const uint32_t t = ws[k];
+                ws[k]            = a;
+                a += t;
+            }
+            carry = a;
+        }
+        __syncthreads();
+        if (i < nb) b[i] = ws[w] + x - v;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) total[(size_t) blockIdx.x * tstride] = carry;
+}
+
+/* values of the kept indices, in index order, at out + row * ostride */
+template <class P>
+__global__ void __launch_bounds__(256) k_sel_scatter(P p, const uint32_t* __restrict__ boff, uint32_t nb,
+                                                     uint32_t* __restrict__ out, size_t ostride)
+{
+    __shared__ uint32_t ws[4];
+    const int           row  = blockIdx.y;
+    const int           lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint32_t            base = boff[(size_t) row * nb + blockIdx.x];
+    for (int k = 0; k < kSelItems / 256; k++) {
+        uint32_t       v    = 0;
+        const bool     keep = p(blockIdx.x * kSelItems + k * 256 + threadIdx.x, row, v);
+        const uint64_t m    = __ballot(keep);
+        if (lane == 0) ws[w] = (uint32_t) __popcll(m);
+        __syncthreads();
+        uint32_t pre = 0;
+        for (int j = 0; j < w; j++) pre += ws[j];
+        const uint32_t tot = ws[0] + ws[1] + ws[2] + ws[3];
+        if (keep) out[(size_t) row * ostride + base + pre + (uint32_t) __popcll(m & ((1ull << lane) - 1ull))] = v;
+        base += tot;
+        __syncthreads();
+    }
+}
+
+template <class P>
+void compact(hipStream_t st, P p, uint32_t n, int rows, uint32_t* bcnt, unsigned* total, int tstride, uint32_t* out,
+             size_t ostride)
+{
+    const uint32_t nb = (n + kSelItems - 1) / kSelItems + 1;
+    k_sel_count<P><<<dim3(nb, rows), 256, 0, st>>>(p, bcnt, nb);
+    k_sel_scan<<<rows, 1024, 0, st>>>(bcnt, nb, total, tstride);
+    k_sel_scatter<P><<<dim3(nb, rows), 256, 0, st>>>(p, bcnt, nb, out, ostride);
 }
 
 /* direction octant of the shadow ray light -> isp */
@@ -491,7 +620,7 @@ int persistent_grid(qbt_ctx* c, K kernel, int bs, size_t lds)
 /* the launch configuration a captured graph depends on (kernel arguments) */
 struct GraphKey {
     SceneDev s;
-    void*    bufs[16];
+    void*    bufs[17];
     uint32_t nitems, npix, nviews, nlights;
     unsigned mask, flags;
     int      bs, refill, gp, gs, stack, reorder;
@@ -533,12 +662,15 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     const bool     surrogate = (flags & QBT_RENDER_SURROGATE) && nl > 0;
     const bool     collect   = (flags & QBT_RENDER_COLLECT) && nl > 0 && !surrogate;
     const bool     reorder   = c->tune_reorder && nl > 0;
+    const uint32_t nspr      = (uint32_t) b.n * (uint32_t) (nl > 0 ? nl : 0); /* sprite traces */
     int            rc;
     /* every buffer exists before a capture starts */
     if ((rc = c->ensure_wave_bufs(npix, nl)) || (rc = c->dev_alloc(c->wb_cand, (size_t) nitems * 4)) ||
         (rc = c->dev_alloc(c->wb_items, (size_t) npix * 4)) || (rc = c->dev_alloc(c->wb_q, sizeof(unsigned) * Q_N)))
         return rc;
     if (reorder && (rc = c->dev_alloc(c->wb_lists, (size_t) npix * 4 * nl))) return rc;
+    const uint32_t nb_max = ((nitems > npix ? nitems : npix) + kSelItems - 1) / kSelItems + 1;
+    if ((rc = c->dev_alloc(c->wb_bcnt, (size_t) nb_max * 4 * (nl > 0 ? nl : 1)))) return rc;
     if (surrogate && (rc = c->dev_alloc(c->wb_fb, (size_t) npix * 4 * nl))) return rc;
 
     const int      bs     = c->block_threads(s.maxlevel, DUAL);
@@ -570,22 +702,24 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     /* the launches of one batch; traversal kernels bracketed for qbt_profile_*
      * (no-ops unless profiling, which is never on while capturing) */
     auto enqueue = [&]() -> int {
+        int err;
         k_zero_u32<<<1, 256, 0, st>>>(q, Q_N);
         k_zero_u8<<<nblk(npix), 256, 0, st>>>((uint8_t*) c->wb_disc.p, npix);
         c->prof_mark_begin(QBT_PROF_PRIMARY);
-        kp<<<gp, bs, lds, st>>>(s, b, po, nitems, q, refill, gstk, cnt);
+        kp<<<gp, bs, lds, st>>>(s, b, po, nitems, nspr, nl > 0 ? nl : 1, (int*) c->sprite.p, q, refill, gstk, cnt);
         c->prof_mark_end();
-        k_compact<<<nblk(nitems), 256, 0, st>>>(po.cand, nitems, items, q + Q_NSEL);
+        compact(st, PredCand{po.cand, nitems}, nitems, 1, (uint32_t*) c->wb_bcnt.p, q + Q_NSEL, 1, items, 0);
         if (reorder) {
             k_oct_bin<false><<<nblk(npix), 256, 0, st>>>(items, q + Q_NSEL, po.isp, L, q + Q_OCT, nullptr, 0);
             k_oct_scan<<<1, 64, 0, st>>>(q + Q_OCT, q + Q_CUR, nl);
             k_oct_bin<true><<<nblk(npix), 256, 0, st>>>(items, q + Q_NSEL, po.isp, L, q + Q_CUR,
                                                         (uint32_t*) c->wb_lists.p, npix);
         }
-        int err;
-        if (surrogate && (err = qbt_surrogate_decide(c, s, DUAL, L, lists, lstride, q + Q_NSEL, npix,
-                                                     (uint32_t*) c->wb_fb.p, npix, q + Q_FB)))
-            return err;
+        if (surrogate) {
+            if ((err = qbt_surrogate_decide(c, s, DUAL, L, lists, lstride, q + Q_NSEL, npix))) return err;
+            compact(st, PredUnsure{lists, lstride, q + Q_NSEL, (const uint8_t*) c->wb_lstat.p, npix}, npix, nl,
+                    (uint32_t*) c->wb_bcnt.p, q + Q_FB, 1, (uint32_t*) c->wb_fb.p, npix);
+        }
         for (int l = 0; l < nl; l++) {
             /* exact mode: every shaded pixel; surrogate mode: this light's undecided ones */
             const uint32_t* li = surrogate ? (const uint32_t*) c->wb_fb.p + (size_t) l * npix : lists + l * lstride;
@@ -605,7 +739,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     if (!use_graph) {
         if ((rc = enqueue())) return rc;
         HIPCHK(hipGetLastError());
-        c->last.launches += 5 + nl + (reorder ? 3 : 0) + (surrogate ? 1 : 0);
+        c->last.launches += 6 + nl + (reorder ? 3 : 0) + (surrogate ? 4 : 0);
         return QBT_OK;
     }
 
@@ -615,7 +749,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     key.s      = s;
     void* bp[] = {c->wb_isp.p,   c->wb_ms.p,    c->wb_md.p,   c->wb_leaf.p,  c->wb_flags.p, c->wb_disc.p,
                   c->wb_lstat.p, c->wb_exp.p,   c->wb_cand.p, c->wb_items.p, c->wb_q.p,     c->wb_lists.p,
-                  c->wb_fb.p,    c->wb_gstack.p, c->bt_tab.p, c->sprite.p};
+                  c->wb_fb.p,    c->wb_gstack.p, c->bt_tab.p, c->sprite.p, c->wb_bcnt.p};
     static_assert(sizeof(bp) == sizeof(key.bufs), "graph key buffers");
     memcpy(key.bufs, bp, sizeof(bp));
     key.nitems = nitems, key.npix = npix, key.nviews = (uint32_t) b.n, key.nlights = (uint32_t) nl;

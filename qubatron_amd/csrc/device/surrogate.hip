@@ -459,17 +459,16 @@ __global__ void __launch_bounds__(256) k_mlp_infer(InferArgs ia, const uint16_t*
 }
 
 /* The decision pass: for each (shaded pixel, light) query, 64 per wave,
- * predicted bits outside the band, the others appended to the light's
- * exact-trace list (one atomic per light and wave). Pixels of light l:
+ * the predicted bit outside the band, LS_UNSURE inside it (those are then
+ * compacted into the light's exact-trace list). Pixels of light l:
  * lists[l * lstride + i], i < *nsel. */
 template <bool DUAL>
 __global__ void __launch_bounds__(256) k_mlp_shadow(InferArgs ia, SceneDev s, LightsDev L,
                                                     const uint32_t* __restrict__ lists, size_t lstride,
                                                     const unsigned* __restrict__ nsel, const float4* __restrict__ isp,
                                                     const int* __restrict__ ms, const int* __restrict__ md,
-                                                    uint8_t* __restrict__ lstat, size_t npix,
-                                                    uint32_t* __restrict__ fb, size_t fb_stride,
-                                                    unsigned* __restrict__ fb_count, float lo_logit, float hi_logit,
+                                                    uint8_t* __restrict__ lstat, size_t npix, float lo_logit,
+                                                    float hi_logit,
                                                     unsigned long long* __restrict__ counters)
 {
     __shared__ float Bs[kLBN];
@@ -501,26 +500,11 @@ __global__ void __launch_bounds__(256) k_mlp_shadow(InferArgs ia, SceneDev s, Li
             nrm[0] = n4[0], nrm[1] = n4[1], nrm[2] = n4[2];
             mlp_scale(L.p[l], ia.bx, ia.by, ia.bz, ia.bw, lu);
         }
-        const float logit  = mlp_logit64(F, Bs, u, nrm, lu, lane);
-        bool        unsure = false;
+        const float logit = mlp_logit64(F, Bs, u, nrm, lu, lane);
         if (valid) {
-            if (logit < lo_logit) lstat[(size_t) l * npix + gp] = LS_PRED_OCC, n_pred++;
-            else if (logit > hi_logit) lstat[(size_t) l * npix + gp] = LS_PRED_LIT, n_pred++;
-            else unsure = true;
-        }
-        uint64_t pending = __ballot(unsure);
-        while (pending) {
-            const int      leader = __ffsll((unsigned long long) pending) - 1;
-            const int      Lq     = __shfl(l, leader, 64);
-            const uint64_t mine   = __ballot(unsure && l == Lq);
-            uint32_t       base   = 0;
-            if (lane == leader) base = atomicAdd(&fb_count[Lq], (unsigned) __popcll(mine));
-            base = __shfl(base, leader, 64);
-            if (unsure && l == Lq) {
-                const uint32_t rank = (uint32_t) __popcll(mine & ((1ull << lane) - 1ull));
-                fb[(size_t) Lq * fb_stride + base + rank] = gp;
-            }
-            pending &= ~mine;
+            const uint8_t d = logit < lo_logit ? LS_PRED_OCC : logit > hi_logit ? LS_PRED_LIT : LS_UNSURE;
+            lstat[(size_t) l * npix + gp] = d;
+            n_pred += d != LS_UNSURE ? 1u : 0u;
         }
     }
     n_pred = wave_sum(n_pred);
@@ -991,8 +975,7 @@ void qbt_surrogate_band(const qbt_ctx* c, float* lo, float* hi)
 }
 
 int qbt_surrogate_decide(qbt_ctx* c, const SceneDev& s, bool dual, const LightsDev& L, const uint32_t* lists,
-                         size_t list_stride, const unsigned* nsel, size_t npix, uint32_t* fb, size_t fb_stride,
-                         unsigned* fb_count)
+                         size_t list_stride, const unsigned* nsel, size_t npix)
 {
     MlpState* m = c->mlp;
     if (!m || !m->ready) return QBT_EINVAL;
@@ -1004,7 +987,7 @@ int qbt_surrogate_decide(qbt_ctx* c, const SceneDev& s, bool dual, const LightsD
     auto        go   = [&](auto kern) {
         kern<<<grid, 256, 0, c->stream>>>(ia, s, L, lists, list_stride, nsel, (const float4*) c->wb_isp.p,
                                           (const int*) c->wb_ms.p, (const int*) c->wb_md.p, (uint8_t*) c->wb_lstat.p,
-                                          npix, fb, fb_stride, fb_count, lo, hi, (unsigned long long*) c->counters.p);
+                                          npix, lo, hi, (unsigned long long*) c->counters.p);
     };
     c->prof_mark_begin(QBT_PROF_SURROGATE);
     if (dual) go(k_mlp_shadow<true>);
