@@ -88,6 +88,7 @@ def parse():
     ap.add_argument("--stack", type=int, default=None, help="0 LDS traversal stacks, 1 global memory")
     ap.add_argument("--reorder", type=int, default=None, help="1 shadow rays in direction-octant order")
     ap.add_argument("--graph", type=int, default=None, help="1 replay captured hipGraphs")
+    ap.add_argument("--strip", type=int, default=None, help="tile order: 0 row-major, S strips of S tiles")
     return ap.parse_args()
 
 
@@ -284,7 +285,7 @@ def main():
 
     ctx = q.Context(0)
     ctx.set_tuning(path=args.path, block_threads=args.block, refill=args.refill, layout=args.layout,
-                   stack=args.stack, reorder=args.reorder, graph=args.graph)
+                   stack=args.stack, reorder=args.reorder, graph=args.graph, strip=args.strip)
     tuning = ctx.tuning()
     copy_ms, copy_gbps = ctx.copy_bandwidth(COPY_BYTES, 5)
     log(f"copy kernel: {copy_gbps:.0f} GB/s (read+write) on {COPY_BYTES >> 20} MiB")

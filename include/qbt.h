@@ -206,7 +206,9 @@ int qbt_profile_end(qbt_ctx* ctx, qbt_profile* out);
  * blocks. stack: 0 = per-lane traversal stacks in LDS, 1 = in global memory.
  * reorder: 1 = shadow rays in direction-octant order per light. graph: 1 =
  * capture the batch's launches into a hipGraph and replay it while the
- * launch configuration is unchanged. Results never depend on these. */
+ * launch configuration is unchanged. strip: order of the 8x8 pixel tiles,
+ * 0 = row-major, S (1..1024) = vertical strips S tiles wide, each row-major.
+ * Results never depend on these. */
 typedef struct qbt_tuning {
     int path;
     int block_threads;
@@ -215,7 +217,7 @@ typedef struct qbt_tuning {
     int stack;
     int reorder;
     int graph;
-    int reserved;
+    int strip;
 } qbt_tuning;
 
 int qbt_get_tuning(qbt_ctx* ctx, qbt_tuning* out);

@@ -324,7 +324,7 @@ class Context:
         out["kind_ms"] = list(p.kind_ms)
         return out
 
-    TUNING_FIELDS = ("path", "block_threads", "refill", "layout", "stack", "reorder", "graph")
+    TUNING_FIELDS = ("path", "block_threads", "refill", "layout", "stack", "reorder", "graph", "strip")
 
     def tuning(self):
         t = _ffi.Tuning()

@@ -79,7 +79,7 @@ class Profile(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("path", C.c_int), ("block_threads", C.c_int), ("refill", C.c_int), ("layout", C.c_int),
-                ("stack", C.c_int), ("reorder", C.c_int), ("graph", C.c_int), ("reserved", C.c_int)]
+                ("stack", C.c_int), ("reorder", C.c_int), ("graph", C.c_int), ("strip", C.c_int)]
 
 
 class TuneResult(C.Structure):

@@ -83,14 +83,34 @@ __device__ __forceinline__ void pixel_dir(const ViewDev& v, int px, int py, floa
     quat_rotate(v.qx, c1, csv);
 }
 
-/* Work item -> pixel: items run over 8x8 tiles in row-major tile order,
- * item = 64 * tile + 8 * (py % 8) + px % 8. False for the padding items of
- * edge tiles. */
-__device__ __forceinline__ bool item_pixel(uint32_t item, int tiles_x, int w, int h, int& px, int& py)
+/* Work item -> pixel: item = 64 * tile + 8 * (py % 8) + px % 8 over 8x8
+ * tiles. strip = 0: tiles in row-major order; strip = S > 0: the tile grid
+ * cut into vertical strips S tiles wide (the last one narrower), strips left
+ * to right, each in row-major order, so that the consecutive tiles that are
+ * in flight together cover a compact screen region. False for the padding
+ * items of edge tiles. */
+__device__ __forceinline__ bool item_pixel(uint32_t item, int w, int h, int strip, int& px, int& py)
 {
-    uint32_t tile = item >> 6, sub = item & 63u;
-    px            = (int) (tile % (uint32_t) tiles_x) * 8 + (int) (sub & 7u);
-    py            = (int) (tile / (uint32_t) tiles_x) * 8 + (int) (sub >> 3);
+    const uint32_t tx = (uint32_t) ((w + 7) >> 3), ty = (uint32_t) ((h + 7) >> 3);
+    const uint32_t tile = item >> 6, sub = item & 63u;
+    uint32_t       cx, cy;
+    if (strip <= 0) {
+        cx = tile % tx;
+        cy = tile / tx;
+    } else {
+        const uint32_t S = (uint32_t) strip, full = tx / S, inf = full * S * ty;
+        if (tile < inf) {
+            const uint32_t k = tile / (S * ty), u = tile - k * S * ty;
+            cx               = k * S + u % S;
+            cy               = u / S;
+        } else {
+            const uint32_t W = tx - full * S, u = tile - inf;
+            cx               = full * S + u % W;
+            cy               = u / W;
+        }
+    }
+    px = (int) cx * 8 + (int) (sub & 7u);
+    py = (int) cy * 8 + (int) (sub >> 3);
     return px < w && py < h;
 }
 

@@ -1069,6 +1069,7 @@ extern "C" int qbt_get_tuning(qbt_ctx* c, qbt_tuning* out)
     out->stack         = c->tune_stack;
     out->reorder       = c->tune_reorder;
     out->graph         = c->tune_graph;
+    out->strip         = c->tune_strip;
     return QBT_OK;
 }
 
@@ -1080,8 +1081,10 @@ extern "C" int qbt_set_tuning(qbt_ctx* c, const qbt_tuning* in)
         return QBT_EINVAL;
     if (in->refill < 0 || in->refill > 64) return QBT_EINVAL;
     if (in->layout < 0 || in->layout > 1) return QBT_EINVAL;
-    if (in->stack < 0 || in->stack > 1 || in->reorder < 0 || in->reorder > 1 || in->graph < 0 || in->graph > 1)
+    if (in->stack < 0 || in->stack > 1 || in->reorder < 0 || in->reorder > 1 || in->graph < 0 || in->graph > 1 ||
+        in->strip < 0 || in->strip > 1024)
         return QBT_EINVAL;
+    c->tune_strip   = in->strip;
     c->path         = in->path;
     c->tune_stack   = in->stack;
     c->tune_reorder = in->reorder;

@@ -169,6 +169,7 @@ struct qbt_ctx {
     int          tune_stack   = 0; /* 0 = LDS, 1 = global memory */
     int          tune_reorder = 0; /* shadow rays in direction-octant order (measured: no gain) */
     int          tune_graph   = 1; /* replay batched renders from a captured hipGraph */
+    int          tune_strip   = 0; /* tile order: 0 row-major, S > 0 vertical strips of S tiles */
     /* captured render graph and the launch configuration it was captured for */
     hipGraph_t           graph      = nullptr;
     hipGraphExec_t       graph_exec = nullptr;

@@ -130,7 +130,7 @@ __device__ __forceinline__ Stack lane_stack(uint32_t* lds, uint32_t* gstack, int
  * b.sprite), then nitems pixel items. */
 template <bool CMP, bool DUAL, bool GRID, bool COUNT, bool GSTK>
 __global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut o, uint32_t nitems, uint32_t nspr,
-                                                 int nl, int* __restrict__ sprite, unsigned* __restrict__ q,
+                                                 int nl, int strip, int* __restrict__ sprite, unsigned* __restrict__ q,
                                                  unsigned refill, uint32_t* gstack,
                                                  unsigned long long* __restrict__ counters)
 {
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut
                 int            px, py;
                 item = it;
                 spr  = -1;
-                if (item_pixel(it - b.item_off[v], (V.w + 7) >> 3, V.w, V.h, px, py)) {
+                if (item_pixel(it - b.item_off[v], V.w, V.h, strip, px, py)) {
                     gp = b.pix_off[v] + (uint32_t) py * (uint32_t) V.w + (uint32_t) px;
                     float csv[3];
                     pixel_dir(V, px, py, csv);
@@ -623,7 +623,7 @@ struct GraphKey {
     void*    bufs[17];
     uint32_t nitems, npix, nviews, nlights;
     unsigned mask, flags;
-    int      bs, refill, gp, gs, stack, reorder;
+    int      bs, refill, gp, gs, stack, reorder, strip;
     float    band[2];
     float    lights[kNL][3];
     uint32_t tmpl;
@@ -706,7 +706,8 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
         k_zero_u32<<<1, 256, 0, st>>>(q, Q_N);
         k_zero_u8<<<nblk(npix), 256, 0, st>>>((uint8_t*) c->wb_disc.p, npix);
         c->prof_mark_begin(QBT_PROF_PRIMARY);
-        kp<<<gp, bs, lds, st>>>(s, b, po, nitems, nspr, nl > 0 ? nl : 1, (int*) c->sprite.p, q, refill, gstk, cnt);
+        kp<<<gp, bs, lds, st>>>(s, b, po, nitems, nspr, nl > 0 ? nl : 1, c->tune_strip, (int*) c->sprite.p, q,
+                                refill, gstk, cnt);
         c->prof_mark_end();
         compact(st, PredCand{po.cand, nitems}, nitems, 1, (uint32_t*) c->wb_bcnt.p, q + Q_NSEL, 1, items, 0);
         if (reorder) {
@@ -754,7 +755,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     memcpy(key.bufs, bp, sizeof(bp));
     key.nitems = nitems, key.npix = npix, key.nviews = (uint32_t) b.n, key.nlights = (uint32_t) nl;
     key.mask = mask, key.flags = flags, key.bs = bs, key.refill = (int) refill, key.gp = gp, key.gs = gs;
-    key.stack = GSTK, key.reorder = reorder;
+    key.stack = GSTK, key.reorder = reorder, key.strip = c->tune_strip;
     memcpy(key.lights, L.p, sizeof(key.lights));
     key.tmpl = (CMP ? 1u : 0u) | (DUAL ? 2u : 0u) | (GRID ? 4u : 0u);
     if (surrogate) qbt_surrogate_band(c, &key.band[0], &key.band[1]);

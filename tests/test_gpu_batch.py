@@ -26,7 +26,7 @@ def scene_ctx(gpu_ctx):
     gpu_ctx.upload_model(col, nrm)
     gpu_ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
     yield gpu_ctx, sc, po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth)
-    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=0, graph=1)
+    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=0, graph=1, strip=0)
 
 
 def _views(sc, seed, sizes=SIZES, maxlevel=None):
@@ -44,10 +44,10 @@ def _same(a, b, what):
         assert np.array_equal(a[k], b[k]), f"{what}: {k} differs ({np.count_nonzero(a[k] != b[k])} elements)"
 
 
-@pytest.mark.parametrize("stack,reorder", [(0, 1), (0, 0), (1, 1), (1, 0)])
-def test_batch_equals_oracle(scene_ctx, stack, reorder):
+@pytest.mark.parametrize("stack,reorder,strip", [(0, 1, 0), (0, 0, 0), (1, 1, 0), (1, 0, 0), (0, 0, 2), (0, 0, 5)])
+def test_batch_equals_oracle(scene_ctx, stack, reorder, strip):
     ctx, sc, osc = scene_ctx
-    ctx.set_tuning(path=1, stack=stack, reorder=reorder, graph=0)
+    ctx.set_tuning(path=1, stack=stack, reorder=reorder, graph=0, strip=strip)
     views, pos, ang = _views(sc, 21)
     print(f"[case] batch of {len(views)} views stack={stack} reorder={reorder}", flush=True)
     ctx.render(views, LIGHTS4, MASK)

@@ -20,15 +20,15 @@ ALL = q.OUT_ALL | q.RENDER_COUNT
 
 
 @pytest.fixture(params=[dict(path=1, layout=0), dict(path=1, layout=1), dict(path=0, layout=1),
-                        dict(path=1, layout=0, stack=1, reorder=0)],
+                        dict(path=1, layout=0, stack=1, reorder=0, strip=3)],
                 ids=["wavefront-compact", "wavefront-direct", "fused", "wavefront-gstack"])
 def ctx(gpu_ctx, request):
     """Every render path: wavefront stages on the compact node layout
     (default), wavefront on the direct layout, the fused kernel, and the
-    wavefront stages with global-memory stacks and unordered shadow rays."""
+    wavefront stages with global-memory stacks and tiles in 3-tile strips."""
     gpu_ctx.set_tuning(**request.param)
     yield gpu_ctx
-    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=0, graph=1)
+    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=0, graph=1, strip=0)
 
 
 def test_shared_child_falls_back_to_direct(ctx):
