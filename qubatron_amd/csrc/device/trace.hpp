@@ -63,7 +63,31 @@ struct SceneDev {
 
 struct Ray {
     float px, py, pz, dx, dy, dz;
+    float ix, iy, iz; /* RN(1/d) per axis (set by trace_begin), NaN where not usable */
 };
+
+/* RN(1 / d) when |d| is in [2^-100, 2^100] (the reciprocal is then a
+ * normal number), else NaN so that qdiv takes the division */
+__device__ __forceinline__ float recip_or_nan(float d)
+{
+    const float a = fabsf(d);
+    return (a >= 0x1p-100f && a <= 0x1p+100f) ? 1.0f / d : __builtin_nanf("");
+}
+
+/* a / b, correctly rounded, from y = RN(1 / b): q = RN(a y), r = a - b q
+ * (exact with an FMA), RN(q + r y) = RN(a / b) (Markstein's theorem for
+ * division with a correctly rounded reciprocal). Taken while q is a normal
+ * number well inside the range (no underflow or overflow in the steps);
+ * otherwise, and when y is NaN, the quotient is the division itself. The
+ * result is the IEEE quotient in every case, as the reference's division. */
+__device__ __forceinline__ float qdiv(float a, float b, float y)
+{
+    const float q  = a * y;
+    const float aq = fabsf(q);
+    if (!(aq >= 0x1p-100f && aq <= 0x1p+120f)) return a / b;
+    const float r = __builtin_fmaf(-q, b, a);
+    return r == 0.0f ? q : __builtin_fmaf(r, y, q);
+}
 
 struct Hit {
     float4 isp;
@@ -78,7 +102,7 @@ __device__ __forceinline__ float4 plane_x(float c, const Ray& r)
 {
     float4 o = CPU ? make_float4(0.0f, 0.0f, 0.0f, kFltMax) : make_float4(kFltMax, kFltMax, kFltMax, kFltMax);
     if (r.dx != 0.0f) {
-        float t = (c - r.px) / r.dx;
+        float t = qdiv(c - r.px, r.dx, r.ix);
         o.w     = t;
         o.y     = r.py + r.dy * t;
         o.z     = r.pz + r.dz * t;
@@ -92,7 +116,7 @@ __device__ __forceinline__ float4 plane_y(float c, const Ray& r)
 {
     float4 o = CPU ? make_float4(0.0f, 0.0f, 0.0f, kFltMax) : make_float4(kFltMax, kFltMax, kFltMax, kFltMax);
     if (r.dy != 0.0f) {
-        float t = (c - r.py) / r.dy;
+        float t = qdiv(c - r.py, r.dy, r.iy);
         o.w     = t;
         o.x     = r.px + r.dx * t;
         o.z     = r.pz + r.dz * t;
@@ -106,7 +130,7 @@ __device__ __forceinline__ float4 plane_z(float c, const Ray& r)
 {
     float4 o = CPU ? make_float4(0.0f, 0.0f, 0.0f, kFltMax) : make_float4(kFltMax, kFltMax, kFltMax, kFltMax);
     if (r.dz != 0.0f) {
-        float t = (c - r.pz) / r.dz;
+        float t = qdiv(c - r.pz, r.dz, r.iz);
         o.w     = t;
         o.x     = r.px + r.dx * t;
         o.y     = r.py + r.dy * t;
@@ -271,8 +295,11 @@ enum : int { ST_CONT = 2 };
 /* Root entry (octree_fsh.c:157-211). Returns ST_CONT, ST_HIT (maxlevel 0)
  * or ST_DISCARD. */
 template <bool CPU>
-__device__ __forceinline__ int trace_begin(const SceneDev& s, const Ray& r, TraceState& t)
+__device__ __forceinline__ int trace_begin(const SceneDev& s, Ray& r, TraceState& t)
 {
+    r.ix = recip_or_nan(r.dx);
+    r.iy = recip_or_nan(r.dy);
+    r.iz = recip_or_nan(r.dz);
     float4 h0 = make_float4(0.f, 0.f, 0.f, 0.f), h1 = h0;
     int    hitc = 0;
     {
@@ -623,8 +650,9 @@ template <bool CMP, bool DUAL, bool CPU, bool GRID>
 __device__ int trace(const SceneDev& s, const Ray& r, const Stack& st, Hit& out)
 {
     TraceState t;
-    int        stt = trace_begin<CPU>(s, r, t);
-    while (stt == ST_CONT) stt = trace_step<CMP, DUAL, CPU, GRID>(s, r, st, t);
+    Ray        rr  = r;
+    int        stt = trace_begin<CPU>(s, rr, t);
+    while (stt == ST_CONT) stt = trace_step<CMP, DUAL, CPU, GRID>(s, rr, st, t);
     out.exps = t.exps;
     if (stt == ST_HIT) {
         out.isp = t.entry;
