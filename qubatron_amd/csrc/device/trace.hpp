@@ -38,6 +38,14 @@ namespace qbt {
 constexpr int kMaxLevel = 24;
 constexpr float kFltMax = 3.402823466e+38f;
 
+/* An empty volatile asm on a value: the compiler must keep what depends on
+ * it inside its branch (a rarely taken slow path would otherwise be
+ * if-converted and computed on every lane). The host test build of this
+ * header defines it as nothing. */
+#ifndef QBT_OPAQUE
+#define QBT_OPAQUE(x) asm volatile("" : "+v"(x))
+#endif
+
 enum TraceStatus : int { ST_MISS = 0, ST_HIT = 1, ST_DISCARD = -1, ST_ABORT = -2 };
 
 struct TreeDev {
@@ -67,27 +75,61 @@ struct Ray {
 };
 
 /* RN(1 / d) when |d| is in [2^-100, 2^100] (the reciprocal is then a
- * normal number), else NaN so that qdiv takes the division */
+ * normal number), else NaN so that plane_k takes the division */
 __device__ __forceinline__ float recip_or_nan(float d)
 {
     const float a = fabsf(d);
     return (a >= 0x1p-100f && a <= 0x1p+100f) ? 1.0f / d : __builtin_nanf("");
 }
 
-/* a / b, correctly rounded, from y = RN(1 / b): q = RN(a y), r = a - b q
- * (exact with an FMA), RN(q + r y) = RN(a / b) (Markstein's theorem for
- * division with a correctly rounded reciprocal). Taken while q is a normal
- * number well inside the range (no underflow or overflow in the steps);
- * otherwise, and when y is NaN, the quotient is the division itself. The
- * result is the IEEE quotient in every case, as the reference's division. */
-__device__ __forceinline__ float qdiv(float a, float b, float y)
+/* Plane test on axis k (0 x, 1 y, 2 z) at coordinate c: is_cube_xplane,
+ * is_cube_yplane and is_cube_zplane (octree_fsh.c:62-99 / octree.c:302-339)
+ * in one form. t = (c - p_k) / d_k; the other two coordinates are
+ * p_j + d_j t, coordinate k is c; d_k == 0 gives the defaults (FLT_MAX, or
+ * 0 with t = FLT_MAX in the octree.c twin).
+ *
+ * The quotient is correctly rounded, as the reference's division, from
+ * y = RN(1 / d_k): q = RN(a y), e = a - d_k q (exact with an FMA),
+ * RN(q + e y) = RN(a / d_k) (Markstein's theorem for division with a
+ * correctly rounded reciprocal). That holds while q is a normal number well
+ * inside the range (no underflow or overflow in the steps); elsewhere, and
+ * where y is NaN, the quotient is the division itself, on a branch that
+ * rays rarely take. Everything else is selects, so that lanes of a wave
+ * do not split on the axis or on d_k == 0. */
+template <bool CPU>
+__device__ __forceinline__ float4 plane_k(int k, float c, const Ray& r)
 {
+    const float p  = k == 0 ? r.px : (k == 1 ? r.py : r.pz);
+    const float d  = k == 0 ? r.dx : (k == 1 ? r.dy : r.dz);
+    const float y  = k == 0 ? r.ix : (k == 1 ? r.iy : r.iz);
+    const float a  = c - p;
     const float q  = a * y;
+    const float e  = __builtin_fmaf(-q, d, a);
+    float       t  = __builtin_fmaf(e, y, q); /* = q where e == 0, since |q| >= 2^-100 there */
     const float aq = fabsf(q);
-    if (!(aq >= 0x1p-100f && aq <= 0x1p+120f)) return a / b;
-    const float r = __builtin_fmaf(-q, b, a);
-    return r == 0.0f ? q : __builtin_fmaf(r, y, q);
+    const bool  ok = d != 0.0f;
+    if (!(aq >= 0x1p-100f && aq <= 0x1p+120f) && ok) {
+        float aa = a, dd = d;
+        QBT_OPAQUE(aa);
+        QBT_OPAQUE(dd);
+        t = aa / dd;
+    }
+    const float X = r.px + r.dx * t, Y = r.py + r.dy * t, Z = r.pz + r.dz * t;
+    const float dflt = CPU ? 0.0f : kFltMax;
+    float4      o;
+    o.x = ok ? (k == 0 ? c : X) : dflt;
+    o.y = ok ? (k == 1 ? c : Y) : dflt;
+    o.z = ok ? (k == 2 ? c : Z) : dflt;
+    o.w = ok ? t : kFltMax;
+    return o;
 }
+
+template <bool CPU>
+__device__ __forceinline__ float4 plane_x(float c, const Ray& r) { return plane_k<CPU>(0, c, r); }
+template <bool CPU>
+__device__ __forceinline__ float4 plane_y(float c, const Ray& r) { return plane_k<CPU>(1, c, r); }
+template <bool CPU>
+__device__ __forceinline__ float4 plane_z(float c, const Ray& r) { return plane_k<CPU>(2, c, r); }
 
 struct Hit {
     float4 isp;
@@ -95,49 +137,6 @@ struct Hit {
     int ix, iy, iz; /* leaf cell coordinates */
     unsigned exps;  /* node expansions */
 };
-
-/* is_cube_{x,y,z}plane (octree_fsh.c:62-99 / octree.c:302-339) */
-template <bool CPU>
-__device__ __forceinline__ float4 plane_x(float c, const Ray& r)
-{
-    float4 o = CPU ? make_float4(0.0f, 0.0f, 0.0f, kFltMax) : make_float4(kFltMax, kFltMax, kFltMax, kFltMax);
-    if (r.dx != 0.0f) {
-        float t = qdiv(c - r.px, r.dx, r.ix);
-        o.w     = t;
-        o.y     = r.py + r.dy * t;
-        o.z     = r.pz + r.dz * t;
-        o.x     = c;
-    }
-    return o;
-}
-
-template <bool CPU>
-__device__ __forceinline__ float4 plane_y(float c, const Ray& r)
-{
-    float4 o = CPU ? make_float4(0.0f, 0.0f, 0.0f, kFltMax) : make_float4(kFltMax, kFltMax, kFltMax, kFltMax);
-    if (r.dy != 0.0f) {
-        float t = qdiv(c - r.py, r.dy, r.iy);
-        o.w     = t;
-        o.x     = r.px + r.dx * t;
-        o.z     = r.pz + r.dz * t;
-        o.y     = c;
-    }
-    return o;
-}
-
-template <bool CPU>
-__device__ __forceinline__ float4 plane_z(float c, const Ray& r)
-{
-    float4 o = CPU ? make_float4(0.0f, 0.0f, 0.0f, kFltMax) : make_float4(kFltMax, kFltMax, kFltMax, kFltMax);
-    if (r.dz != 0.0f) {
-        float t = qdiv(c - r.pz, r.dz, r.iz);
-        o.w     = t;
-        o.x     = r.px + r.dx * t;
-        o.y     = r.py + r.dy * t;
-        o.z     = c;
-    }
-    return o;
-}
 
 /* Node cube (tlf x, y, z) at `level` with cell coordinates (ix, iy, iz).
  * GRID: every coordinate of the subdivision is an exact fp32 multiple of the
@@ -174,7 +173,9 @@ __device__ __forceinline__ float4 midplane(int src, const Ray& r, float x, float
     float hx = bxx + (x - bxx) * 0.5f;
     float hy = byy + (y - byy) * 0.5f;
     float hz = bzz + (z - bzz) * 0.5f;
-    return src == 1 ? plane_z<CPU>(hz, r) : src == 2 ? plane_x<CPU>(hx, r) : plane_y<CPU>(hy, r);
+    const int   k = src == 1 ? 2 : src - 2; /* src 1 z, 2 x, 3 y */
+    const float c = src == 1 ? hz : (src == 2 ? hx : hy);
+    return plane_k<CPU>(k, c, r);
 }
 
 /* component selects by value (a conditional over lvalues selects an address
@@ -354,22 +355,6 @@ __device__ __forceinline__ int trace_begin(const SceneDev& s, Ray& r, TraceState
  * comparisons (octree_fsh.c:292-311), and its source; the entry point of the
  * first child is selected by source afterwards. Same results, fewer moves.
  */
-template <bool CPU, bool GRID>
-__device__ __forceinline__ float4 entry_of_c1(const SceneDev& s, const Ray& r, const Stack& st, int level, int ix,
-                                           int iy, int iz, float4 start)
-{
-    for (int m = level; m > 0; m--) {
-        const int src = (int) ((st.pw[(m - 1) * st.stride] >> 3) & 3u);
-        if (src != 0) {
-            const int sh = level - (m - 1);
-            float     x, y, z, ww;
-            cube_at<GRID>(s, m - 1, ix >> sh, iy >> sh, iz >> sh, x, y, z, ww);
-            return midplane<CPU>(src, r, x, y, z, ww);
-        }
-    }
-    return start;
-}
-
 /* candidate src (0 entry, 1 z, 2 x, 3 y) by value: a conditional over the
  * lvalues would select an address and keep the candidates in scratch memory */
 __device__ __forceinline__ float4 pick_entry(int src, float4 e, float4 z, float4 x, float4 y)
@@ -392,114 +377,119 @@ __device__ __forceinline__ void kswap(float& wa, int& ka, float& wb, int& kb)
     ka             = tk;
 }
 
+/* The octet of a candidate point and its tie-flipped octet
+ * (octree_fsh.c:292-311): the flip toggles the bit of the first axis, in
+ * x, y, z order, on which the point lies on the mid-plane (the lowest set
+ * bit of the on-plane mask). Key = octet | flipped << 3 | src << 6. */
+__device__ __forceinline__ int octet_key(float x, float y, float z, float hx, float hy, float hz, int src)
+{
+    const int o  = (x > hx ? 1 : 0) | (y < hy ? 2 : 0) | (z < hz ? 4 : 0);
+    const int on = (x == hx ? 1 : 0) | (y == hy ? 2 : 0) | (z == hz ? 4 : 0);
+    return o | ((o ^ (on & -on)) << 3) | (src << 6);
+}
+
 template <bool CPU, bool GRID>
 __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, const Stack& st, TraceState& t)
 {
-    if (++t.it > s.iter_cap) return ST_ABORT;
+    /* the iteration cap is checked at the end: a lane over it still takes
+     * this step (in bounds, like any other), then reports the abort */
+    const bool  over  = ++t.it > s.iter_cap;
     const int   level = t.level;
     const uint2 rec   = s.stat.rec[t.ns];
     t.exps++;
-    const uint32_t mask = rec.y;
-    if (mask) {
-        float tx, ty, tz, tw;
-        cube_at<GRID>(s, level, t.ix, t.iy, t.iz, tx, ty, tz, tw);
-        const float bxx = tx + tw, byy = ty - tw, bzz = tz - tw;
-        const float hx = bxx + (tx - bxx) * 0.5f;
-        const float hy = byy + (ty - byy) * 0.5f;
-        const float hz = bzz + (tz - bzz) * 0.5f;
-        const float qnan = __builtin_nanf("");
-        auto okey = [&](float x, float y, float z, int src) -> int {
-            const int o = (x > hx ? 1 : 0) | (y < hy ? 2 : 0) | (z < hz ? 4 : 0);
-            const int f = x == hx ? o ^ 1 : y == hy ? o ^ 2 : z == hz ? o ^ 4 : o;
-            return o | (f << 3) | (src << 6);
-        };
-        const float4 az = plane_z<CPU>(hz, r);
-        const float4 ax = plane_x<CPU>(hx, r);
-        const float4 ay = plane_y<CPU>(hy, r);
-        float        w0 = t.entry.w;
-        float        w1 = (az.w > 0.0f && tx < az.x && az.x <= bxx && ty > az.y && az.y >= byy) ? az.w : qnan;
-        float        w2 = (ax.w > 0.0f && ty > ax.y && ax.y >= byy && tz > ax.z && ax.z >= bzz) ? ax.w : qnan;
-        float        w3 = (ay.w > 0.0f && tx < ay.x && ay.x <= bxx && tz > ay.z && ay.z >= bzz) ? ay.w : qnan;
-        int          k0 = okey(t.entry.x, t.entry.y, t.entry.z, 0);
-        int          k1 = okey(az.x, az.y, az.z, 1);
-        int          k2 = okey(ax.x, ax.y, ax.z, 2);
-        int          k3 = okey(ay.x, ay.y, ay.z, 3);
-        kswap(w0, k0, w1, k1);
-        kswap(w0, k0, w2, k2);
-        kswap(w0, k0, w3, k3);
-        kswap(w1, k1, w2, k2);
-        kswap(w1, k1, w3, k3);
-        kswap(w2, k2, w3, k3);
+    const uint32_t mask = rec.y; /* 0 only for the root of an empty tree: then no candidate is taken */
 
-        int      pre = -1, n = 0, src0 = 0;
-        uint32_t word = 0, rank0 = 0;
-        int      oct0 = 0;
-#define QBT_CLASSIFY(w, k)                                                       \
-    if ((w) == (w)) {                                                            \
-        int o = (k) & 7;                                                         \
-        if (o == pre) o = ((k) >> 3) & 7;                                        \
-        pre = o;                                                                 \
-        if ((mask >> o) & 1u) {                                                  \
-            const uint32_t rk = (uint32_t) __popc(mask & ((1u << o) - 1u));      \
-            word |= ((uint32_t) o | ((uint32_t) ((k) >> 6) << 3) | (rk << 5)) << (8 * n); \
-            if (n == 0) {                                                        \
-                src0  = (k) >> 6;                                                \
-                oct0  = o;                                                       \
-                rank0 = rk;                                                      \
-            }                                                                    \
-            n++;                                                                 \
-        }                                                                        \
+    /* ---- expansion (octree_fsh.c:251-330), without branches ---- */
+    float tx, ty, tz, tw;
+    cube_at<GRID>(s, level, t.ix, t.iy, t.iz, tx, ty, tz, tw);
+    const float  bxx = tx + tw, byy = ty - tw, bzz = tz - tw;
+    const float  hx = bxx + (tx - bxx) * 0.5f;
+    const float  hy = byy + (ty - byy) * 0.5f;
+    const float  hz = bzz + (tz - bzz) * 0.5f;
+    const float  qnan = __builtin_nanf("");
+    const float4 az = plane_z<CPU>(hz, r);
+    const float4 ax = plane_x<CPU>(hx, r);
+    const float4 ay = plane_y<CPU>(hy, r);
+    float        w0 = t.entry.w;
+    float        w1 = (az.w > 0.0f && tx < az.x && az.x <= bxx && ty > az.y && az.y >= byy) ? az.w : qnan;
+    float        w2 = (ax.w > 0.0f && ty > ax.y && ax.y >= byy && tz > ax.z && ax.z >= bzz) ? ax.w : qnan;
+    float        w3 = (ay.w > 0.0f && tx < ay.x && ay.x <= bxx && tz > ay.z && ay.z >= bzz) ? ay.w : qnan;
+    int          k0 = octet_key(t.entry.x, t.entry.y, t.entry.z, hx, hy, hz, 0);
+    int          k1 = octet_key(az.x, az.y, az.z, hx, hy, hz, 1);
+    int          k2 = octet_key(ax.x, ax.y, ax.z, hx, hy, hz, 2);
+    int          k3 = octet_key(ay.x, ay.y, ay.z, hx, hy, hz, 3);
+    kswap(w0, k0, w1, k1);
+    kswap(w0, k0, w2, k2);
+    kswap(w0, k0, w3, k3);
+    kswap(w1, k1, w2, k2);
+    kswap(w1, k1, w3, k3);
+    kswap(w2, k2, w3, k3);
+
+    /* walk the sorted candidates: flip an octet equal to the previous one,
+     * keep those whose child exists, in order, as entry bytes */
+    int      pre = -1, n = 0;
+    uint32_t word = 0;
+#define QBT_CLASSIFY(w, k)                                                                   \
+    {                                                                                        \
+        const bool     v    = (w) == (w);                                                    \
+        const int      o0   = (k) & 7;                                                       \
+        const int      o    = o0 == pre ? (((k) >> 3) & 7) : o0;                             \
+        const bool     take = v && ((mask >> o) & 1u);                                       \
+        const uint32_t rk   = (uint32_t) __popc(mask & ((1u << o) - 1u));                    \
+        const uint32_t e    = (uint32_t) o | ((uint32_t) ((k) >> 6) << 3) | (rk << 5);       \
+        pre                 = v ? o : pre;                                                   \
+        word |= take ? e << (8 * n) : 0u;                                                    \
+        n += take ? 1 : 0;                                                                   \
     }
-        QBT_CLASSIFY(w0, k0)
-        QBT_CLASSIFY(w1, k1)
-        QBT_CLASSIFY(w2, k2)
-        QBT_CLASSIFY(w3, k3)
+    QBT_CLASSIFY(w0, k0)
+    QBT_CLASSIFY(w1, k1)
+    QBT_CLASSIFY(w2, k2)
+    QBT_CLASSIFY(w3, k3)
 #undef QBT_CLASSIFY
 
-        if (n > 0) {
-            st.pw[level * st.stride] = word;
-            st.ns[level * st.stride] = rec.x | ((uint32_t) (n - 1) << 30);
-            t.entry = pick_entry(src0, t.entry, az, ax, ay);
-            t.ns    = (int) (rec.x + rank0);
-            t.ix    = (t.ix << 1) | (oct0 & 1);
-            t.iy    = (t.iy << 1) | ((oct0 >> 1) & 1);
-            t.iz    = (t.iz << 1) | (oct0 >> 2);
-            t.level = level + 1;
-            return t.level == s.maxlevel ? ST_HIT : ST_CONT;
+    /* the entry to descend into: the first kept candidate, or, backtracking
+     * (octree_fsh.c:368-375), the next pending entry of the nearest
+     * ancestor that has one */
+    int      lv  = level;
+    uint32_t w   = word, a = rec.x;
+    float4   isp = pick_entry((int) ((word >> 3) & 3u), t.entry, az, ax, ay);
+    if (n > 0) {
+        st.pw[level * st.stride] = word;
+        st.ns[level * st.stride] = rec.x | ((uint32_t) (n - 1) << 30);
+    } else {
+        uint32_t an;
+        do {
+            if (--lv < 0) return over ? ST_ABORT : ST_MISS;
+            an = (uint32_t) st.ns[lv * st.stride];
+        } while (!(an >> 30));
+        w                     = st.pw[lv * st.stride] >> 8; /* the next entry becomes the current one */
+        st.pw[lv * st.stride] = w;
+        st.ns[lv * st.stride] = (int) (an - (1u << 30));
+        a                     = an & 0x3FFFFFFFu;
+        /* its entry point: the mid-plane hit of the nearest node on the path
+         * (this one or an ancestor) entered through a mid-plane, else the
+         * root entry */
+        int j = lv, src = (int) ((w >> 3) & 3u);
+        while (src == 0 && j > 0) {
+            j--;
+            src = (int) ((st.pw[j * st.stride] >> 3) & 3u);
+        }
+        isp = t.start;
+        if (src != 0) {
+            const int sh = level - j;
+            float     x, y, z, ww;
+            cube_at<GRID>(s, j, t.ix >> sh, t.iy >> sh, t.iz >> sh, x, y, z, ww);
+            isp = midplane<CPU>(src, r, x, y, z, ww);
         }
     }
-
-    /* backtrack (octree_fsh.c:368-375) and pop the next pending entry */
-    int lv = level, ix = t.ix, iy = t.iy, iz = t.iz;
-    for (;;) {
-        lv--;
-        if (lv < 0) return ST_MISS;
-        ix >>= 1;
-        iy >>= 1;
-        iz >>= 1;
-        const uint32_t a = (uint32_t) st.ns[lv * st.stride];
-        if (a >> 30) {
-            const uint32_t w = st.pw[lv * st.stride] >> 8; /* the next entry becomes the current one */
-            st.pw[lv * st.stride] = w;
-            st.ns[lv * st.stride] = (int) (a - (1u << 30));
-            const int o = (int) (w & 7u), src = (int) ((w >> 3) & 3u);
-            float4    isp;
-            if (src == 0) {
-                isp = entry_of_c1<CPU, GRID>(s, r, st, lv, ix, iy, iz, t.start);
-            } else {
-                float x, y, z, ww;
-                cube_at<GRID>(s, lv, ix, iy, iz, x, y, z, ww);
-                isp = midplane<CPU>(src, r, x, y, z, ww);
-            }
-            t.ns    = (int) ((a & 0x3FFFFFFFu) + ((w >> 5) & 7u));
-            t.ix    = (ix << 1) | (o & 1);
-            t.iy    = (iy << 1) | ((o >> 1) & 1);
-            t.iz    = (iz << 1) | (o >> 2);
-            t.entry = isp;
-            t.level = lv + 1;
-            return t.level == s.maxlevel ? ST_HIT : ST_CONT;
-        }
-    }
+    const int sh = level - lv, o = (int) (w & 7u);
+    t.ns    = (int) (a + ((w >> 5) & 7u));
+    t.ix    = ((t.ix >> sh) << 1) | (o & 1);
+    t.iy    = ((t.iy >> sh) << 1) | ((o >> 1) & 1);
+    t.iz    = ((t.iz >> sh) << 1) | (o >> 2);
+    t.entry = isp;
+    t.level = lv + 1;
+    return over ? ST_ABORT : (t.level == s.maxlevel ? ST_HIT : ST_CONT);
 }
 
 /*

@@ -12,6 +12,7 @@
 #define __device__
 #define __host__
 #define __forceinline__ inline
+#define QBT_OPAQUE(x) ((void) 0)
 
 struct float4 {
     float x, y, z, w;
