@@ -117,6 +117,30 @@ struct Refill {
     }
 };
 
+/* Lanes with a ray step until it ends; the wave leaves the loop together
+ * once fewer than `keep` lanes are left (wave-uniform), so that the refill
+ * test, the ballots and the merge of the ray state run once per refill,
+ * not once per step. */
+__device__ __forceinline__ int keep_active(const Refill& rf, unsigned refill)
+{
+    return rf.left > 0 ? 65 - (int) refill : 1; /* refill when idle lanes >= refill */
+}
+
+template <bool CMP, bool DUAL, bool GRID, class F>
+__device__ __forceinline__ void step_run(const SceneDev& s, const Ray& r, const Stack& st, TraceState& t, bool& active,
+                                         int keep, F&& finish)
+{
+    for (;;) {
+        const int stt = trace_step<CMP, DUAL, false, GRID>(s, r, st, t);
+        if (stt != ST_CONT) {
+            finish(stt);
+            active = false;
+            break;
+        }
+        if (__popcll(__ballot(1)) < keep) break;
+    }
+}
+
 /* the calling lane's traversal stack: LDS, or global memory (GSTK) */
 template <bool DUAL, bool GSTK>
 __device__ __forceinline__ Stack lane_stack(uint32_t* lds, uint32_t* gstack, int L)
@@ -160,7 +184,10 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut
         uint8_t  f    = 0;
         uint32_t cand = UINT32_MAX;
         if (stt == ST_DISCARD) f = QBT_PX_DISCARD;
-        if (stt == ST_ABORT) n_abort++;
+        /* counters by value: increments under different branches would be
+         * merged into a select of their addresses (scratch memory) */
+        n_abort += stt == ST_ABORT ? 1u : 0u;
+        n_hit += (stt == ST_HIT && t.entry.w > 0.0f) ? 1u : 0u;
         if (stt == ST_HIT) {
             const int L = s.maxlevel;
             ms          = (t.ns == 0 && L > 0) ? 0 : s.stat.model[t.ns];
@@ -172,7 +199,6 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut
             if (isp.w > 0.0f) {
                 f |= QBT_PX_SHADED;
                 cand = gp;
-                n_hit++;
             }
         }
         o.isp[gp]    = isp;
@@ -233,13 +259,7 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut
             if (rf.left <= 0) break;
             continue;
         }
-        if (active) {
-            int stt = trace_step<CMP, DUAL, false, GRID>(s, r, st, t);
-            if (stt != ST_CONT) {
-                finish(stt);
-                active = false;
-            }
-        }
+        if (active) step_run<CMP, DUAL, GRID>(s, r, st, t, active, keep_active(rf, refill), finish);
     }
     n_hit   = wave_sum(n_hit);
     n_abort = wave_sum(n_abort);
@@ -481,13 +501,7 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, float Lx, float Ly, 
             if (rf.left <= 0) break;
             continue;
         }
-        if (active) {
-            int stt = trace_step<CMP, DUAL, false, GRID>(s, r, st, t);
-            if (stt != ST_CONT) {
-                finish(stt);
-                active = false;
-            }
-        }
+        if (active) step_run<CMP, DUAL, GRID>(s, r, st, t, active, keep_active(rf, refill), finish);
     }
     n_ray   = wave_sum(n_ray);
     n_abort = wave_sum(n_abort);

@@ -96,12 +96,19 @@ __device__ __forceinline__ float recip_or_nan(float d)
  * where y is NaN, the quotient is the division itself, on a branch that
  * rays rarely take. Everything else is selects, so that lanes of a wave
  * do not split on the axis or on d_k == 0. */
+/* component k by value (a conditional over the Ray's members would select an
+ * address and keep the Ray in scratch memory) */
+__device__ __forceinline__ float axis3(int k, float x, float y, float z)
+{
+    return k == 0 ? x : (k == 1 ? y : z);
+}
+
 template <bool CPU>
 __device__ __forceinline__ float4 plane_k(int k, float c, const Ray& r)
 {
-    const float p  = k == 0 ? r.px : (k == 1 ? r.py : r.pz);
-    const float d  = k == 0 ? r.dx : (k == 1 ? r.dy : r.dz);
-    const float y  = k == 0 ? r.ix : (k == 1 ? r.iy : r.iz);
+    const float p  = axis3(k, r.px, r.py, r.pz);
+    const float d  = axis3(k, r.dx, r.dy, r.dz);
+    const float y  = axis3(k, r.ix, r.iy, r.iz);
     const float a  = c - p;
     const float q  = a * y;
     const float e  = __builtin_fmaf(-q, d, a);
@@ -149,10 +156,10 @@ __device__ __forceinline__ void cube_at(const SceneDev& s, int level, int ix, in
                                         float& z, float& w)
 {
     w = ldexpf(s.bw, -level);
-    if (GRID) {
-        x = s.bx + (float) ix * w;
-        y = s.by - (float) iy * w;
-        z = s.bz - (float) iz * w;
+    if (GRID) { /* i * w is exact, so the FMA rounds as the separate add */
+        x = __builtin_fmaf((float) ix, w, s.bx);
+        y = __builtin_fmaf(-(float) iy, w, s.by);
+        z = __builtin_fmaf(-(float) iz, w, s.bz);
     } else {
         x = s.bx, y = s.by, z = s.bz;
         for (int l = 0; l < level; l++) {
@@ -165,14 +172,33 @@ __device__ __forceinline__ void cube_at(const SceneDev& s, int level, int ix, in
     }
 }
 
+/* Far corner (bxx, byy, bzz) and mid-planes (hx, hy, hz) of the node cube
+ * (x, y, z, w), as the shader computes them (octree_fsh.c:255-256:
+ * h = b + (t - b) * 0.5). On an exact grid every one of these values is a
+ * multiple of the leaf edge that fp32 holds exactly, so h = t +- w / 2 is
+ * the same number in one FMA. */
+template <bool GRID>
+__device__ __forceinline__ void cube_mid(float x, float y, float z, float w, float& bxx, float& byy, float& bzz,
+                                         float& hx, float& hy, float& hz)
+{
+    bxx = x + w, byy = y - w, bzz = z - w;
+    if (GRID) {
+        hx = __builtin_fmaf(0.5f, w, x);
+        hy = __builtin_fmaf(-0.5f, w, y);
+        hz = __builtin_fmaf(-0.5f, w, z);
+    } else {
+        hx = bxx + (x - bxx) * 0.5f;
+        hy = byy + (y - byy) * 0.5f;
+        hz = bzz + (z - bzz) * 0.5f;
+    }
+}
+
 /* Mid-plane hit `src` (1 z, 2 x, 3 y) of the node cube (x, y, z, w). */
-template <bool CPU>
+template <bool CPU, bool GRID>
 __device__ __forceinline__ float4 midplane(int src, const Ray& r, float x, float y, float z, float w)
 {
-    float bxx = x + w, byy = y - w, bzz = z - w;
-    float hx = bxx + (x - bxx) * 0.5f;
-    float hy = byy + (y - byy) * 0.5f;
-    float hz = bzz + (z - bzz) * 0.5f;
+    float bxx, byy, bzz, hx, hy, hz;
+    cube_mid<GRID>(x, y, z, w, bxx, byy, bzz, hx, hy, hz);
     const int   k = src == 1 ? 2 : src - 2; /* src 1 z, 2 x, 3 y */
     const float c = src == 1 ? hz : (src == 2 ? hx : hy);
     return plane_k<CPU>(k, c, r);
@@ -274,7 +300,7 @@ __device__ __forceinline__ float4 entry_of(const SceneDev& s, const Ray& r, cons
             int   sh = level - (m - 1);
             float x, y, z, ww;
             cube_at<GRID>(s, m - 1, ix >> sh, iy >> sh, iz >> sh, x, y, z, ww);
-            return midplane<CPU>(src, r, x, y, z, ww);
+            return midplane<CPU, GRID>(src, r, x, y, z, ww);
         }
     }
     return start;
@@ -380,12 +406,34 @@ __device__ __forceinline__ void kswap(float& wa, int& ka, float& wb, int& kb)
 /* The octet of a candidate point and its tie-flipped octet
  * (octree_fsh.c:292-311): the flip toggles the bit of the first axis, in
  * x, y, z order, on which the point lies on the mid-plane (the lowest set
- * bit of the on-plane mask). Key = octet | flipped << 3 | src << 6. */
+ * bit of the on-plane mask). Key = octet | src << 3 | flipped << 5. */
 __device__ __forceinline__ int octet_key(float x, float y, float z, float hx, float hy, float hz, int src)
 {
     const int o  = (x > hx ? 1 : 0) | (y < hy ? 2 : 0) | (z < hz ? 4 : 0);
     const int on = (x == hx ? 1 : 0) | (y == hy ? 2 : 0) | (z == hz ? 4 : 0);
-    return o | ((o ^ (on & -on)) << 3) | (src << 6);
+    return o | (src << 3) | ((o ^ (on & -on)) << 5);
+}
+
+/* The same keys for the three mid-plane candidates, whose coordinate on
+ * their own plane is the mid-plane value itself (plane_k sets it) wherever
+ * the candidate is valid, so that one comparison of each is known: z == hz
+ * for the z-plane hit, x == hx for the x-plane hit, y == hy for the y-plane
+ * hit. (Keys of invalid candidates are never read.) */
+__device__ __forceinline__ int zplane_key(float x, float y, float hx, float hy)
+{
+    const int o  = (x > hx ? 1 : 0) | (y < hy ? 2 : 0);
+    const int fl = x == hx ? 1 : (y == hy ? 2 : 4);
+    return o | (1 << 3) | ((o ^ fl) << 5);
+}
+__device__ __forceinline__ int xplane_key(float y, float z, float hy, float hz)
+{
+    const int o = (y < hy ? 2 : 0) | (z < hz ? 4 : 0);
+    return o | (2 << 3) | ((o ^ 1) << 5);
+}
+__device__ __forceinline__ int yplane_key(float x, float z, float hx, float hz)
+{
+    const int o = (x > hx ? 1 : 0) | (z < hz ? 4 : 0);
+    return o | (3 << 3) | ((o ^ (x == hx ? 1 : 2)) << 5);
 }
 
 template <bool CPU, bool GRID>
@@ -400,12 +448,9 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     const uint32_t mask = rec.y; /* 0 only for the root of an empty tree: then no candidate is taken */
 
     /* ---- expansion (octree_fsh.c:251-330), without branches ---- */
-    float tx, ty, tz, tw;
+    float tx, ty, tz, tw, bxx, byy, bzz, hx, hy, hz;
     cube_at<GRID>(s, level, t.ix, t.iy, t.iz, tx, ty, tz, tw);
-    const float  bxx = tx + tw, byy = ty - tw, bzz = tz - tw;
-    const float  hx = bxx + (tx - bxx) * 0.5f;
-    const float  hy = byy + (ty - byy) * 0.5f;
-    const float  hz = bzz + (tz - bzz) * 0.5f;
+    cube_mid<GRID>(tx, ty, tz, tw, bxx, byy, bzz, hx, hy, hz);
     const float  qnan = __builtin_nanf("");
     const float4 az = plane_z<CPU>(hz, r);
     const float4 ax = plane_x<CPU>(hx, r);
@@ -415,9 +460,11 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     float        w2 = (ax.w > 0.0f && ty > ax.y && ax.y >= byy && tz > ax.z && ax.z >= bzz) ? ax.w : qnan;
     float        w3 = (ay.w > 0.0f && tx < ay.x && ay.x <= bxx && tz > ay.z && ay.z >= bzz) ? ay.w : qnan;
     int          k0 = octet_key(t.entry.x, t.entry.y, t.entry.z, hx, hy, hz, 0);
-    int          k1 = octet_key(az.x, az.y, az.z, hx, hy, hz, 1);
-    int          k2 = octet_key(ax.x, ax.y, ax.z, hx, hy, hz, 2);
-    int          k3 = octet_key(ay.x, ay.y, ay.z, hx, hy, hz, 3);
+    /* the octree.c twin's defaults for d == 0 (0, 0, 0) can pass the
+     * containment tests, so it keeps the general form */
+    int k1 = CPU ? octet_key(az.x, az.y, az.z, hx, hy, hz, 1) : zplane_key(az.x, az.y, hx, hy);
+    int k2 = CPU ? octet_key(ax.x, ax.y, ax.z, hx, hy, hz, 2) : xplane_key(ax.y, ax.z, hy, hz);
+    int k3 = CPU ? octet_key(ay.x, ay.y, ay.z, hx, hy, hz, 3) : yplane_key(ay.x, ay.z, hx, hz);
     kswap(w0, k0, w1, k1);
     kswap(w0, k0, w2, k2);
     kswap(w0, k0, w3, k3);
@@ -433,10 +480,10 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     {                                                                                        \
         const bool     v    = (w) == (w);                                                    \
         const int      o0   = (k) & 7;                                                       \
-        const int      o    = o0 == pre ? (((k) >> 3) & 7) : o0;                             \
+        const int      o    = o0 == pre ? (((k) >> 5) & 7) : o0;                             \
         const bool     take = v && ((mask >> o) & 1u);                                       \
         const uint32_t rk   = (uint32_t) __popc(mask & ((1u << o) - 1u));                    \
-        const uint32_t e    = (uint32_t) o | ((uint32_t) ((k) >> 6) << 3) | (rk << 5);       \
+        const uint32_t e    = (uint32_t) o | ((uint32_t) (k) & 0x18u) | (rk << 5);           \
         pre                 = v ? o : pre;                                                   \
         word |= take ? e << (8 * n) : 0u;                                                    \
         n += take ? 1 : 0;                                                                   \
@@ -479,7 +526,7 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
             const int sh = level - j;
             float     x, y, z, ww;
             cube_at<GRID>(s, j, t.ix >> sh, t.iy >> sh, t.iz >> sh, x, y, z, ww);
-            isp = midplane<CPU>(src, r, x, y, z, ww);
+            isp = midplane<CPU, GRID>(src, r, x, y, z, ww);
         }
     }
     const int sh = level - lv, o = (int) (w & 7u);
@@ -521,12 +568,9 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
     if (DUAL) mask |= db.mask();
 
     if (mask) {
-        float tx, ty, tz, tw;
+        float tx, ty, tz, tw, bxx, byy, bzz, hx, hy, hz;
         cube_at<GRID>(s, level, t.ix, t.iy, t.iz, tx, ty, tz, tw);
-        const float bxx = tx + tw, byy = ty - tw, bzz = tz - tw;
-        const float hx = bxx + (tx - bxx) * 0.5f;
-        const float hy = byy + (ty - byy) * 0.5f;
-        const float hz = bzz + (tz - bzz) * 0.5f;
+        cube_mid<GRID>(tx, ty, tz, tw, bxx, byy, bzz, hx, hy, hz);
 
         const float qnan = __builtin_nanf("");
         Cand        c0 = {t.entry.x, t.entry.y, t.entry.z, t.entry.w, 0};
@@ -617,7 +661,7 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
             } else {
                 float x, y, z, ww;
                 cube_at<GRID>(s, lv, ix, iy, iz, x, y, z, ww);
-                isp = midplane<CPU>(src, r, x, y, z, ww);
+                isp = midplane<CPU, GRID>(src, r, x, y, z, ww);
             }
             t.ns = child_slot<CMP>(s.stat, st.ns[lv * st.stride], o, lv);
             t.nd = DUAL ? child_slot<CMP>(s.dyn, st.nd[lv * st.stride], o, lv) : 0;
