@@ -85,8 +85,8 @@ __device__ __forceinline__ float recip_or_nan(float d)
 /* Plane test on axis k (0 x, 1 y, 2 z) at coordinate c: is_cube_xplane,
  * is_cube_yplane and is_cube_zplane (octree_fsh.c:62-99 / octree.c:302-339)
  * in one form. t = (c - p_k) / d_k; the other two coordinates are
- * p_j + d_j t, coordinate k is c; d_k == 0 gives the defaults (FLT_MAX, or
- * 0 with t = FLT_MAX in the octree.c twin).
+ * p_j + d_j t, coordinate k is c; d_k == 0 gives the octree.c twin's
+ * defaults (0, 0, 0, t = FLT_MAX), and NaN in shader semantics (see below).
  *
  * The quotient is correctly rounded, as the reference's division, from
  * y = RN(1 / d_k): q = RN(a y), e = a - d_k q (exact with an FMA),
@@ -122,12 +122,22 @@ __device__ __forceinline__ float4 plane_k(int k, float c, const Ray& r)
         t = aa / dd;
     }
     const float X = r.px + r.dx * t, Y = r.py + r.dy * t, Z = r.pz + r.dz * t;
-    const float dflt = CPU ? 0.0f : kFltMax;
     float4      o;
-    o.x = ok ? (k == 0 ? c : X) : dflt;
-    o.y = ok ? (k == 1 ? c : Y) : dflt;
-    o.z = ok ? (k == 2 ? c : Z) : dflt;
-    o.w = ok ? t : kFltMax;
+    o.x = k == 0 ? c : X;
+    o.y = k == 1 ? c : Y;
+    o.z = k == 2 ? c : Z;
+    o.w = t;
+    if (CPU) { /* the octree.c twin's defaults can pass the containment tests */
+        o.x = ok ? o.x : 0.0f;
+        o.y = ok ? o.y : 0.0f;
+        o.z = ok ? o.z : 0.0f;
+        o.w = ok ? o.w : kFltMax;
+    }
+    /* Shader semantics: where d_k == 0, y is NaN and so are t and the other
+     * two coordinates; every caller tests t > 0 and the containment of the
+     * point before using it, and those comparisons are false for NaN just
+     * as they are for the shader's FLT_MAX defaults (octree_fsh.c:64).
+     * No select is spent on the defaults. */
     return o;
 }
 
@@ -465,34 +475,45 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     int k1 = CPU ? octet_key(az.x, az.y, az.z, hx, hy, hz, 1) : zplane_key(az.x, az.y, hx, hy);
     int k2 = CPU ? octet_key(ax.x, ax.y, ax.z, hx, hy, hz, 2) : xplane_key(ax.y, ax.z, hy, hz);
     int k3 = CPU ? octet_key(ay.x, ay.y, ay.z, hx, hy, hz, 3) : yplane_key(ay.x, ay.z, hx, hz);
-    kswap(w0, k0, w1, k1);
-    kswap(w0, k0, w2, k2);
-    kswap(w0, k0, w3, k3);
+    /* Sort network (octree_fsh.c:283, strict: ties keep their order). Slot
+     * 0 holds the node's own entry point, which lies before every point of
+     * the ray inside the node but for rounding near an edge, so the three
+     * exchanges with slot 0 are no-ops for almost every lane; the branch
+     * skips them where no lane of the wave needs them. NaN never moves, so
+     * slot 0 stays valid. */
+    if (w1 < w0 || w2 < w0 || w3 < w0) {
+        kswap(w0, k0, w1, k1);
+        kswap(w0, k0, w2, k2);
+        kswap(w0, k0, w3, k3);
+    }
     kswap(w1, k1, w2, k2);
     kswap(w1, k1, w3, k3);
     kswap(w2, k2, w3, k3);
 
-    /* walk the sorted candidates: flip an octet equal to the previous one,
-     * keep those whose child exists, in order, as entry bytes */
-    int      pre = -1, n = 0;
-    uint32_t word = 0;
-#define QBT_CLASSIFY(w, k)                                                                   \
-    {                                                                                        \
-        const bool     v    = (w) == (w);                                                    \
-        const int      o0   = (k) & 7;                                                       \
-        const int      o    = o0 == pre ? (((k) >> 5) & 7) : o0;                             \
-        const bool     take = v && ((mask >> o) & 1u);                                       \
-        const uint32_t rk   = (uint32_t) __popc(mask & ((1u << o) - 1u));                    \
-        const uint32_t e    = (uint32_t) o | ((uint32_t) (k) & 0x18u) | (rk << 5);           \
-        pre                 = v ? o : pre;                                                   \
-        word |= take ? e << (8 * n) : 0u;                                                    \
-        n += take ? 1 : 0;                                                                   \
-    }
-    QBT_CLASSIFY(w0, k0)
-    QBT_CLASSIFY(w1, k1)
-    QBT_CLASSIFY(w2, k2)
-    QBT_CLASSIFY(w3, k3)
-#undef QBT_CLASSIFY
+    /* walk the sorted candidates: an octet equal to the previous valid one
+     * is replaced by its flipped octet; those whose child exists are kept,
+     * in order, as entry bytes (the word is assembled from the last one, so
+     * that the first kept one ends in the low byte) */
+    const bool v1 = w1 == w1, v2 = w2 == w2, v3 = w3 == w3;
+    const int  o0 = k0 & 7;
+    const int  o1 = (k1 & 7) == o0 ? ((k1 >> 5) & 7) : (k1 & 7);
+    const int  p1 = v1 ? o1 : o0;
+    const int  o2 = (k2 & 7) == p1 ? ((k2 >> 5) & 7) : (k2 & 7);
+    const int  p2 = v2 ? o2 : p1;
+    const int  o3 = (k3 & 7) == p2 ? ((k3 >> 5) & 7) : (k3 & 7);
+    auto entry = [&](int o, int k) -> uint32_t {
+        const uint32_t rk = (uint32_t) __popc(mask & ((1u << o) - 1u));
+        return (uint32_t) o | ((uint32_t) k & 0x18u) | (rk << 5);
+    };
+    const bool t0 = (mask >> o0) & 1u;
+    const bool t1 = v1 && ((mask >> o1) & 1u);
+    const bool t2 = v2 && ((mask >> o2) & 1u);
+    const bool t3 = v3 && ((mask >> o3) & 1u);
+    uint32_t   word = t3 ? entry(o3, k3) : 0u;
+    word            = t2 ? (word << 8) | entry(o2, k2) : word;
+    word            = t1 ? (word << 8) | entry(o1, k1) : word;
+    word            = t0 ? (word << 8) | entry(o0, k0) : word;
+    const int n     = (t0 ? 1 : 0) + (t1 ? 1 : 0) + (t2 ? 1 : 0) + (t3 ? 1 : 0);
 
     /* the entry to descend into: the first kept candidate, or, backtracking
      * (octree_fsh.c:368-375), the next pending entry of the nearest
