@@ -27,6 +27,12 @@ CPU baseline: the C oracle (restatement of the reference traversal) on whole
 frames of the timed camera path on the host's cores, each frame checked
 bit-exactly against the GPU frame.
 
+Two more legs, reported beside the headline: the same scene at 3840x2160
+(uhd_3840x2160), and batched views (batch_views): the 90M-point scene with 4
+lights, 64 views of resolutions drawn from 320x240 ... 3840x2160 at random
+poses rendered as one hipGraph-captured batch with the autotuner's tuning,
+two of its views checked bit for bit against single-view renders.
+
 Prints ONE JSON line on stdout; progress goes to stderr.
 
 ``python bench.py --summarize-pmc DIR...`` instead reads rocprofv3 counter
@@ -79,6 +85,9 @@ def parse():
     ap.add_argument("--sur-batch", type=int, default=4096)
     ap.add_argument("--sur-lr", type=float, default=2e-3)
     ap.add_argument("--sur-band", type=float, nargs=2, default=(0.1, 0.9))
+    ap.add_argument("--uhd-frames", type=int, default=5, help="frames of the 3840x2160 leg (0 = skip)")
+    ap.add_argument("--batch-views", type=int, default=64, help="views of the batched leg (0 = skip)")
+    ap.add_argument("--batch-reps", type=int, default=3, help="timed batches of the batched leg")
     ap.add_argument("--traffic", default=TRAFFIC_FILE)
     ap.add_argument("--summarize-pmc", nargs="*", default=None, metavar="DIR")
     ap.add_argument("--path", type=int, default=None, help="render path (0 fused, 1 wavefront); default: library's")
@@ -156,7 +165,7 @@ def load_traffic(path):
         return {}
 
 
-def roofline(args, prof, K, W, H, nl, exps, exps_p, leafhits, copy_gbps):
+def roofline(args, prof, K, W, H, nl, exps, exps_p, leafhits, copy_gbps, with_traffic=True):
     """Per-kernel and per-frame algorithmic bytes against the measured kernel times."""
     km, kl = prof["kind_ms"], prof["kind_launches"]
     e_p = float(np.mean(exps_p))
@@ -171,7 +180,7 @@ def roofline(args, prof, K, W, H, nl, exps, exps_p, leafhits, copy_gbps):
     if kl[KIND_SHADOW]:
         kern["k_shadow"] = {"alg_bytes_per_launch": 32.0 * e_s / nl, "avg_ms": km[KIND_SHADOW] / kl[KIND_SHADOW],
                             "share": km[KIND_SHADOW], "launches": kl[KIND_SHADOW]}
-    traffic = load_traffic(args.traffic)
+    traffic = load_traffic(args.traffic) if with_traffic else {}
     for name, e in kern.items():
         e["achieved_gbs"] = e["alg_bytes_per_launch"] / (e["avg_ms"] * 1e-3) / 1e9
         e["frac"] = e["achieved_gbs"] / HBM_PEAK_GBS
@@ -259,6 +268,110 @@ def surrogate_leg(args, q, ctx, sc, views, Wm, K, mask, exact_wall, exact_rays, 
     return res
 
 
+def count_frames(q, ctx, views, lights, mask):
+    """Untimed counting renders: per view, rays, expansions (all, primary)
+    and primary leaf hits."""
+    rays, exps, exps_p, leafhits = [], [], [], []
+    for v in views:
+        ctx.render(v, lights, mask | q.OUT_FLAGS | q.RENDER_COUNT)
+        st = ctx.stats()
+        fl = ctx.read_frame(0, fields=("flags",))["flags"]
+        rays.append(st["primary_rays"] + st["shadow_rays"])
+        exps.append(st["expansions"])
+        exps_p.append(st["expansions_primary"])
+        leafhits.append(int(np.count_nonzero(fl & q.PX_LEAF)))
+    return rays, exps, exps_p, leafhits
+
+
+def uhd_leg(args, q, ctx, sc, mask, copy_gbps):
+    """The same scene and lights at 3840x2160: K frames of a seeded path."""
+    W, H, K = 3840, 2160, args.uhd_frames
+    lights = sc.lights
+    pos, ang = sc.poses(K + 1, seed=args.pose_seed + 2160)
+    views = [q.view(pos[i], ang[i], W, H, sc.basecube, sc.depth) for i in range(K + 1)]
+    rays, exps, exps_p, leafhits = count_frames(q, ctx, views[1:], lights, mask)
+    ctx.render(views[0], lights, mask)  # warmup
+    ctx.synchronize()
+    ctx.profile_begin()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for v in views[1:]:
+        ctx.render(v, lights, mask)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_end()
+    roof = roofline(args, prof, K, W, H, len(lights), exps, exps_p, leafhits, copy_gbps, with_traffic=False)
+    value = float(sum(rays)) / wall / 1e6
+    log(f"3840x2160: {value:.1f} Mrays/s, {wall * 1e3 / K:.2f} ms/frame, {roof['kernel']} at "
+        f"{roof['frac']:.4f} of HBM peak")
+    return {"value": round(value, 3), "unit": "Mrays/s", "ms_per_step": round(wall * 1e3 / K, 4), "steps": K,
+            "rays_per_frame": round(float(np.mean(rays))), "pose_seed": args.pose_seed + 2160,
+            "roofline": {k: roof[k] for k in ("kernel", "achieved", "frac", "kernel_avg_ms", "frame")}}
+
+
+BATCH_RES = ((320, 240), (640, 480), (1280, 720), (1920, 1080), (3840, 2160))
+
+
+def batch_leg(args, q, ctx, nthreads, mask):
+    """Batched views: the 90M-point scene with 4 lights, N views of
+    resolutions drawn from BATCH_RES at random poses, rendered as one batch
+    (one persistent launch per stage, hipGraph-captured and replayed), with
+    the tuning the autotuner picks on the first 8 views; two views checked
+    bit for bit against their single-view renders."""
+    sc = q.Scene(4, seed=args.seed)
+    lights = sc.lights
+    t0 = time.time()
+    pnt, nrm, col = sc.points(nthreads=nthreads)
+    ctx.upload_points(pnt)
+    ctx.upload_model(col, nrm)
+    nnodes = ctx.build_octree(sc.basecube, sc.depth, sc.npoints)
+    ctx.synchronize()
+    del pnt, nrm, col
+    log(f"batch leg: config 4 scene ({sc.npoints} points, {nnodes} nodes, {len(lights)} lights) in "
+        f"{time.time() - t0:.2f}s")
+    n = args.batch_views
+    rng = np.random.default_rng(args.pose_seed + 4)
+    sizes = [BATCH_RES[i] for i in rng.integers(0, len(BATCH_RES), n)]
+    pos, ang = sc.poses(n, seed=args.pose_seed + 4)
+    views = [q.view(pos[i], ang[i], w, h, sc.basecube, sc.depth) for i, (w, h) in enumerate(sizes)]
+    t0 = time.time()
+    best, table = ctx.autotune(views[:8], lights, mask, reps=1)
+    tune_s = time.time() - t0
+    ctx.set_tuning(graph=1)
+    log(f"batch leg: autotuned on 8 views in {tune_s:.1f}s ({len(table)} candidates): block "
+        f"{best['block_threads']}, refill {best['refill']}, stack {best['stack']}, reorder {best['reorder']}")
+    ctx.render(views, lights, mask)  # capture
+    st = ctx.stats()
+    rays = st["primary_rays"] + st["shadow_rays"]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.batch_reps):
+        ctx.render(views, lights, mask)
+    ctx.synchronize()
+    wall = (time.perf_counter() - t0) / args.batch_reps
+    # two views against their single-view renders, bit for bit
+    order = np.argsort([w * h for w, h in sizes])
+    check = [int(order[0]), int(order[n // 2])]
+    batch_frames = [ctx.read_frame(i) for i in check]
+    same = True
+    for i, bf in zip(check, batch_frames):
+        ctx.render(views[i], lights, mask)
+        one = ctx.read_frame(0)
+        same = same and all(np.array_equal(bf[k], one[k]) for k in bf)
+    value = rays / wall / 1e6
+    log(f"batch leg: {n} views, {sum(w * h for w, h in sizes) / 1e6:.1f} M pixels, {rays / 1e6:.1f} M rays per "
+        f"batch in {wall * 1e3:.1f} ms = {value:.1f} Mrays/s; bit-exact vs single views: {same}")
+    if not same:
+        raise SystemExit("batched frames differ from single-view frames")
+    return {"value": round(value, 3), "unit": "Mrays/s", "ms_per_batch": round(wall * 1e3, 3),
+            "reps": args.batch_reps, "views": n, "pixels": int(sum(w * h for w, h in sizes)),
+            "rays_per_batch": int(rays), "lights": int(len(lights)), "points": sc.npoints, "nodes": nnodes,
+            "resolutions": {f"{w}x{h}": int(sum(1 for s_ in sizes if s_ == (w, h))) for w, h in BATCH_RES},
+            "tuning": best, "autotune_candidates": len(table), "autotune_s": round(tune_s, 2),
+            "graph": True, "bit_exact_vs_single_view": same, "checked_views": check,
+            "pose_seed": args.pose_seed + 4}
+
+
 def main():
     args = parse()
     if args.summarize_pmc is not None:
@@ -307,16 +420,8 @@ def main():
     mask = q.OUT_DEFAULT
 
     # untimed counting pass: per-frame rays, expansions and primary leaf hits
-    rays, exps, exps_p, leafhits = [], [], [], []
-    for i in range(Wm, K + Wm):
-        ctx.render(views[i], lights, mask | q.OUT_FLAGS | q.RENDER_COUNT)
-        st = ctx.stats()
-        fl = ctx.read_frame(0, fields=("flags",))["flags"]
-        rays.append(st["primary_rays"] + st["shadow_rays"])
-        exps.append(st["expansions"])
-        exps_p.append(st["expansions_primary"])
-        tuning["compact_layout_used"] = bool(st["compact"])
-        leafhits.append(int(np.count_nonzero(fl & q.PX_LEAF)))
+    rays, exps, exps_p, leafhits = count_frames(q, ctx, views[Wm:K + Wm], lights, mask)
+    tuning["compact_layout_used"] = bool(ctx.stats()["compact"])
     log(f"counting pass: {np.mean(rays) / 1e6:.3f} Mrays/frame, {np.mean(exps) / 1e6:.2f} M expansions/frame "
         f"({np.mean(exps_p) / 1e6:.2f} M primary)")
 
@@ -377,6 +482,9 @@ def main():
                          f"C oracle -O2 -ffp-contract=off, pthreads",
                "seconds": round(tc, 2), "bit_exact_vs_gpu": same}
 
+    uhd = uhd_leg(args, q, ctx, sc, mask, copy_gbps) if args.uhd_frames > 0 else None
+    batch = batch_leg(args, q, ctx, nthreads, mask) if args.batch_views > 0 else None  # replaces the scene
+
     res = {
         "metric": "primary+shadow Mrays/s @1080p, 90M-pt depth-12 octree; surrogate vs exact",
         "value": round(value, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": K, "warmup": Wm,
@@ -390,6 +498,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "surrogate": sur,
+        "uhd_3840x2160": uhd,
+        "batch_views": batch,
         "tuning": tuning,
         "build_s": round(t_build, 3),
         "total_s": round(time.time() - t_all, 1),

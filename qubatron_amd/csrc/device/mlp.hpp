@@ -37,15 +37,11 @@ constexpr int kP1 = 0, kP2 = kP1 + 64 * 48, kP3 = kP2 + 64 * 64, kP4 = kP3 + 64 
 
 typedef short  bf16x4 __attribute__((ext_vector_type(4)));
 typedef float  f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bfv4 __attribute__((ext_vector_type(4)));
 
-/* fp32 -> bf16, round to nearest even (NaN quieted) */
-__device__ __forceinline__ uint16_t f2bf(float x)
-{
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t) ((u >> 16) | ((u & 0x007FFFFFu) ? 0x40u : 0u));
-    u += 0x7FFFu + ((u >> 16) & 1u);
-    return (uint16_t) (u >> 16);
-}
+/* fp32 -> bf16, round to nearest even (NaN quieted): gfx950's
+ * v_cvt_pk_bf16_f32 */
+__device__ __forceinline__ uint16_t f2bf(float x) { return __builtin_bit_cast(uint16_t, (__bf16) x); }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t) b << 16); }
 
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
@@ -55,12 +51,8 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c)
 
 __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
 {
-    bf16x4 r;
-    r[0] = (short) f2bf(a);
-    r[1] = (short) f2bf(b);
-    r[2] = (short) f2bf(c);
-    r[3] = (short) f2bf(d);
-    return r;
+    const f32x4 v = {a, b, c, d};
+    return __builtin_bit_cast(bf16x4, __builtin_convertvector(v, bfv4)); /* two v_cvt_pk_bf16_f32 */
 }
 
 /* Feature f (0..47) of one query: f = 6k + 2a + {0 sin, 1 cos} for band k
