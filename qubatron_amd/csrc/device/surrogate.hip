@@ -390,25 +390,35 @@ __device__ __forceinline__ float mlp_logit(const MlpFrags& F, const float* __res
     return acc[0] + Bs[kLB4]; /* D[0][query]: lanes 0..15, element 0 */
 }
 
-/* Logits of the 64 queries of a wave, one per lane: four fragment groups of
- * 16, each lane's inputs handed to its group's fragment lanes by shuffles. */
+/* Logit of query 16 j + (lane % 16) of the wave's 64 (fragment group j):
+ * the group's inputs are handed to the fragment lanes by shuffles, and the
+ * result is returned in every lane (it is the one for this lane in the lanes
+ * of group j). */
+__device__ __forceinline__ float mlp_group(const MlpFrags& F, const float* __restrict__ Bs, const float u[3],
+                                           const float nrm[3], const float lu[3], int lane, int j)
+{
+    const int r = lane & 15, g = lane >> 4;
+    const int src = 16 * j + r;
+    float     uj[3], nj[3], lj[3];
+    for (int a = 0; a < 3; a++) {
+        uj[a] = __shfl(u[a], src, 64);
+        nj[a] = __shfl(nrm[a], src, 64);
+        lj[a] = __shfl(lu[a], src, 64);
+    }
+    bf16x4 xin[3];
+    query_frag(uj, nj, lj, g, xin);
+    const float lg = mlp_logit(F, Bs, xin, lane);
+    return __shfl(lg, r, 64); /* query 16 j + r's logit sits in lane r */
+}
+
+/* Logits of the 64 queries of a wave, one per lane: four fragment groups of 16. */
 __device__ __forceinline__ float mlp_logit64(const MlpFrags& F, const float* __restrict__ Bs, const float u[3],
                                              const float nrm[3], const float lu[3], int lane)
 {
-    const int r = lane & 15, g = lane >> 4;
+    const int g   = lane >> 4;
     float     res = 0.0f;
     for (int j = 0; j < 4; j++) {
-        const int src = 16 * j + r;
-        float     uj[3], nj[3], lj[3];
-        for (int a = 0; a < 3; a++) {
-            uj[a] = __shfl(u[a], src, 64);
-            nj[a] = __shfl(nrm[a], src, 64);
-            lj[a] = __shfl(lu[a], src, 64);
-        }
-        bf16x4 xin[3];
-        query_frag(uj, nj, lj, g, xin);
-        const float lg  = mlp_logit(F, Bs, xin, lane);
-        const float got = __shfl(lg, r, 64); /* query 16 j + r's logit sits in lane r */
+        const float got = mlp_group(F, Bs, u, nrm, lu, lane, j);
         if (g == j) res = got;
     }
     return res;
@@ -482,29 +492,71 @@ __global__ void __launch_bounds__(256) k_mlp_shadow(InferArgs ia, SceneDev s, Li
     if ((uint32_t) wave * 64u >= total) return; /* uniform over the wave; no barrier follows */
     MlpFrags F;
     load_frags(ia.W, lane, F);
-    for (uint32_t q0 = (uint32_t) wave * 64u; q0 < total; q0 += (uint32_t) nw * 64u) {
-        const uint32_t qi    = q0 + (uint32_t) lane;
-        const bool     valid = qi < total;
-        int            l     = 0;
-        uint32_t       gp    = 0;
-        float          u[3] = {0.f, 0.f, 0.f}, nrm[3] = {0.f, 0.f, 0.f}, lu[3] = {0.f, 0.f, 0.f};
-        if (valid) {
-            l                  = (int) (qi / nq);
-            gp                 = lists[(size_t) l * lstride + (qi - (uint32_t) l * nq)];
-            const float4 ip    = isp[gp];
-            const float  p3[3] = {ip.x, ip.y, ip.z};
-            mlp_scale(p3, ia.bx, ia.by, ia.bz, ia.bw, u);
-            float col[4], n4[4];
-            fetch_attr(s.astat, ms[gp], col, n4);
-            if (DUAL && md[gp] > 0) fetch_attr(s.adyn, md[gp], col, n4);
-            nrm[0] = n4[0], nrm[1] = n4[1], nrm[2] = n4[2];
-            mlp_scale(L.p[l], ia.bx, ia.by, ia.bz, ia.bw, lu);
+    /* The inputs of the wave's next 64 queries are gathered while the
+     * current ones are evaluated: list entry before fragment group 0, hit
+     * point and model index before group 1, normal before groups 2-3, so
+     * that each dependent load is covered by a group's MFMA work. */
+    const uint32_t step = (uint32_t) nw * 64u;
+    uint32_t       q0   = (uint32_t) wave * 64u;
+    bool           valid = q0 + (uint32_t) lane < total;
+    int            l     = 0;
+    uint32_t       gp    = 0;
+    float          u[3] = {0.f, 0.f, 0.f}, nrm[3] = {0.f, 0.f, 0.f}, lu[3] = {0.f, 0.f, 0.f};
+    if (valid) {
+        const uint32_t qi = q0 + (uint32_t) lane;
+        l                 = (int) (qi / nq);
+        gp                = lists[(size_t) l * lstride + (qi - (uint32_t) l * nq)];
+        const float4 ip    = isp[gp];
+        const float  p3[3] = {ip.x, ip.y, ip.z};
+        mlp_scale(p3, ia.bx, ia.by, ia.bz, ia.bw, u);
+        float col[4], n4[4];
+        fetch_attr(s.astat, ms[gp], col, n4);
+        if (DUAL && md[gp] > 0) fetch_attr(s.adyn, md[gp], col, n4);
+        nrm[0] = n4[0], nrm[1] = n4[1], nrm[2] = n4[2];
+        mlp_scale(L.p[l], ia.bx, ia.by, ia.bz, ia.bw, lu);
+    }
+    for (; q0 < total; q0 += step) {
+        const int      g      = lane >> 4;
+        const uint32_t qn     = q0 + step + (uint32_t) lane;
+        const bool     nvalid = qn < total;
+        int            nl_    = 0;
+        uint32_t       ngp    = 0;
+        if (nvalid) {
+            nl_ = (int) (qn / nq);
+            ngp = lists[(size_t) nl_ * lstride + (qn - (uint32_t) nl_ * nq)];
         }
-        const float logit = mlp_logit64(F, Bs, u, nrm, lu, lane);
+        float res = mlp_group(F, Bs, u, nrm, lu, lane, 0);
+        float4 nip = make_float4(0.f, 0.f, 0.f, 0.f);
+        int    nms = 0, nmd = 0;
+        if (nvalid) {
+            nip = isp[ngp];
+            nms = ms[ngp];
+            if (DUAL) nmd = md[ngp];
+        }
+        const float r1 = mlp_group(F, Bs, u, nrm, lu, lane, 1);
+        float       ncol[4], n4[4] = {0.f, 0.f, 0.f, 0.f};
+        if (nvalid) {
+            fetch_attr(s.astat, nms, ncol, n4);
+            if (DUAL && nmd > 0) fetch_attr(s.adyn, nmd, ncol, n4);
+        }
+        const float r2 = mlp_group(F, Bs, u, nrm, lu, lane, 2);
+        const float r3 = mlp_group(F, Bs, u, nrm, lu, lane, 3);
+        res = g == 1 ? r1 : res;
+        res = g == 2 ? r2 : res;
+        res = g == 3 ? r3 : res;
         if (valid) {
-            const uint8_t d = logit < lo_logit ? LS_PRED_OCC : logit > hi_logit ? LS_PRED_LIT : LS_UNSURE;
+            const uint8_t d = res < lo_logit ? LS_PRED_OCC : res > hi_logit ? LS_PRED_LIT : LS_UNSURE;
             lstat[(size_t) l * npix + gp] = d;
             n_pred += d != LS_UNSURE ? 1u : 0u;
+        }
+        valid = nvalid;
+        l     = nl_;
+        gp    = ngp;
+        if (nvalid) {
+            const float p3[3] = {nip.x, nip.y, nip.z};
+            mlp_scale(p3, ia.bx, ia.by, ia.bz, ia.bw, u);
+            nrm[0] = n4[0], nrm[1] = n4[1], nrm[2] = n4[2];
+            mlp_scale(L.p[nl_], ia.bx, ia.by, ia.bz, ia.bw, lu);
         }
     }
     n_pred = wave_sum(n_pred);
