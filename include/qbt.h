@@ -14,6 +14,8 @@
  *   octree_trace_line (octree.c:341-537)     qbt_trace_rays (batched, on GPU)
  *   octree_insert_point loop (modelutil.c:203-217)
  *                                            qbt_build_octree (on GPU)
+ *   particle_glc_update (particle_glc.c:118-156)
+ *                                            qbt_particle_update
  *
  * Plain pointers and sizes only. All entry points return QBT_OK (0) or a
  * negative QBT_E* code (qbt_host.h); qbt_strerror names it. A context owns one
@@ -240,7 +242,7 @@ typedef struct qbt_tune_result {
 int qbt_autotune(qbt_ctx* ctx, const qbt_view* views, int nviews, const float* lights, int nlights,
                  unsigned mask, int reps, qbt_tune_result* results, int max_results, int* nresults);
 
-/* Diagnostic: copy `bytes`/* Diagnostic: copy `bytes` (a multiple of 16) between two scratch buffers
+/* Diagnostic: copy `bytes` (a multiple of 16) between two scratch buffers
  * with a plain streaming kernel (k_copy16) `reps` times and report the
  * average launch time and (read + write) GB/s. Used to calibrate hardware
  * byte counters and as the achievable-bandwidth reference. */
@@ -248,13 +250,29 @@ int qbt_copy_bandwidth(qbt_ctx* ctx, size_t bytes, int reps, double* avg_ms, dou
 
 /* Batched octree_trace_line: n rays (origins, dirs: n x 3 floats, host).
  * semantics 0 = shader (octree_fsh.c cube_trace_line, dual tree), 1 = CPU
- * twin (octree.c:341-537: static tree only, its plane-test returns). Outputs
+ * twin (octree.c:341-537: static tree only, its plane-test returns), 2 = the
+ * particle shader's copy (particle_vsh.c:121-343: static tree only, (0,0,0,0)
+ * plane-test returns, its slot address, see qbt_particle_update). Outputs
  * (host, any may be NULL): status (1 hit, 0 miss, -1 no entry),
  * model (slot 8 of the static leaf, -1 none), leaf node, isp (4 floats),
  * tlf (leaf cube, 4 floats). basecube/maxlevel as in qbt_view. */
 int qbt_trace_rays(qbt_ctx* ctx, const float* origins, const float* dirs, size_t n, int semantics,
                    const float basecube[4], int maxlevel, int32_t* status, int32_t* model, int32_t* leaf,
                    float* isp, float* tlf);
+
+/* The reference's particle pass: particle_glc_alloc_in (particle_glc.c:158-167)
+ * + particle_glc_update (:118-156) + the read-back into pos_out/spd_out
+ * (:147-150). n particles, position and velocity (n x 3 floats each, host)
+ * in, the updated ones out (host; may alias the inputs). Each particle runs
+ * the particle shader's main (shaders/particle_vsh.c:345-373): a particle
+ * with spd.x > -90000 gets spd.y -= 0.4, traces the static octree from pos
+ * along spd with that shader's copy of the traversal (:62-343: one tree,
+ * (0,0,0,0) plane-test defaults, its slot address, see trace.hpp), stops at
+ * the leaf corner if it hit a leaf closer than 10 units (spd.x = -100000),
+ * else moves by spd and stops below y = -10. basecube = (0, basesize,
+ * basesize, basesize) as particle_glc.c:130. */
+int qbt_particle_update(qbt_ctx* ctx, const float* pos, const float* spd, size_t n, int maxlevel, float basesize,
+                        float* pos_out, float* spd_out);
 
 /* Build the static or dynamic octree (tree = QBT_BUFFER_STATIC_OCTREE or
  * QBT_BUFFER_DYNAMIC_OCTREE) on the GPU from the first npoints points of the

@@ -12,7 +12,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 
-SEM_SHADER, SEM_CPU = 0, 1
+SEM_SHADER, SEM_CPU, SEM_PARTICLE = 0, 1, 2
 HIT, MISS, DISCARD, ERR = 1, 0, -1, -2
 PX_DISCARD, PX_LEAF, PX_SHADED, PX_SPRITE = 1, 2, 4, 8
 MLP_IN, MLP_H = 42, 64
@@ -60,6 +60,7 @@ def lib():
             raise FileNotFoundError(f"{LIB} not built (make -C oracle)")
         L = C.CDLL(LIB)
         L.orc_trace.argtypes = [C.POINTER(SceneC), f32p, f32p, C.c_int, C.POINTER(TraceOut)]
+        L.orc_particle_step.argtypes = [C.POINTER(SceneC), f32p, f32p, f32p, f32p]
         L.orc_render.argtypes = [C.POINTER(SceneC), C.POINTER(ViewC), C.c_int, f32p, C.POINTER(FrameC), C.c_int,
                                  C.c_int, C.c_int]
         L.orc_camera.argtypes = [f32p, f32p, f32p, f32p]
@@ -138,6 +139,18 @@ class OracleScene:
             res["tlf"][i] = out.tlf[:] if hit else (0, 0, 0, 0)
             res["exps"][i] = out.expansions
         return res
+
+    def particle_step(self, pos, spd):
+        """One transform-feedback pass of the particle shader over n particles
+        (particle_vsh.c:345-373): returns (pos_out, spd_out), n x 3 each."""
+        p = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        v = np.ascontiguousarray(spd, dtype=np.float32).reshape(-1, 3)
+        po, vo = np.empty_like(p), np.empty_like(v)
+        L = lib()
+        for i in range(p.shape[0]):
+            if L.orc_particle_step(C.byref(self.c), _f(p[i]), _f(v[i]), _f(po[i]), _f(vo[i])) != 0:
+                raise RuntimeError("orc_particle_step: node index out of range")
+        return po, vo
 
     def render(self, camfp, angle, W, H, lights, shoot=0, rows=None, nthreads=8):
         v = ViewC()

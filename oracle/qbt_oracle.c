@@ -30,10 +30,18 @@ static const int   VERPAIRS[8] = {2, 3, 0, 1, 6, 7, 4, 5};
 static const int   DEPPAIRS[8] = {4, 5, 6, 7, 0, 1, 2, 3};
 
 /* is_cube_xplane/yplane/zplane: octree_fsh.c:62-99 (shader: all FLT_MAX when
- * the direction component is 0) and octree.c:302-339 (CPU: (0,0,0,FLT_MAX)). */
+ * the direction component is 0), octree.c:302-339 (CPU: (0,0,0,FLT_MAX)) and
+ * particle_vsh.c:62-99 (particle: (0,0,0,0)). */
+static v4 plane_default(int sem)
+{
+    if (sem == ORC_SEM_CPU) return (v4){0.0f, 0.0f, 0.0f, FLT_MAX};
+    if (sem == ORC_SEM_PARTICLE) return (v4){0.0f, 0.0f, 0.0f, 0.0f};
+    return (v4){FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
+}
+
 static v4 plane_x(float c, const float* p, const float* d, int cpu)
 {
-    v4 r = cpu ? (v4){0.0f, 0.0f, 0.0f, FLT_MAX} : (v4){FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
+    v4 r = plane_default(cpu);
     if (d[0] != 0.0f) {
         r.w = (c - p[0]) / d[0];
         r.y = p[1] + d[1] * r.w;
@@ -45,7 +53,7 @@ static v4 plane_x(float c, const float* p, const float* d, int cpu)
 
 static v4 plane_y(float c, const float* p, const float* d, int cpu)
 {
-    v4 r = cpu ? (v4){0.0f, 0.0f, 0.0f, FLT_MAX} : (v4){FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
+    v4 r = plane_default(cpu);
     if (d[1] != 0.0f) {
         r.w = (c - p[1]) / d[1];
         r.x = p[0] + d[0] * r.w;
@@ -57,7 +65,7 @@ static v4 plane_y(float c, const float* p, const float* d, int cpu)
 
 static v4 plane_z(float c, const float* p, const float* d, int cpu)
 {
-    v4 r = cpu ? (v4){0.0f, 0.0f, 0.0f, FLT_MAX} : (v4){FLT_MAX, FLT_MAX, FLT_MAX, FLT_MAX};
+    v4 r = plane_default(cpu);
     if (d[2] != 0.0f) {
         r.w = (c - p[2]) / d[2];
         r.x = p[0] + d[0] * r.w;
@@ -79,6 +87,20 @@ static int32_t slot(const orc_tree* t, int k, int32_t i, int level, int* err)
         return 0;
     }
     return t->octs[(size_t) i * 12 + (size_t) k];
+}
+
+/* The particle shader's oct_from_octets_for_index (particle_vsh.c:110-119):
+ * texel column 3 i mod 8192 + k / 4 on row 3 i / 8192, without carrying into
+ * the next row. A column past 8191 is outside the 8192-wide texture; GLSL ES
+ * leaves such a fetch undefined, and it is defined here as 0 (robust access). */
+static int32_t slot_particle(const orc_tree* t, int k, int32_t i, int level, int* err)
+{
+    if (i == 0 && level > 0) return 0;
+    const int64_t e  = (int64_t) i * 3;
+    const int64_t cy = e / 8192;
+    const int64_t cx = e - cy * 8192 + k / 4;
+    if (cx >= 8192) return 0;
+    return slot(t, k, i, level, err);
 }
 
 static void fetch_attr(const float* arr, size_t n, int32_t m, float out[4])
@@ -104,7 +126,10 @@ typedef struct level_rec {
 
 int orc_trace(const orc_scene* sc, const float pos[3], const float dir[3], int semantics, orc_trace_out* out)
 {
-    const int cpu = semantics == ORC_SEM_CPU;
+    const int cpu = semantics; /* plane-test defaults, see plane_default */
+    const int one = semantics != ORC_SEM_SHADER; /* a single tree */
+    const int prt = semantics == ORC_SEM_PARTICLE;
+#define SLOT(t, k, i, lv, e) (prt ? slot_particle(t, k, i, lv, e) : slot(t, k, i, lv, e))
     memset(out, 0, sizeof(*out));
     out->status = ORC_MISS;
     out->socti = out->docti = -1;
@@ -113,7 +138,7 @@ int orc_trace(const orc_scene* sc, const float pos[3], const float dir[3], int s
         out->status = ORC_ERR;
         return ORC_ERR;
     }
-    const orc_tree* dyn   = cpu ? &(orc_tree){NULL, 0} : &sc->dyn; /* CPU twin: one tree */
+    const orc_tree* dyn   = one ? &(orc_tree){NULL, 0} : &sc->dyn; /* CPU twin, particles: one tree */
     int             err   = 0;
     int             level = 0;
     level_rec       st[ORC_MAXLEVEL + 1];
@@ -133,7 +158,7 @@ int orc_trace(const orc_scene* sc, const float pos[3], const float dir[3], int s
 
     /* root faces in the reference order (octree_fsh.c:164-192); the CPU twin
      * also requires act.w < FLT_MAX (octree.c:359-386) */
-#define ROOT_OK(a) (!cpu || (a).w < FLT_MAX)
+#define ROOT_OK(a) (semantics != ORC_SEM_CPU || (a).w < FLT_MAX)
     act = plane_z(tlf.z, pos, dir, cpu); /* front */
     if (ROOT_OK(act) && tlf.x < act.x && act.x <= brb.x && tlf.y > act.y && act.y >= brb.y) hitp[hitc++] = act;
     act = plane_z(brb.z, pos, dir, cpu); /* back */
@@ -172,7 +197,7 @@ int orc_trace(const orc_scene* sc, const float pos[3], const float dir[3], int s
             memcpy(out->tlf, &tlf, sizeof(tlf));
             out->socti = st[level].socti;
             out->docti = st[level].docti;
-            out->ms    = slot(&sc->stat, 8, st[level].socti, level, &err);
+            out->ms    = SLOT(&sc->stat, 8, st[level].socti, level, &err);
             out->md    = slot(dyn, 8, st[level].docti, level, &err);
             fetch_attr(sc->col_s, sc->npts_s, out->ms, out->col);
             fetch_attr(sc->nrm_s, sc->npts_s, out->ms, out->nrm);
@@ -231,7 +256,7 @@ int orc_trace(const orc_scene* sc, const float pos[3], const float dir[3], int s
                 }
                 pre = oct;
 
-                int32_t s = slot(&sc->stat, oct, st[level].socti, level, &err);
+                int32_t s = SLOT(&sc->stat, oct, st[level].socti, level, &err);
                 int32_t d = slot(dyn, oct, st[level].docti, level, &err);
                 if (s > 0 || d > 0) {
                     int ind               = st[level].ispsi;
@@ -261,7 +286,7 @@ int orc_trace(const orc_scene* sc, const float pos[3], const float dir[3], int s
             nxt_ind++;
             cur_len--;
             st[level].ispsi = 128 | (nxt_ind << 4) | cur_len;
-            int32_t s = slot(&sc->stat, nxt_oct, st[level].socti, level, &err);
+            int32_t s = SLOT(&sc->stat, nxt_oct, st[level].socti, level, &err);
             int32_t d = slot(dyn, nxt_oct, st[level].docti, level, &err);
             if (err) {
                 out->status = ORC_ERR;
@@ -281,6 +306,29 @@ int orc_trace(const orc_scene* sc, const float pos[3], const float dir[3], int s
             }
         }
     }
+#undef SLOT
+}
+
+/* particle_vsh.c:345-373 */
+int orc_particle_step(const orc_scene* sc, const float pos[3], const float spd[3], float pos_out[3],
+                      float spd_out[3])
+{
+    for (int a = 0; a < 3; a++) pos_out[a] = pos[a], spd_out[a] = spd[a];
+    if (!(spd_out[0] > -90000.0f)) return 0;
+    spd_out[1] -= 0.4f;
+    orc_trace_out r;
+    if (orc_trace(sc, pos, spd_out, ORC_SEM_PARTICLE, &r) == ORC_ERR) return ORC_ERR;
+    if (r.status == ORC_HIT && r.isp[3] > 0.0f) { /* res.isp.w > 0 (a miss leaves it 0) */
+        const float dx = r.tlf[0] - pos[0], dy = r.tlf[1] - pos[1], dz = r.tlf[2] - pos[2];
+        if (sqrtf(dx * dx + dy * dy + dz * dz) < 10.0f) {
+            pos_out[0] = r.tlf[0], pos_out[1] = r.tlf[1], pos_out[2] = r.tlf[2];
+            spd_out[0] = -100000.0f;
+            return 0;
+        }
+    }
+    for (int a = 0; a < 3; a++) pos_out[a] = pos[a] + spd_out[a];
+    if (pos_out[1] < -10.0f) spd_out[0] = -100000.0f;
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -10,7 +10,11 @@
  *                  :127-136 (slot addressing), :138-379 (cube_trace_line,
  *                  static + dynamic tree in lockstep);
  *                  with ORC_SEM_CPU: src/qubatron/octree.c:293-537
- *                  (octree_trace_line, single tree, CPU plane-test returns).
+ *                  (octree_trace_line, single tree, CPU plane-test returns);
+ *                  with ORC_SEM_PARTICLE: src/qubatron/shaders/
+ *                  particle_vsh.c:62-343 (single tree, (0,0,0,0) plane-test
+ *                  returns, its slot address :110-119, return for discard).
+ *   orc_particle_step  particle_vsh.c:345-373 (gravity step, trace, stop).
  *   orc_render     src/qubatron/shaders/octree_fsh.c:381-464 (camera
  *                  quaternions, primary trace, per-light shadow trace,
  *                  shading, light sprite), pixel centres of
@@ -41,6 +45,7 @@ extern "C" {
 
 #define ORC_SEM_SHADER 0 /* octree_fsh.c semantics (the GPU hot path) */
 #define ORC_SEM_CPU 1    /* octree.c octree_trace_line semantics */
+#define ORC_SEM_PARTICLE 2 /* particle_vsh.c semantics */
 
 #define ORC_HIT 1
 #define ORC_MISS 0
@@ -79,6 +84,12 @@ typedef struct orc_trace_out {
 } orc_trace_out;
 
 int orc_trace(const orc_scene* sc, const float pos[3], const float dir[3], int semantics, orc_trace_out* out);
+
+/* One particle of the particle shader's transform-feedback pass: pos/spd in,
+ * pos_out/spd_out out (particle_vsh.c:345-373). Returns 0, or ORC_ERR if the
+ * trace hit an out-of-range node. length() is sqrtf(dot) with dot as above. */
+int orc_particle_step(const orc_scene* sc, const float pos[3], const float spd[3], float pos_out[3],
+                      float spd_out[3]);
 
 typedef struct orc_view {
     float camfp[3];

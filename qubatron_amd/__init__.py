@@ -454,6 +454,17 @@ class Context:
               "qbt_trace_rays")
         return dict(status=status, model=model, leaf=leaf, isp=isp, tlf=tlf)
 
+    def particle_update(self, pos, spd, maxlevel, basesize):
+        """The reference's particle pass (``particle_glc_update``,
+        ``src/qubatron/particle_glc.c:118-156``): returns (pos_out, spd_out),
+        n x 3 float32 each."""
+        p = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        v = np.ascontiguousarray(spd, dtype=np.float32).reshape(-1, 3)
+        po, vo = np.empty_like(p), np.empty_like(v)
+        check(self._d.qbt_particle_update(self._h, fptr(p), fptr(v), p.shape[0], int(maxlevel), float(basesize),
+                                          fptr(po), fptr(vo)), "qbt_particle_update")
+        return po, vo
+
 
 class OctreeGLC:
     """Python mirror of ``octree_glc_t`` (``src/qubatron/octree_glc.c:65-85``):
@@ -473,3 +484,31 @@ class OctreeGLC:
 
     def frame(self):
         return self.ctx.read_frame(0)
+
+
+class ParticleGLC:
+    """Python mirror of ``particle_glc_t`` (``src/qubatron/particle_glc.c``):
+    ``alloc_in`` takes the particle positions and velocities, ``alloc_out``
+    sizes the outputs, ``update`` runs the particle pass on the GPU and leaves
+    the results in ``pos_out`` / ``spd_out`` (float32, n x 3), as the
+    reference's read-back does (``:147-150``)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.pos_in = self.spd_in = None
+        self.pos_out = self.spd_out = None
+
+    def alloc_in(self, posdata, spddata, size):
+        n = size // 12
+        self.pos_in = np.ascontiguousarray(posdata, dtype=np.float32).reshape(-1, 3)[:n]
+        self.spd_in = np.ascontiguousarray(spddata, dtype=np.float32).reshape(-1, 3)[:n]
+
+    def alloc_out(self, size):
+        n = size // 12
+        self.pos_out = np.zeros((n, 3), np.float32)
+        self.spd_out = np.zeros((n, 3), np.float32)
+
+    def update(self, model_count, maxlevel, basesize):
+        p, v = self.ctx.particle_update(self.pos_in[:model_count], self.spd_in[:model_count], maxlevel, basesize)
+        self.pos_out[:model_count] = p
+        self.spd_out[:model_count] = v

@@ -277,8 +277,8 @@ __global__ void __launch_bounds__(256) k_render(SceneDev s, ViewDev v, const int
     }
 }
 
-/* batched octree_trace_line */
-template <bool CMP, bool DUAL, bool CPU, bool GRID>
+/* batched octree_trace_line (SEM: kSemShader, kSemCpu or kSemParticle) */
+template <bool CMP, bool DUAL, int SEM, bool GRID>
 __global__ void __launch_bounds__(256) k_trace_rays(SceneDev s, const float* __restrict__ org,
                                                      const float* __restrict__ dir, size_t n, int* __restrict__ status,
                                                      int* __restrict__ model, int* __restrict__ leaf,
@@ -292,13 +292,14 @@ __global__ void __launch_bounds__(256) k_trace_rays(SceneDev s, const float* __r
     if (i < n) {
         Ray r = {org[3 * i], org[3 * i + 1], org[3 * i + 2], dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]};
         Hit h;
-        int stt = trace<CMP, DUAL, CPU, GRID>(s, r, st, h);
+        int stt = trace<CMP, DUAL, SEM, GRID>(s, r, st, h);
         if (stt == ST_ABORT) ab = 1;
         int    m = -1, lf = -1;
         float4 ip = make_float4(0.f, 0.f, 0.f, 0.f), tc = ip;
         if (stt == ST_HIT) {
             const int L = s.maxlevel;
             m           = (h.ns == 0 && L > 0) ? 0 : s.stat.model[h.ns];
+            if (SEM == kSemParticle && particle_model_lost(h.ns)) m = 0; /* its slot-8 address */
             lf          = CMP ? s.stat.ref[h.ns] : h.ns;
             ip          = h.isp;
             float x, y, z, w;
@@ -310,6 +311,55 @@ __global__ void __launch_bounds__(256) k_trace_rays(SceneDev s, const float* __r
         if (leaf) leaf[i] = lf;
         if (isp) isp[i] = ip;
         if (tlf) tlf[i] = tc;
+    }
+    ab = wave_sum(ab);
+    if ((threadIdx.x & 63) == 0 && ab) atomicAdd(&counters[CNT_ABORT], (unsigned long long) ab);
+}
+
+/* The particle shader's main (particle_vsh.c:345-373), one particle per
+ * thread: gravity step, a trace of the static tree (direct layout: the
+ * shader's slot address is a function of the reference node index) with
+ * that shader's semantics, stop at the leaf corner when it is closer than
+ * 10 units, else move. pos/spd may alias pos_out/spd_out. */
+template <bool GRID>
+__global__ void __launch_bounds__(256) k_particles(SceneDev s, const float* pos, const float* spd, size_t n,
+                                                   float* pos_out, float* spd_out,
+                                                   unsigned long long* __restrict__ counters)
+{
+    extern __shared__ uint32_t lds[];
+    const Stack  st = make_stack(lds, s.maxlevel, false);
+    const size_t i  = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned     ab = 0;
+    if (i < n) {
+        const float p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+        float       v[3] = {spd[3 * i], spd[3 * i + 1], spd[3 * i + 2]};
+        float       o[3] = {p[0], p[1], p[2]};
+        if (v[0] > -90000.0f) {
+            v[1] -= 0.4f;
+            Ray r = {p[0], p[1], p[2], v[0], v[1], v[2]};
+            Hit h;
+            const int stt  = trace<false, false, kSemParticle, GRID>(s, r, st, h);
+            bool      stop = false;
+            if (stt == ST_ABORT) ab = 1;
+            if (stt == ST_HIT && h.isp.w > 0.0f) {
+                float x, y, z, w;
+                cube_at<GRID>(s, s.maxlevel, h.ix, h.iy, h.iz, x, y, z, w);
+                const float dx = x - p[0], dy = y - p[1], dz = z - p[2];
+                if (sqrtf(dx * dx + dy * dy + dz * dz) < 10.0f) {
+                    o[0] = x, o[1] = y, o[2] = z;
+                    v[0] = -100000.0f;
+                    stop = true;
+                }
+            }
+            if (!stop) {
+                o[0] = p[0] + v[0], o[1] = p[1] + v[1], o[2] = p[2] + v[2];
+                if (o[1] < -10.0f) v[0] = -100000.0f;
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            pos_out[3 * i + a] = o[a];
+            spd_out[3 * i + a] = v[a];
+        }
     }
     ab = wave_sum(ab);
     if ((threadIdx.x & 63) == 0 && ab) atomicAdd(&counters[CNT_ABORT], (unsigned long long) ab);
@@ -1254,15 +1304,16 @@ extern "C" int qbt_trace_rays(qbt_ctx* c, const float* origins, const float* dir
                               const float basecube[4], int maxlevel, int32_t* status, int32_t* model, int32_t* leaf,
                               float* isp, float* tlf)
 {
-    if (!c || !basecube || (n && (!origins || !dirs)) || (semantics != 0 && semantics != 1)) return QBT_EINVAL;
+    if (!c || !basecube || (n && (!origins || !dirs)) || semantics < 0 || semantics > 2) return QBT_EINVAL;
     if (n == 0) return QBT_OK;
     if (n > (size_t) 1 << 31) return QBT_ENOTSUP;
     HIPCHK(hipSetDevice(c->device));
     SceneDev s;
     bool     dual, grid, cmp;
-    int      rc = c->scene_for(basecube, maxlevel, s, dual, grid, cmp, true);
+    /* the particle semantics addresses reference node indices: direct layout */
+    int rc = c->scene_for(basecube, maxlevel, s, dual, grid, cmp, semantics != 2);
     if (rc) return rc;
-    if (semantics == 1) dual = false; /* octree_trace_line walks one tree */
+    if (semantics != 0) dual = false; /* octree_trace_line and the particle copy walk one tree */
     /* ray scratch: org, dir (n x 12 B each), status/model/leaf (n x 4 B each),
      * then isp and tlf (n x 16 B each) at a 16-byte aligned offset */
     const size_t off_isp = ((n * 36 + 15) / 16) * 16;
@@ -1284,13 +1335,19 @@ extern "C" int qbt_trace_rays(qbt_ctx* c, const float* origins, const float* dir
     unsigned     nb  = blocks_for(n, bs);
     unsigned long long* cnt = (unsigned long long*) c->counters.p;
     c->prof_mark_begin(QBT_PROF_PRIMARY);
-    /* (DUAL, CPU) in {(0,0), (1,0), (0,1)}: the CPU twin walks the static tree only */
-    qbt_dispatch4(cmp, dual, semantics == 1, grid, [&](auto C_, auto D_, auto P_, auto G_) {
-        constexpr bool kD = decltype(D_)::value && !decltype(P_)::value;
-        k_trace_rays<decltype(C_)::value, kD, decltype(P_)::value, decltype(G_)::value>
-            <<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
-        return 0;
-    });
+    /* (DUAL, SEM) in {(0,shader), (1,shader), (0,cpu)}: the CPU twin walks the
+     * static tree only; the particle copy: static tree, direct layout */
+    if (semantics == 2) {
+        auto k = grid ? k_trace_rays<false, false, kSemParticle, true> : k_trace_rays<false, false, kSemParticle, false>;
+        k<<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
+    } else {
+        qbt_dispatch4(cmp, dual, semantics == 1, grid, [&](auto C_, auto D_, auto P_, auto G_) {
+            constexpr bool kD = decltype(D_)::value && !decltype(P_)::value;
+            k_trace_rays<decltype(C_)::value, kD, decltype(P_)::value ? kSemCpu : kSemShader, decltype(G_)::value>
+                <<<nb, bs, lds, c->stream>>>(s, dorg, ddir, n, dst, dmod, dlf, disp, dtlf, cnt);
+            return 0;
+        });
+    }
     HIPCHK(hipGetLastError());
     c->prof_mark_end();
     HIPCHK(hipEventRecord(c->ev1, c->stream));
@@ -1304,6 +1361,50 @@ extern "C" int qbt_trace_rays(qbt_ctx* c, const float* origins, const float* dir
     c->last.primary_rays = n;
     c->last.grid_exact   = grid;
     c->last.dual         = dual;
+    c->last.launches     = 1;
+    c->pending_stats     = true;
+    c->stats_has_trace   = false;
+    qbt_stats st;
+    return qbt_get_stats(c, &st);
+}
+
+/* particle_glc_alloc_in + particle_glc_update + read-back (particle_glc.c:118-167) */
+extern "C" int qbt_particle_update(qbt_ctx* c, const float* pos, const float* spd, size_t n, int maxlevel,
+                                   float basesize, float* pos_out, float* spd_out)
+{
+    if (!c || (n && (!pos || !spd || !pos_out || !spd_out))) return QBT_EINVAL;
+    if (n == 0) return QBT_OK;
+    if (n > (size_t) 1 << 31) return QBT_ENOTSUP;
+    HIPCHK(hipSetDevice(c->device));
+    const float basecube[4] = {0.0f, basesize, basesize, basesize}; /* particle_glc.c:130 */
+    SceneDev    s;
+    bool        dual, grid, cmp;
+    int         rc = c->scene_for(basecube, maxlevel, s, dual, grid, cmp, false); /* direct layout */
+    if (rc) return rc;
+    /* particle scratch: pos, spd in place (n x 12 B each) */
+    if ((rc = c->dev_alloc(c->rays, n * 24))) return rc;
+    float* dpos = (float*) c->rays.p;
+    float* dspd = dpos + 3 * n;
+    HIPCHK(hipMemcpyAsync(dpos, pos, n * 12, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(dspd, spd, n * 12, hipMemcpyHostToDevice, c->stream));
+    k_zero_u64<<<1, 64, 0, c->stream>>>((unsigned long long*) c->counters.p, CNT_N);
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    const int    bs  = c->block_threads(maxlevel, false);
+    const size_t lds = qbt_stack_bytes(maxlevel, false, bs);
+    const unsigned nb = blocks_for(n, bs);
+    unsigned long long* cnt = (unsigned long long*) c->counters.p;
+    c->prof_mark_begin(QBT_PROF_PRIMARY);
+    if (grid) k_particles<true><<<nb, bs, lds, c->stream>>>(s, dpos, dspd, n, dpos, dspd, cnt);
+    else k_particles<false><<<nb, bs, lds, c->stream>>>(s, dpos, dspd, n, dpos, dspd, cnt);
+    HIPCHK(hipGetLastError());
+    c->prof_mark_end();
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    HIPCHK(hipMemcpyAsync(pos_out, dpos, n * 12, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(spd_out, dspd, n * 12, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->last              = qbt_stats();
+    c->last.primary_rays = n;
+    c->last.grid_exact   = grid;
     c->last.launches     = 1;
     c->pending_stats     = true;
     c->stats_has_trace   = false;

@@ -1,9 +1,11 @@
 /*
  * trace.hpp - per-ray sparse-octree traversal for gfx950.
  *
- * Semantics: cube_trace_line of src/qubatron/shaders/octree_fsh.c:138-379
- * (static + dynamic tree in lockstep) and, with CPU = true, octree_trace_line
- * of src/qubatron/octree.c:341-537. Results (leaf node, entry point bits,
+ * Semantics (template parameter SEM): cube_trace_line of
+ * src/qubatron/shaders/octree_fsh.c:138-379 (static + dynamic tree in
+ * lockstep), octree_trace_line of src/qubatron/octree.c:341-537, or the
+ * particle shader's copy (src/qubatron/shaders/particle_vsh.c:121-343: one
+ * tree, (0, 0, 0, 0) plane-test defaults, its slot address). Results (leaf node, entry point bits,
  * miss/discard) are identical to the reference algorithm; only the storage
  * differs:
  *
@@ -37,6 +39,13 @@ namespace qbt {
 
 constexpr int kMaxLevel = 24;
 constexpr float kFltMax = 3.402823466e+38f;
+
+/* traversal semantics */
+enum : int {
+    kSemShader   = 0, /* octree_fsh.c */
+    kSemCpu      = 1, /* octree.c octree_trace_line */
+    kSemParticle = 2  /* particle_vsh.c */
+};
 
 /* An empty volatile asm on a value: the compiler must keep what depends on
  * it inside its branch (a rarely taken slow path would otherwise be
@@ -86,7 +95,8 @@ __device__ __forceinline__ float recip_or_nan(float d)
  * is_cube_yplane and is_cube_zplane (octree_fsh.c:62-99 / octree.c:302-339)
  * in one form. t = (c - p_k) / d_k; the other two coordinates are
  * p_j + d_j t, coordinate k is c; d_k == 0 gives the octree.c twin's
- * defaults (0, 0, 0, t = FLT_MAX), and NaN in shader semantics (see below).
+ * defaults (0, 0, 0, t = FLT_MAX), the particle shader's (0, 0, 0, 0)
+ * (particle_vsh.c:64-65), and NaN in shader semantics (see below).
  *
  * The quotient is correctly rounded, as the reference's division, from
  * y = RN(1 / d_k): q = RN(a y), e = a - d_k q (exact with an FMA),
@@ -103,7 +113,7 @@ __device__ __forceinline__ float axis3(int k, float x, float y, float z)
     return k == 0 ? x : (k == 1 ? y : z);
 }
 
-template <bool CPU>
+template <int SEM>
 __device__ __forceinline__ float4 plane_k(int k, float c, const Ray& r)
 {
     const float p  = axis3(k, r.px, r.py, r.pz);
@@ -127,11 +137,11 @@ __device__ __forceinline__ float4 plane_k(int k, float c, const Ray& r)
     o.y = k == 1 ? c : Y;
     o.z = k == 2 ? c : Z;
     o.w = t;
-    if (CPU) { /* the octree.c twin's defaults can pass the containment tests */
+    if (SEM != kSemShader) { /* these defaults can pass the containment tests */
         o.x = ok ? o.x : 0.0f;
         o.y = ok ? o.y : 0.0f;
         o.z = ok ? o.z : 0.0f;
-        o.w = ok ? o.w : kFltMax;
+        o.w = ok ? o.w : (SEM == kSemCpu ? kFltMax : 0.0f);
     }
     /* Shader semantics: where d_k == 0, y is NaN and so are t and the other
      * two coordinates; every caller tests t > 0 and the containment of the
@@ -141,12 +151,12 @@ __device__ __forceinline__ float4 plane_k(int k, float c, const Ray& r)
     return o;
 }
 
-template <bool CPU>
-__device__ __forceinline__ float4 plane_x(float c, const Ray& r) { return plane_k<CPU>(0, c, r); }
-template <bool CPU>
-__device__ __forceinline__ float4 plane_y(float c, const Ray& r) { return plane_k<CPU>(1, c, r); }
-template <bool CPU>
-__device__ __forceinline__ float4 plane_z(float c, const Ray& r) { return plane_k<CPU>(2, c, r); }
+template <int SEM>
+__device__ __forceinline__ float4 plane_x(float c, const Ray& r) { return plane_k<SEM>(0, c, r); }
+template <int SEM>
+__device__ __forceinline__ float4 plane_y(float c, const Ray& r) { return plane_k<SEM>(1, c, r); }
+template <int SEM>
+__device__ __forceinline__ float4 plane_z(float c, const Ray& r) { return plane_k<SEM>(2, c, r); }
 
 struct Hit {
     float4 isp;
@@ -204,14 +214,14 @@ __device__ __forceinline__ void cube_mid(float x, float y, float z, float w, flo
 }
 
 /* Mid-plane hit `src` (1 z, 2 x, 3 y) of the node cube (x, y, z, w). */
-template <bool CPU, bool GRID>
+template <int SEM, bool GRID>
 __device__ __forceinline__ float4 midplane(int src, const Ray& r, float x, float y, float z, float w)
 {
     float bxx, byy, bzz, hx, hy, hz;
     cube_mid<GRID>(x, y, z, w, bxx, byy, bzz, hx, hy, hz);
     const int   k = src == 1 ? 2 : src - 2; /* src 1 z, 2 x, 3 y */
     const float c = src == 1 ? hz : (src == 2 ? hx : hy);
-    return plane_k<CPU>(k, c, r);
+    return plane_k<SEM>(k, c, r);
 }
 
 /* component selects by value (a conditional over lvalues selects an address
@@ -263,14 +273,39 @@ struct Block<true> {
     }
 };
 
+/* The particle shader addresses slot k of node i as texel
+ * (3 i mod 8192 + k / 4, 3 i / 8192) (particle_vsh.c:110-119): the texel
+ * column does not carry into the next row, so for a node whose three texels
+ * straddle a row end the column passes 8191. Such a fetch is outside the
+ * texture, for which GLSL ES leaves the result undefined; here it reads 0,
+ * the value of a robust-access fetch. Slots 0-7 are affected only where
+ * 3 i mod 8192 == 8191 (slots 4-7), the model index (slot 8) also where
+ * 3 i mod 8192 == 8190. */
+__device__ __forceinline__ bool particle_hi_lost(int i) { return ((uint32_t) i * 3u) % 8192u == 8191u; }
+__device__ __forceinline__ bool particle_model_lost(int i) { return ((uint32_t) i * 3u) % 8192u >= 8190u; }
+
+/* the child block of node i as the semantics SEM reads it (direct layout) */
+template <int SEM>
+__device__ __forceinline__ void load_block(Block<false>& b, const TreeDev& t, int i)
+{
+    b.load(t, i);
+    if (SEM == kSemParticle && particle_hi_lost(i)) b.hi = make_int4(0, 0, 0, 0);
+}
+template <int SEM>
+__device__ __forceinline__ void load_block(Block<true>& b, const TreeDev& t, int i)
+{
+    static_assert(SEM != kSemParticle, "the particle semantics addresses reference node indices: direct layout");
+    b.load(t, i);
+}
+
 /* slot k (< 8) of node i at `level` (oct_from_octets_for_index,
  * octree_fsh.c:127-136): 0 for index 0 below the root. */
-template <bool CMP>
+template <bool CMP, int SEM = kSemShader>
 __device__ __forceinline__ int child_slot(const TreeDev& t, int i, int k, int level)
 {
     if (i == 0 && level > 0) return 0;
     Block<CMP> b;
-    b.load(t, i);
+    load_block<SEM>(b, t, i);
     return b.child(k);
 }
 
@@ -298,7 +333,7 @@ struct Stack {
 
 /* Entry point of the node at `level` on the current path (level >= 1 uses
  * the entry popped one level up). */
-template <bool CPU, bool GRID>
+template <int SEM, bool GRID>
 __device__ __forceinline__ float4 entry_of(const SceneDev& s, const Ray& r, const Stack& st, int level, int ix, int iy,
                                         int iz, float4 start)
 {
@@ -310,7 +345,7 @@ __device__ __forceinline__ float4 entry_of(const SceneDev& s, const Ray& r, cons
             int   sh = level - (m - 1);
             float x, y, z, ww;
             cube_at<GRID>(s, m - 1, ix >> sh, iy >> sh, iz >> sh, x, y, z, ww);
-            return midplane<CPU, GRID>(src, r, x, y, z, ww);
+            return midplane<SEM, GRID>(src, r, x, y, z, ww);
         }
     }
     return start;
@@ -331,7 +366,7 @@ enum : int { ST_CONT = 2 };
 
 /* Root entry (octree_fsh.c:157-211). Returns ST_CONT, ST_HIT (maxlevel 0)
  * or ST_DISCARD. */
-template <bool CPU>
+template <int SEM>
 __device__ __forceinline__ int trace_begin(const SceneDev& s, Ray& r, TraceState& t)
 {
     r.ix = recip_or_nan(r.dx);
@@ -344,22 +379,22 @@ __device__ __forceinline__ int trace_begin(const SceneDev& s, Ray& r, TraceState
         const float rx = tx + tw, ry = ty - tw, rz = tz - tw;
         float4      a;
 #define QBT_CONSIDER(cond)                       \
-    if ((!CPU || a.w < kFltMax) && (cond)) {     \
+    if ((SEM != kSemCpu || a.w < kFltMax) && (cond)) { \
         if (hitc == 0) h0 = a;                   \
         else if (hitc == 1) h1 = a;              \
         hitc++;                                  \
     }
-        a = plane_z<CPU>(tz, r);
+        a = plane_z<SEM>(tz, r);
         QBT_CONSIDER(tx < a.x && a.x <= rx && ty > a.y && a.y >= ry)
-        a = plane_z<CPU>(rz, r);
+        a = plane_z<SEM>(rz, r);
         QBT_CONSIDER(tx < a.x && a.x <= rx && ty > a.y && a.y >= ry)
-        a = plane_x<CPU>(tx, r);
+        a = plane_x<SEM>(tx, r);
         QBT_CONSIDER(ty > a.y && a.y >= ry && tz > a.z && a.z >= rz)
-        a = plane_x<CPU>(rx, r);
+        a = plane_x<SEM>(rx, r);
         QBT_CONSIDER(ty > a.y && a.y >= ry && tz > a.z && a.z >= rz)
-        a = plane_y<CPU>(ty, r);
+        a = plane_y<SEM>(ty, r);
         QBT_CONSIDER(tx < a.x && a.x <= rx && tz > a.z && a.z >= rz)
-        a = plane_y<CPU>(ry, r);
+        a = plane_y<SEM>(ry, r);
         QBT_CONSIDER(tx < a.x && a.x <= rx && tz > a.z && a.z >= rz)
 #undef QBT_CONSIDER
     }
@@ -446,7 +481,7 @@ __device__ __forceinline__ int yplane_key(float x, float z, float hx, float hz)
     return o | (3 << 3) | ((o ^ (x == hx ? 1 : 2)) << 5);
 }
 
-template <bool CPU, bool GRID>
+template <int SEM, bool GRID>
 __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, const Stack& st, TraceState& t)
 {
     /* the iteration cap is checked at the end: a lane over it still takes
@@ -462,9 +497,9 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     cube_at<GRID>(s, level, t.ix, t.iy, t.iz, tx, ty, tz, tw);
     cube_mid<GRID>(tx, ty, tz, tw, bxx, byy, bzz, hx, hy, hz);
     const float  qnan = __builtin_nanf("");
-    const float4 az = plane_z<CPU>(hz, r);
-    const float4 ax = plane_x<CPU>(hx, r);
-    const float4 ay = plane_y<CPU>(hy, r);
+    const float4 az = plane_z<SEM>(hz, r);
+    const float4 ax = plane_x<SEM>(hx, r);
+    const float4 ay = plane_y<SEM>(hy, r);
     float        w0 = t.entry.w;
     float        w1 = (az.w > 0.0f && tx < az.x && az.x <= bxx && ty > az.y && az.y >= byy) ? az.w : qnan;
     float        w2 = (ax.w > 0.0f && ty > ax.y && ax.y >= byy && tz > ax.z && ax.z >= bzz) ? ax.w : qnan;
@@ -472,9 +507,9 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     int          k0 = octet_key(t.entry.x, t.entry.y, t.entry.z, hx, hy, hz, 0);
     /* the octree.c twin's defaults for d == 0 (0, 0, 0) can pass the
      * containment tests, so it keeps the general form */
-    int k1 = CPU ? octet_key(az.x, az.y, az.z, hx, hy, hz, 1) : zplane_key(az.x, az.y, hx, hy);
-    int k2 = CPU ? octet_key(ax.x, ax.y, ax.z, hx, hy, hz, 2) : xplane_key(ax.y, ax.z, hy, hz);
-    int k3 = CPU ? octet_key(ay.x, ay.y, ay.z, hx, hy, hz, 3) : yplane_key(ay.x, ay.z, hx, hz);
+    int k1 = SEM != kSemShader ? octet_key(az.x, az.y, az.z, hx, hy, hz, 1) : zplane_key(az.x, az.y, hx, hy);
+    int k2 = SEM != kSemShader ? octet_key(ax.x, ax.y, ax.z, hx, hy, hz, 2) : xplane_key(ax.y, ax.z, hy, hz);
+    int k3 = SEM != kSemShader ? octet_key(ay.x, ay.y, ay.z, hx, hy, hz, 3) : yplane_key(ay.x, ay.z, hx, hz);
     /* Sort network (octree_fsh.c:283, strict: ties keep their order). Slot
      * 0 holds the node's own entry point, which lies before every point of
      * the ray inside the node but for rounding near an edge, so the three
@@ -547,7 +582,7 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
             const int sh = level - j;
             float     x, y, z, ww;
             cube_at<GRID>(s, j, t.ix >> sh, t.iy >> sh, t.iz >> sh, x, y, z, ww);
-            isp = midplane<CPU, GRID>(src, r, x, y, z, ww);
+            isp = midplane<SEM, GRID>(src, r, x, y, z, ww);
         }
     }
     const int sh = level - lv, o = (int) (w & 7u);
@@ -566,10 +601,10 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
  * ancestor. Returns ST_CONT, ST_HIT (t.level == maxlevel; the leaf is
  * t.ns / t.nd with entry point t.entry), ST_MISS or ST_ABORT.
  */
-template <bool CMP, bool DUAL, bool CPU, bool GRID>
+template <bool CMP, bool DUAL, int SEM, bool GRID>
 __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const Stack& st, TraceState& t)
 {
-    if constexpr (CMP && !DUAL) return trace_step_c1<CPU, GRID>(s, r, st, t);
+    if constexpr (CMP && !DUAL) return trace_step_c1<SEM, GRID>(s, r, st, t);
     if (++t.it > s.iter_cap) return ST_ABORT;
     const int level = t.level;
 
@@ -578,7 +613,7 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
     sb.clear();
     db.clear();
     if (t.ns != 0 || level == 0) {
-        sb.load(s.stat, t.ns);
+        load_block<SEM>(sb, s.stat, t.ns);
         t.exps++;
     }
     if (DUAL && (t.nd != 0 || level == 0)) {
@@ -597,17 +632,17 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
         Cand        c0 = {t.entry.x, t.entry.y, t.entry.z, t.entry.w, 0};
         Cand        c1, c2, c3;
         {
-            float4 a = plane_z<CPU>(hz, r);
+            float4 a = plane_z<SEM>(hz, r);
             bool   ok = a.w > 0.0f && tx < a.x && a.x <= bxx && ty > a.y && a.y >= byy;
             c1        = {a.x, a.y, a.z, ok ? a.w : qnan, 1};
         }
         {
-            float4 a = plane_x<CPU>(hx, r);
+            float4 a = plane_x<SEM>(hx, r);
             bool   ok = a.w > 0.0f && ty > a.y && a.y >= byy && tz > a.z && a.z >= bzz;
             c2        = {a.x, a.y, a.z, ok ? a.w : qnan, 2};
         }
         {
-            float4 a = plane_y<CPU>(hy, r);
+            float4 a = plane_y<SEM>(hy, r);
             bool   ok = a.w > 0.0f && tx < a.x && a.x <= bxx && tz > a.z && a.z >= bzz;
             c3        = {a.x, a.y, a.z, ok ? a.w : qnan, 3};
         }
@@ -678,13 +713,13 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
             st.pw[lv * st.stride] = (w & ~(7u << 3)) | ((uint32_t) (k + 1) << 3);
             float4 isp;
             if (src == 0) {
-                isp = entry_of<CPU, GRID>(s, r, st, lv, ix, iy, iz, t.start);
+                isp = entry_of<SEM, GRID>(s, r, st, lv, ix, iy, iz, t.start);
             } else {
                 float x, y, z, ww;
                 cube_at<GRID>(s, lv, ix, iy, iz, x, y, z, ww);
-                isp = midplane<CPU, GRID>(src, r, x, y, z, ww);
+                isp = midplane<SEM, GRID>(src, r, x, y, z, ww);
             }
-            t.ns = child_slot<CMP>(s.stat, st.ns[lv * st.stride], o, lv);
+            t.ns = child_slot<CMP, SEM>(s.stat, st.ns[lv * st.stride], o, lv);
             t.nd = DUAL ? child_slot<CMP>(s.dyn, st.nd[lv * st.stride], o, lv) : 0;
             t.ix    = (ix << 1) | (o & 1);
             t.iy    = (iy << 1) | ((o >> 1) & 1);
@@ -698,16 +733,17 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
 
 /*
  * Trace one ray to the end. Returns ST_HIT / ST_MISS / ST_DISCARD / ST_ABORT.
- * DUAL: the dynamic tree takes part. CPU: octree.c plane-test returns and
- * root acceptance (single tree; callers pass DUAL = false).
+ * DUAL: the dynamic tree takes part. SEM: kSemCpu for octree.c plane-test
+ * returns and root acceptance, kSemParticle for the particle shader's (both
+ * single tree; callers pass DUAL = false).
  */
-template <bool CMP, bool DUAL, bool CPU, bool GRID>
+template <bool CMP, bool DUAL, int SEM, bool GRID>
 __device__ int trace(const SceneDev& s, const Ray& r, const Stack& st, Hit& out)
 {
     TraceState t;
     Ray        rr  = r;
-    int        stt = trace_begin<CPU>(s, rr, t);
-    while (stt == ST_CONT) stt = trace_step<CMP, DUAL, CPU, GRID>(s, rr, st, t);
+    int        stt = trace_begin<SEM>(s, rr, t);
+    while (stt == ST_CONT) stt = trace_step<CMP, DUAL, SEM, GRID>(s, rr, st, t);
     out.exps = t.exps;
     if (stt == ST_HIT) {
         out.isp = t.entry;
