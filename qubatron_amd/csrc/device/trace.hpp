@@ -360,6 +360,7 @@ struct TraceState {
     int      ns, nd; /* current static / dynamic node */
     int      it;
     unsigned exps;
+    uint32_t pend; /* single-tree compact steps: bit l set while level l has pending entries */
 };
 
 enum : int { ST_CONT = 2 };
@@ -407,7 +408,8 @@ __device__ __forceinline__ int trace_begin(const SceneDev& s, Ray& r, TraceState
     t.entry = h0;
     t.level = t.ix = t.iy = t.iz = 0;
     t.ns = t.nd = 0;
-    t.it = 0;
+    t.it   = 0;
+    t.pend = 0;
     return s.maxlevel == 0 ? ST_HIT : ST_CONT;
 }
 
@@ -421,6 +423,9 @@ __device__ __forceinline__ int trace_begin(const SceneDev& s, Ray& r, TraceState
  *            source << 3 (2 bits) | rank << 5 (3 bits), rank = the child's
  *            position among the node's children, so the child is
  *            first + rank.
+ * A per-lane bit mask marks the levels that still hold pending entries, so a
+ * backtrack finds its level with one count-leading-zeros instead of a loop
+ * over the stack.
  * Candidates are sorted as (t, key) pairs; each key carries the candidate's
  * octet and its tie-flipped octet, computed before the sort from the same
  * comparisons (octree_fsh.c:292-311), and its source; the entry point of the
@@ -559,16 +564,19 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     if (n > 0) {
         st.pw[level * st.stride] = word;
         st.ns[level * st.stride] = rec.x | ((uint32_t) (n - 1) << 30);
+        t.pend |= (n > 1 ? 1u : 0u) << level;
     } else {
-        uint32_t an;
-        do {
-            if (--lv < 0) return over ? ST_ABORT : ST_MISS;
-            an = (uint32_t) st.ns[lv * st.stride];
-        } while (!(an >> 30));
+        /* the deepest level with a pending entry, from the per-lane mask
+         * (levels between it and this one have none: no loop over them) */
+        const uint32_t below = t.pend & ((1u << level) - 1u);
+        if (below == 0u) return over ? ST_ABORT : ST_MISS;
+        lv                    = 31 - __clz((int) below);
+        const uint32_t an     = (uint32_t) st.ns[lv * st.stride];
         w                     = st.pw[lv * st.stride] >> 8; /* the next entry becomes the current one */
         st.pw[lv * st.stride] = w;
         st.ns[lv * st.stride] = (int) (an - (1u << 30));
-        a                     = an & 0x3FFFFFFFu;
+        t.pend &= (an >> 30) == 1u ? ~(1u << lv) : ~0u;
+        a = an & 0x3FFFFFFFu;
         /* its entry point: the mid-plane hit of the nearest node on the path
          * (this one or an ancestor) entered through a mid-plane, else the
          * root entry */

@@ -27,3 +27,4 @@ inline float4 make_float4(float x, float y, float z, float w) { return float4{x,
 inline int4   make_int4(int x, int y, int z, int w) { return int4{x, y, z, w}; }
 inline uint2  make_uint2(unsigned x, unsigned y) { return uint2{x, y}; }
 inline int    __popc(unsigned x) { return __builtin_popcount(x); }
+inline int    __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned) x); }
