@@ -442,6 +442,9 @@ __device__ __forceinline__ float4 pick_entry(int src, float4 e, float4 z, float4
     return r;
 }
 
+/* bit o of m (one bit-field extract) */
+__device__ __forceinline__ bool bit_of(uint32_t m, int o) { return __builtin_amdgcn_ubfe(m, (uint32_t) o, 1u) != 0u; }
+
 __device__ __forceinline__ void kswap(float& wa, int& ka, float& wb, int& kb)
 {
     const bool sw = wb < wa; /* strict: ties keep order (octree_fsh.c:283); NaN never moves */
@@ -509,7 +512,11 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     float        w1 = (az.w > 0.0f && tx < az.x && az.x <= bxx && ty > az.y && az.y >= byy) ? az.w : qnan;
     float        w2 = (ax.w > 0.0f && ty > ax.y && ax.y >= byy && tz > ax.z && ax.z >= bzz) ? ax.w : qnan;
     float        w3 = (ay.w > 0.0f && tx < ay.x && ay.x <= bxx && tz > ay.z && ay.z >= bzz) ? ay.w : qnan;
-    int          k0 = octet_key(t.entry.x, t.entry.y, t.entry.z, hx, hy, hz, 0);
+    /* the entry's key without its tie-flipped octet: a key in slot 0 after
+     * the sort never flips (there is no previous octet), and the entry only
+     * leaves slot 0 in the branch below, which completes its key first */
+    const int    oe = (t.entry.x > hx ? 1 : 0) | (t.entry.y < hy ? 2 : 0) | (t.entry.z < hz ? 4 : 0);
+    int          k0 = oe | (oe << 5);
     /* the octree.c twin's defaults for d == 0 (0, 0, 0) can pass the
      * containment tests, so it keeps the general form */
     int k1 = SEM != kSemShader ? octet_key(az.x, az.y, az.z, hx, hy, hz, 1) : zplane_key(az.x, az.y, hx, hy);
@@ -522,6 +529,7 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
      * skips them where no lane of the wave needs them. NaN never moves, so
      * slot 0 stays valid. */
     if (w1 < w0 || w2 < w0 || w3 < w0) {
+        k0 = octet_key(t.entry.x, t.entry.y, t.entry.z, hx, hy, hz, 0);
         kswap(w0, k0, w1, k1);
         kswap(w0, k0, w2, k2);
         kswap(w0, k0, w3, k3);
@@ -545,10 +553,10 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
         const uint32_t rk = (uint32_t) __popc(mask & ((1u << o) - 1u));
         return (uint32_t) o | ((uint32_t) k & 0x18u) | (rk << 5);
     };
-    const bool t0 = (mask >> o0) & 1u;
-    const bool t1 = v1 && ((mask >> o1) & 1u);
-    const bool t2 = v2 && ((mask >> o2) & 1u);
-    const bool t3 = v3 && ((mask >> o3) & 1u);
+    const bool t0 = bit_of(mask, o0);
+    const bool t1 = v1 && bit_of(mask, o1);
+    const bool t2 = v2 && bit_of(mask, o2);
+    const bool t3 = v3 && bit_of(mask, o3);
     uint32_t   word = t3 ? entry(o3, k3) : 0u;
     word            = t2 ? (word << 8) | entry(o2, k2) : word;
     word            = t1 ? (word << 8) | entry(o1, k1) : word;

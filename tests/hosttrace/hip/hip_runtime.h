@@ -28,3 +28,7 @@ inline int4   make_int4(int x, int y, int z, int w) { return int4{x, y, z, w}; }
 inline uint2  make_uint2(unsigned x, unsigned y) { return uint2{x, y}; }
 inline int    __popc(unsigned x) { return __builtin_popcount(x); }
 inline int    __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned) x); }
+inline unsigned __builtin_amdgcn_ubfe(unsigned x, unsigned off, unsigned w)
+{
+    return (x >> (off & 31u)) & ((w & 31u) ? (1u << (w & 31u)) - 1u : 0u);
+}
