@@ -700,6 +700,7 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
         if (n > 0) {
             /* pop the first pending entry now (octree_fsh.c:336-367) */
             st.pw[level * st.stride] = word | (uint32_t) n | (1u << 3);
+            t.pend |= (n > 1 ? 1u : 0u) << level;
             st.ns[level * st.stride] = t.ns;
             if (DUAL) st.nd[level * st.stride] = t.nd;
             t.ns    = sb.child(foct);
@@ -714,19 +715,20 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
     }
 
     /* ---- backtrack (octree_fsh.c:368-375) and pop the next pending ---- */
-    int lv = level, ix = t.ix, iy = t.iy, iz = t.iz;
-    for (;;) {
-        lv--;
-        if (lv < 0) return ST_MISS;
-        ix >>= 1;
-        iy >>= 1;
-        iz >>= 1;
+    /* the deepest level with a pending entry, from the per-lane mask (bit l
+     * set while level l has one; the levels between have none) */
+    const uint32_t below = t.pend & ((1u << level) - 1u);
+    if (below == 0u) return ST_MISS;
+    const int lv = 31 - __clz((int) below), sh = level - lv;
+    const int ix = t.ix >> sh, iy = t.iy >> sh, iz = t.iz >> sh;
+    {
         uint32_t w = st.pw[lv * st.stride];
         int      k = (int) ((w >> 3) & 7u), cnt = (int) (w & 7u);
-        if (k < cnt) {
+        {
             int e   = (int) ((w >> (8 + 5 * k)) & 31u);
             int o   = e & 7, src = e >> 3;
             st.pw[lv * st.stride] = (w & ~(7u << 3)) | ((uint32_t) (k + 1) << 3);
+            t.pend &= k + 1 == cnt ? ~(1u << lv) : ~0u;
             float4 isp;
             if (src == 0) {
                 isp = entry_of<SEM, GRID>(s, r, st, lv, ix, iy, iz, t.start);
