@@ -309,20 +309,6 @@ __device__ __forceinline__ int child_slot(const TreeDev& t, int i, int k, int le
     return b.child(k);
 }
 
-struct Cand {
-    float x, y, z, w;
-    int src;
-};
-
-__device__ __forceinline__ void cswap(Cand& a, Cand& b)
-{
-    if (b.w < a.w) { /* strict: ties keep order (octree_fsh.c:283) */
-        Cand t = a;
-        a      = b;
-        b      = t;
-    }
-}
-
 /* Stack columns in LDS: word j of the calling lane is at base[j * stride]. */
 struct Stack {
     uint32_t* pw;
@@ -644,71 +630,63 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
         cube_at<GRID>(s, level, t.ix, t.iy, t.iz, tx, ty, tz, tw);
         cube_mid<GRID>(tx, ty, tz, tw, bxx, byy, bzz, hx, hy, hz);
 
-        const float qnan = __builtin_nanf("");
-        Cand        c0 = {t.entry.x, t.entry.y, t.entry.z, t.entry.w, 0};
-        Cand        c1, c2, c3;
-        {
-            float4 a = plane_z<SEM>(hz, r);
-            bool   ok = a.w > 0.0f && tx < a.x && a.x <= bxx && ty > a.y && a.y >= byy;
-            c1        = {a.x, a.y, a.z, ok ? a.w : qnan, 1};
+        /* candidates as (t, key) pairs, as in trace_step_c1 */
+        const float  qnan = __builtin_nanf("");
+        const float4 az   = plane_z<SEM>(hz, r);
+        const float4 ax   = plane_x<SEM>(hx, r);
+        const float4 ay   = plane_y<SEM>(hy, r);
+        float        w0   = t.entry.w;
+        float        w1 = (az.w > 0.0f && tx < az.x && az.x <= bxx && ty > az.y && az.y >= byy) ? az.w : qnan;
+        float        w2 = (ax.w > 0.0f && ty > ax.y && ax.y >= byy && tz > ax.z && ax.z >= bzz) ? ax.w : qnan;
+        float        w3 = (ay.w > 0.0f && tx < ay.x && ay.x <= bxx && tz > ay.z && ay.z >= bzz) ? ay.w : qnan;
+        const int    oe = (t.entry.x > hx ? 1 : 0) | (t.entry.y < hy ? 2 : 0) | (t.entry.z < hz ? 4 : 0);
+        int          k0 = oe | (oe << 5);
+        int k1 = SEM != kSemShader ? octet_key(az.x, az.y, az.z, hx, hy, hz, 1) : zplane_key(az.x, az.y, hx, hy);
+        int k2 = SEM != kSemShader ? octet_key(ax.x, ax.y, ax.z, hx, hy, hz, 2) : xplane_key(ax.y, ax.z, hy, hz);
+        int k3 = SEM != kSemShader ? octet_key(ay.x, ay.y, ay.z, hx, hy, hz, 3) : yplane_key(ay.x, ay.z, hx, hz);
+        if (w1 < w0 || w2 < w0 || w3 < w0) { /* selection-swap sort (octree_fsh.c:283); NaN never moves */
+            k0 = octet_key(t.entry.x, t.entry.y, t.entry.z, hx, hy, hz, 0);
+            kswap(w0, k0, w1, k1);
+            kswap(w0, k0, w2, k2);
+            kswap(w0, k0, w3, k3);
         }
-        {
-            float4 a = plane_x<SEM>(hx, r);
-            bool   ok = a.w > 0.0f && ty > a.y && a.y >= byy && tz > a.z && a.z >= bzz;
-            c2        = {a.x, a.y, a.z, ok ? a.w : qnan, 2};
-        }
-        {
-            float4 a = plane_y<SEM>(hy, r);
-            bool   ok = a.w > 0.0f && tx < a.x && a.x <= bxx && tz > a.z && a.z >= bzz;
-            c3        = {a.x, a.y, a.z, ok ? a.w : qnan, 3};
-        }
-        /* selection-swap sort; rejected slots hold NaN and never move */
-        cswap(c0, c1);
-        cswap(c0, c2);
-        cswap(c0, c3);
-        cswap(c1, c2);
-        cswap(c1, c3);
-        cswap(c2, c3);
+        kswap(w1, k1, w2, k2);
+        kswap(w1, k1, w3, k3);
+        kswap(w2, k2, w3, k3);
 
-        int      pre = -1, n = 0, foct = 0;
-        uint32_t word = 0;
-        float4   first = t.entry;
-#define QBT_CLASSIFY(c)                                                                     \
-    if ((c).w == (c).w) {                                                                   \
-        int o = ((c).x > hx ? 1 : 0) | ((c).y < hy ? 2 : 0) | ((c).z < hz ? 4 : 0);         \
-        if (o == pre) {                                                                     \
-            if ((c).x == hx) o ^= 1;                                                        \
-            else if ((c).y == hy) o ^= 2;                                                   \
-            else if ((c).z == hz) o ^= 4;                                                   \
-        }                                                                                   \
-        pre = o;                                                                            \
-        if ((mask >> o) & 1u) {                                                             \
-            word |= (uint32_t) (o | ((c).src << 3)) << (8 + 5 * n);                         \
-            if (n == 0) {                                                                   \
-                first = make_float4((c).x, (c).y, (c).z, (c).w);                            \
-                foct  = o;                                                                  \
-            }                                                                               \
-            n++;                                                                            \
-        }                                                                                   \
-    }
-        QBT_CLASSIFY(c0)
-        QBT_CLASSIFY(c1)
-        QBT_CLASSIFY(c2)
-        QBT_CLASSIFY(c3)
-#undef QBT_CLASSIFY
+        /* flip an octet equal to the previous valid one; keep those whose
+         * child exists in either tree, in order, as 5-bit entries */
+        const bool v1 = w1 == w1, v2 = w2 == w2, v3 = w3 == w3;
+        const int  o0 = k0 & 7;
+        const int  o1 = (k1 & 7) == o0 ? ((k1 >> 5) & 7) : (k1 & 7);
+        const int  p1 = v1 ? o1 : o0;
+        const int  o2 = (k2 & 7) == p1 ? ((k2 >> 5) & 7) : (k2 & 7);
+        const int  p2 = v2 ? o2 : p1;
+        const int  o3 = (k3 & 7) == p2 ? ((k3 >> 5) & 7) : (k3 & 7);
+        const bool t0 = bit_of(mask, o0);
+        const bool t1 = v1 && bit_of(mask, o1);
+        const bool t2 = v2 && bit_of(mask, o2);
+        const bool t3 = v3 && bit_of(mask, o3);
+        auto ent = [](int o, int k) -> uint32_t { return (uint32_t) o | ((uint32_t) k & 0x18u); };
+        uint32_t entries = t3 ? ent(o3, k3) : 0u;
+        entries          = t2 ? (entries << 5) | ent(o2, k2) : entries;
+        entries          = t1 ? (entries << 5) | ent(o1, k1) : entries;
+        entries          = t0 ? (entries << 5) | ent(o0, k0) : entries;
+        const int n      = (t0 ? 1 : 0) + (t1 ? 1 : 0) + (t2 ? 1 : 0) + (t3 ? 1 : 0);
 
         if (n > 0) {
             /* pop the first pending entry now (octree_fsh.c:336-367) */
-            st.pw[level * st.stride] = word | (uint32_t) n | (1u << 3);
-            t.pend |= (n > 1 ? 1u : 0u) << level;
+            const int foct = (int) (entries & 7u);
+            st.pw[level * st.stride] = (entries << 8) | (uint32_t) n | (1u << 3);
             st.ns[level * st.stride] = t.ns;
             if (DUAL) st.nd[level * st.stride] = t.nd;
+            t.pend |= (n > 1 ? 1u : 0u) << level;
+            t.entry = pick_entry((int) ((entries >> 3) & 3u), t.entry, az, ax, ay);
             t.ns    = sb.child(foct);
             t.nd    = DUAL ? db.child(foct) : 0;
             t.ix    = (t.ix << 1) | (foct & 1);
             t.iy    = (t.iy << 1) | ((foct >> 1) & 1);
             t.iz    = (t.iz << 1) | (foct >> 2);
-            t.entry = first;
             t.level = level + 1;
             return t.level == s.maxlevel ? ST_HIT : ST_CONT;
         }
