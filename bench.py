@@ -27,8 +27,9 @@ CPU baseline: the C oracle (restatement of the reference traversal) on whole
 frames of the timed camera path on the host's cores, each frame checked
 bit-exactly against the GPU frame.
 
-Two more legs, reported beside the headline: the same scene at 3840x2160
-(uhd_3840x2160), and batched views (batch_views): the 90M-point scene with 4
+More legs, reported beside the headline: the same scene at 3840x2160
+(uhd_3840x2160); the same scene with a populated dynamic tree walked in
+lockstep (dynamic_tree); and batched views (batch_views): the 90M-point scene with 4
 lights, 64 views of resolutions drawn from 320x240 ... 3840x2160 at random
 poses rendered as one hipGraph-captured batch with the autotuner's tuning,
 two of its views checked bit for bit against single-view renders.
@@ -86,6 +87,7 @@ def parse():
     ap.add_argument("--sur-lr", type=float, default=2e-3)
     ap.add_argument("--sur-band", type=float, nargs=2, default=(0.1, 0.9))
     ap.add_argument("--uhd-frames", type=int, default=5, help="frames of the 3840x2160 leg (0 = skip)")
+    ap.add_argument("--dyn-frames", type=int, default=5, help="frames of the dynamic-tree leg (0 = skip)")
     ap.add_argument("--batch-views", type=int, default=64, help="views of the batched leg (0 = skip)")
     ap.add_argument("--batch-reps", type=int, default=3, help="timed batches of the batched leg")
     ap.add_argument("--traffic", default=TRAFFIC_FILE)
@@ -309,6 +311,47 @@ def uhd_leg(args, q, ctx, sc, mask, copy_gbps):
             "roofline": {k: roof[k] for k in ("kernel", "achieved", "frac", "kernel_avg_ms", "frame")}}
 
 
+def dynamic_leg(args, q, ctx, sc, mask, copy_gbps):
+    """The same scene with a populated dynamic tree walked in lockstep with
+    the static one (octree_fsh.c:223-241, 313-317): a seeded 2M-point shell
+    (a standing figure, 240 x 600 x 240 units) near the cube centre, its own
+    colours and normals; K frames of the timed path at 1920x1080."""
+    W, H, K = args.width, args.height, args.dyn_frames
+    rng = np.random.default_rng(args.seed + 77)
+    m = 2000000
+    u = rng.normal(size=(m, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    dp = (np.array([900.0, 700.0, 900.0]) + u * np.array([120.0, 300.0, 120.0])).astype(np.float32)
+    ctx.upload_points(dp, dynamic=True)
+    ctx.upload_model(rng.uniform(0.0, 1.0, (m, 3)).astype(np.float32), u.astype(np.float32), dynamic=True)
+    dn = ctx.build_octree(sc.basecube, sc.depth, m, dynamic=True)
+    lights = sc.lights
+    pos, ang = sc.poses(K + 1, seed=args.pose_seed)
+    views = [q.view(pos[i], ang[i], W, H, sc.basecube, sc.depth) for i in range(K + 1)]
+    rays, exps, exps_p, leafhits = count_frames(q, ctx, views[1:], lights, mask)
+    ctx.render(views[0], lights, mask)
+    ctx.synchronize()
+    ctx.profile_begin()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for v in views[1:]:
+        ctx.render(v, lights, mask)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_end()
+    dual = bool(ctx.stats()["dual"])
+    ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)  # back to the static-only scene
+    value = float(sum(rays)) / wall / 1e6
+    km, kl = prof["kind_ms"], prof["kind_launches"]
+    log(f"dynamic tree ({m} points, {dn} nodes): {value:.1f} Mrays/s, {wall * 1e3 / K:.2f} ms/frame")
+    return {"value": round(value, 3), "unit": "Mrays/s", "ms_per_step": round(wall * 1e3 / K, 4), "steps": K,
+            "dynamic_points": m, "dynamic_nodes": dn, "dual_walk": dual,
+            "rays_per_frame": round(float(np.mean(rays))),
+            "expansions_per_frame": round(float(np.mean(exps))),
+            "kernel_ms_per_frame": {"primary": round(km[KIND_PRIMARY] / K, 4),
+                                    "shadow": round(km[KIND_SHADOW] / K, 4)}}
+
+
 BATCH_RES = ((320, 240), (640, 480), (1280, 720), (1920, 1080), (3840, 2160))
 
 
@@ -416,6 +459,9 @@ def main():
 
     K, Wm = args.steps, args.warmup
     pos, ang = sc.poses(K + Wm, seed=args.pose_seed)
+    if args.config == 1:  # the reference demo camera: cube centre, looking +z (yaw pi), every frame
+        pos[:] = (900.0, 900.0, 900.0)
+        ang[:] = (np.pi, 0.0)
     views = [q.view(pos[i], ang[i], W, H, sc.basecube, sc.depth) for i in range(K + Wm)]
     mask = q.OUT_DEFAULT
 
@@ -483,6 +529,7 @@ def main():
                "seconds": round(tc, 2), "bit_exact_vs_gpu": same}
 
     uhd = uhd_leg(args, q, ctx, sc, mask, copy_gbps) if args.uhd_frames > 0 else None
+    dyn = dynamic_leg(args, q, ctx, sc, mask, copy_gbps) if args.dyn_frames > 0 else None
     batch = batch_leg(args, q, ctx, nthreads, mask) if args.batch_views > 0 else None  # replaces the scene
 
     res = {
@@ -491,7 +538,9 @@ def main():
         "ms_per_step": round(wall * 1e3 / K, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator, no assets)",
         "config": {"workload": f"config{args.config} exact primary+shadow traversal, {sc.npoints} points, "
-                               f"depth {sc.depth}, {W}x{H}, {nl} lights, seeded camera path",
+                               f"depth {sc.depth}, {W}x{H}, {nl} lights, "
+                               + ("the demo camera (900, 900, 900) looking +z" if args.config == 1
+                                  else "seeded camera path"),
                    "points": sc.npoints, "nodes": nnodes, "width": W, "height": H, "lights": int(nl),
                    "poses": K, "seed": args.seed, "pose_seed": args.pose_seed,
                    "rays_per_frame": round(total_rays / K), "expansions_per_frame": round(float(np.mean(exps)))},
@@ -499,6 +548,7 @@ def main():
         "cpu_baseline": cpu,
         "surrogate": sur,
         "uhd_3840x2160": uhd,
+        "dynamic_tree": dyn,
         "batch_views": batch,
         "tuning": tuning,
         "build_s": round(t_build, 3),
