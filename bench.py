@@ -60,6 +60,20 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traversal_config3_1080p.json"
 KIND_PRIMARY, KIND_SHADOW, KIND_SURROGATE = 0, 1, 2
 
 
+def host_cpu():
+    """model name and logical CPU count of the host (the CPU baseline's machine)"""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count()}
+
+
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
@@ -526,7 +540,7 @@ def main():
         cpu = {"value": round(cpu_rays / tc / 1e6, 4), "unit": "Mrays/s", "cores": nthreads, "kind": "port",
                "sample": f"{frames} whole {W}x{H} frames of the timed camera path ({cpu_rays} rays), "
                          f"C oracle -O2 -ffp-contract=off, pthreads",
-               "seconds": round(tc, 2), "bit_exact_vs_gpu": same}
+               "seconds": round(tc, 2), "bit_exact_vs_gpu": same, "host_cpu": host_cpu()}
 
     uhd = uhd_leg(args, q, ctx, sc, mask, copy_gbps) if args.uhd_frames > 0 else None
     dyn = dynamic_leg(args, q, ctx, sc, mask, copy_gbps) if args.dyn_frames > 0 else None

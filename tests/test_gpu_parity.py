@@ -313,3 +313,27 @@ def test_upload_argument_checks(ctx):
         with pytest.raises(q.QbtError) as e:
             ctx.upload(nodes, *args)
         assert e.value.code == T.QBT_EINVAL
+
+
+def test_render_and_trace_limits(ctx):
+    """Argument limits: at most QBT_MAX_LIGHTS_RENDER lights and QBT_MAX_VIEWS
+    views per call, views of at least 1x1, maxlevel within the supported
+    depth, known trace semantics; an empty particle or ray batch is a no-op."""
+    sc, pnt, nrm, col, nodes = H.scene(1, npoints=20000)
+    _upload_scene(ctx, nodes, col, nrm)
+    T = q._ffi
+    v = q.view((900, 900, 900), (0.3, 0.1), 16, 16, sc.basecube, sc.depth)
+    cases = [
+        lambda: ctx.render(v, np.full((9, 3), 900.0, np.float32)),                       # 9 lights
+        lambda: ctx.render([v] * 257, sc.lights),                                         # 257 views
+        lambda: ctx.render(q.view((900, 900, 900), (0, 0), 0, 16, sc.basecube, sc.depth), sc.lights),
+        lambda: ctx.render(q.view((900, 900, 900), (0, 0), 16, 16, sc.basecube, 24), sc.lights),
+        lambda: ctx.trace_rays(np.zeros((1, 3)), np.ones((1, 3)), sc.basecube, sc.depth, semantics=3),
+    ]
+    for k, f in enumerate(cases):
+        with pytest.raises(q.QbtError) as e:
+            f()
+        assert e.value.code == T.QBT_EINVAL, k
+    ctx.render(v, np.full((8, 3), 900.0, np.float32))  # 8 lights is allowed
+    p, s = ctx.particle_update(np.zeros((0, 3)), np.zeros((0, 3)), sc.depth, float(sc.basecube[3]))
+    assert p.shape == (0, 3) and s.shape == (0, 3)
