@@ -692,8 +692,11 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     const unsigned refill = (unsigned) (c->tune_refill > 0 && c->tune_refill <= 64 ? c->tune_refill : 32);
     auto           kp     = k_primary<CMP, DUAL, GRID, COUNT, GSTK>;
     auto           ks     = k_shadow<CMP, DUAL, GRID, COUNT, GSTK>;
-    const int      gp     = persistent_grid(c, kp, bs, lds);
-    const int      gs     = persistent_grid(c, ks, bs, lds);
+    /* resident blocks, but no more than one work item per lane: small
+     * batches then start fewer waves (less launch and refill-counter
+     * contention); shadow items are at most one per batch pixel */
+    const int      gp = std::min(persistent_grid(c, kp, bs, lds), (int) nblk((size_t) nitems + nspr, (unsigned) bs));
+    const int      gs = std::min(persistent_grid(c, ks, bs, lds), (int) nblk(npix, (unsigned) bs));
     if (GSTK && (rc = c->dev_alloc(c->wb_gstack, qbt_stack_bytes(s.maxlevel, DUAL, (gp > gs ? gp : gs) * bs))))
         return rc;
 

@@ -317,8 +317,8 @@ def test_upload_argument_checks(ctx):
 
 def test_render_and_trace_limits(ctx):
     """Argument limits: at most QBT_MAX_LIGHTS_RENDER lights and QBT_MAX_VIEWS
-    views per call, views of at least 1x1, maxlevel within the supported
-    depth, known trace semantics; an empty particle or ray batch is a no-op."""
+    views per call, views of at least 1x1, maxlevel at most 24 (the LDS
+    stack depth), known trace semantics; an empty particle batch is a no-op."""
     sc, pnt, nrm, col, nodes = H.scene(1, npoints=20000)
     _upload_scene(ctx, nodes, col, nrm)
     T = q._ffi
@@ -327,7 +327,7 @@ def test_render_and_trace_limits(ctx):
         lambda: ctx.render(v, np.full((9, 3), 900.0, np.float32)),                       # 9 lights
         lambda: ctx.render([v] * 257, sc.lights),                                         # 257 views
         lambda: ctx.render(q.view((900, 900, 900), (0, 0), 0, 16, sc.basecube, sc.depth), sc.lights),
-        lambda: ctx.render(q.view((900, 900, 900), (0, 0), 16, 16, sc.basecube, 24), sc.lights),
+        lambda: ctx.render(q.view((900, 900, 900), (0, 0), 16, 16, sc.basecube, 25), sc.lights),  # deeper than 24
         lambda: ctx.trace_rays(np.zeros((1, 3)), np.ones((1, 3)), sc.basecube, sc.depth, semantics=3),
     ]
     for k, f in enumerate(cases):
