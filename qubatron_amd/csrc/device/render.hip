@@ -58,23 +58,25 @@ using namespace qbt;
 
 namespace {
 
-constexpr int kSeg = 8;
-constexpr int kNL  = QBT_MAX_LIGHTS_RENDER;
+constexpr int kSeg     = 8;  /* work segments per launch (block b starts in segment b % kSeg: one XCD each) */
+constexpr int kCtrPad  = 64; /* uint32 between refill counters: each on its own 256-byte line */
+constexpr int kNL      = QBT_MAX_LIGHTS_RENDER;
 
-/* batch counters (uint32), zeroed before each batch: refill counters of the
- * primary and per-light shadow launches, the number of shaded pixels, the
- * per-light exact-trace counts of the surrogate mode, the per-(light,
- * octant) counts and cursors of the reordering */
+/* batch counters (uint32), zeroed before each batch: the number of shaded
+ * pixels, the per-light exact-trace counts of the surrogate mode, the
+ * per-(light, octant) counts and cursors of the reordering, then the refill
+ * counters of the primary and per-light shadow launches, kCtrPad apart
+ * (every refill is a device-scope atomic on one of them; counters that
+ * share a line would queue behind each other) */
 enum {
-    Q_PRIM   = 0,
-    Q_SHADOW = kSeg,
-    Q_NSEL   = kSeg + kSeg * kNL,
+    Q_NSEL   = 0,
     Q_FB     = Q_NSEL + 1,
     Q_OCT    = Q_FB + kNL,
     Q_CUR    = Q_OCT + 8 * kNL,
-    Q_N      = Q_CUR + 8 * kNL
+    Q_PRIM   = ((Q_CUR + 8 * kNL + kCtrPad - 1) / kCtrPad) * kCtrPad,
+    Q_SHADOW = Q_PRIM + kSeg * kCtrPad,
+    Q_N      = Q_SHADOW + kSeg * kCtrPad * kNL
 };
-static_assert(Q_N <= 256, "counter block");
 
 struct PrimOut {
     float4*   isp;   /* entry point of the primary leaf, 0 when none */
@@ -101,7 +103,7 @@ struct Refill {
         const int      leader = __ffsll((unsigned long long) idle) - 1;
         const uint32_t lo = seg_bound(n, seg), hi = seg_bound(n, seg + 1);
         uint32_t       base = 0;
-        if (lane == leader) base = atomicAdd(&ctr[seg], nidle);
+        if (lane == leader) base = atomicAdd(&ctr[seg * kCtrPad], nidle);
         base          = __shfl(base, leader, 64);
         uint32_t item = UINT32_MAX;
         if ((idle >> lane) & 1ull) {
@@ -480,7 +482,7 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, float Lx, float Ly, 
     for (;;) {
         const uint64_t idle = __ballot(!active);
         if (rf.left > 0 && (__popcll(idle) >= (int) refill || idle == ~0ull)) {
-            const uint32_t k = rf.fetch(&q[Q_SHADOW + kSeg * l], idle, lane);
+            const uint32_t k = rf.fetch(&q[Q_SHADOW + kSeg * kCtrPad * l], idle, lane);
             if (k != UINT32_MAX) {
                 gp = list[k];
                 if (l > 0 && disc[gp]) {
@@ -608,7 +610,7 @@ __global__ void __launch_bounds__(256) k_shade(SceneDev s, BatchDev b, PrimOut i
 
 __global__ void k_zero_u32(unsigned* p, int n)
 {
-    int i = threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
 }
 
@@ -720,7 +722,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
      * (no-ops unless profiling, which is never on while capturing) */
     auto enqueue = [&]() -> int {
         int err;
-        k_zero_u32<<<1, 256, 0, st>>>(q, Q_N);
+        k_zero_u32<<<nblk(Q_N), 256, 0, st>>>(q, Q_N);
         k_zero_u8<<<nblk(npix), 256, 0, st>>>((uint8_t*) c->wb_disc.p, npix);
         c->prof_mark_begin(QBT_PROF_PRIMARY);
         kp<<<gp, bs, lds, st>>>(s, b, po, nitems, nspr, nl > 0 ? nl : 1, c->tune_strip, (int*) c->sprite.p, q,
