@@ -201,8 +201,8 @@ int qbt_profile_end(qbt_ctx* ctx, qbt_profile* out);
  * and shadow rays back to back), 1 = wavefront stages with persistent,
  * lane-refilling traversal kernels over a batch of views. block_threads:
  * 64..256 (0 = default 256; halved while the LDS stack exceeds 64 KiB).
- * refill: idle lanes that make a wave claim new rays, 1..64 (0 = 32): the
- * rays a wave takes per refill. layout: 0 = traverse the compact
+ * refill: idle lanes that make a wave claim new rays, 1..64 (0 = 24 for
+ * camera rays and 16 for shadow rays): the rays a wave takes per refill. layout: 0 = traverse the compact
  * breadth-first node layout (8 bytes per node, children contiguous) whenever
  * the node array is a proper tree, 1 = always the direct 32-byte child
  * blocks. stack: 0 = per-lane traversal stacks in LDS, 1 = in global memory.
@@ -228,7 +228,7 @@ int qbt_set_tuning(qbt_ctx* ctx, const qbt_tuning* in);
 /* Autotuner: times qbt_render of the given views and lights (QBT_RENDER_*
  * counting/collection flags are dropped) under candidate tunings of the
  * wavefront path - coordinate descent over block_threads {64, 128, 256},
- * refill {8, 16, 32, 48, 64}, stack {LDS, global} and reorder {0, 1}, two
+ * refill {defaults, 8, 16, 24, 32, 48}, stack {LDS, global} and reorder {0, 1}, two
  * sweeps, a candidate kept only if it is at least 1% faster - and leaves the
  * fastest set on the context. results (up to max_results) receive every
  * timed candidate, *nresults how many were timed. reps: timed renders per

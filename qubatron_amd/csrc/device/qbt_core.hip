@@ -99,8 +99,7 @@ __global__ void k_validate(const int4* __restrict__ child, size_t lo, size_t hi,
 
 __global__ void k_zero_u64(unsigned long long* p, int n)
 {
-    int i = threadIdx.x;
-    if (i < n) p[i] = 0ull;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v)
@@ -1186,7 +1185,6 @@ extern "C" int qbt_autotune(qbt_ctx* c, const qbt_view* views, int nviews, const
     if (rc) return rc;
     best.path          = 1;
     best.block_threads = best.block_threads ? best.block_threads : 256; /* the defaults, made explicit */
-    best.refill        = best.refill ? best.refill : 32;
     double best_ms     = 0.0;
     int    n       = 0;
     auto   run     = [&](const qbt_tuning& t, double* ms) -> int {
@@ -1199,12 +1197,12 @@ extern "C" int qbt_autotune(qbt_ctx* c, const qbt_view* views, int nviews, const
     if ((rc = run(best, &best_ms))) return rc;
     /* coordinate descent over the knobs, two sweeps */
     static const int blocks[]  = {64, 128, 256};
-    static const int refills[] = {8, 16, 32, 48, 64};
+    static const int refills[] = {0, 8, 16, 24, 32, 48}; /* 0: the per-launch defaults */
     for (int sweep = 0; sweep < 2; sweep++) {
         bool improved = false;
         for (int knob = 0; knob < 4; knob++) {
             const int* vals = knob == 0 ? blocks : knob == 1 ? refills : nullptr;
-            const int  nv   = knob == 0 ? 3 : knob == 1 ? 5 : 2;
+            const int  nv   = knob == 0 ? 3 : knob == 1 ? 6 : 2;
             for (int k = 0; k < nv; k++) {
                 qbt_tuning t = best;
                 int*       f = knob == 0 ? &t.block_threads : knob == 1 ? &t.refill : knob == 2 ? &t.stack : &t.reorder;

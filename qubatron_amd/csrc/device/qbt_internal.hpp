@@ -152,7 +152,7 @@ struct qbt_ctx {
     uint64_t     aborted         = 0;
     int          iter_cap        = 1 << 22;
     int          tune_block      = 0;
-    int          tune_refill     = 0;  /* idle lanes that trigger a wave refill (1..64; 0 = 32) */
+    int          tune_refill     = 0;  /* idle lanes that trigger a wave refill (1..64; 0 = 24 camera, 16 shadow) */
     int          path            = 1;  /* 0 = fused per-pixel kernel, 1 = wavefront stages */
     int          num_cus         = 256;
     /* wavefront render buffers */

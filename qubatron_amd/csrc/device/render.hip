@@ -691,7 +691,11 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
 
     const int      bs     = c->block_threads(s.maxlevel, DUAL);
     const size_t   lds    = GSTK ? 0 : qbt_stack_bytes(s.maxlevel, DUAL, bs);
-    const unsigned refill = (unsigned) (c->tune_refill > 0 && c->tune_refill <= 64 ? c->tune_refill : 32);
+    /* refill thresholds: the tuned one for both launches, else the measured
+     * defaults (camera rays are shorter, so their refills cost more of a ray) */
+    const bool     tuned     = c->tune_refill > 0 && c->tune_refill <= 64;
+    const unsigned refill    = (unsigned) (tuned ? c->tune_refill : 24);
+    const unsigned refill_sh = (unsigned) (tuned ? c->tune_refill : 16);
     auto           kp     = k_primary<CMP, DUAL, GRID, COUNT, GSTK>;
     auto           ks     = k_shadow<CMP, DUAL, GRID, COUNT, GSTK>;
     /* resident blocks, but no more than one work item per lane: small
@@ -747,7 +751,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
             c->prof_mark_begin(QBT_PROF_SHADOW);
             ks<<<gs, bs, lds, st>>>(s, L.p[l][0], L.p[l][1], L.p[l][2], l, li, ln, po.isp,
                                     (uint8_t*) c->wb_lstat.p + (size_t) l * npix, (uint8_t*) c->wb_disc.p, q,
-                                    refill, gstk, po.exp, cnt);
+                                    refill_sh, gstk, po.exp, cnt);
             c->prof_mark_end();
         }
         if (collect && (err = qbt_collect_batch(c, s, DUAL, L, lists, lstride, q + Q_NSEL, npix))) return err;
@@ -773,7 +777,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     static_assert(sizeof(bp) == sizeof(key.bufs), "graph key buffers");
     memcpy(key.bufs, bp, sizeof(bp));
     key.nitems = nitems, key.npix = npix, key.nviews = (uint32_t) b.n, key.nlights = (uint32_t) nl;
-    key.mask = mask, key.flags = flags, key.bs = bs, key.refill = (int) refill, key.gp = gp, key.gs = gs;
+    key.mask = mask, key.flags = flags, key.bs = bs, key.refill = (int) (refill | refill_sh << 8), key.gp = gp, key.gs = gs;
     key.stack = GSTK, key.reorder = reorder, key.strip = c->tune_strip;
     memcpy(key.lights, L.p, sizeof(key.lights));
     key.tmpl = (CMP ? 1u : 0u) | (DUAL ? 2u : 0u) | (GRID ? 4u : 0u);

@@ -572,15 +572,31 @@ __global__ void k_collect(SceneDev s, LightsDev L, const uint32_t* __restrict__ 
                           size_t npix, uint32_t stride, uint32_t phase, RawSample* __restrict__ out, size_t cap,
                           unsigned long long* __restrict__ ctr)
 {
-    const uint64_t nq = *nsel;
-    const uint64_t qi = ((uint64_t) blockIdx.x * blockDim.x + threadIdx.x) * stride + phase;
-    if (qi >= nq * (uint64_t) L.n) return;
-    const int      l  = (int) (qi / nq);
-    const uint32_t gp = lists[(size_t) l * lstride + (qi - (uint64_t) l * nq)];
-    const uint8_t  ls = lstat[(size_t) l * npix + gp];
-    if (ls != LS_LIT && ls != LS_OCCLUDED) return;
-    const unsigned long long slot = atomicAdd(&ctr[0], 1ull);
-    if (slot >= cap) return;
+    const uint64_t nq    = *nsel;
+    const uint64_t qi    = ((uint64_t) blockIdx.x * blockDim.x + threadIdx.x) * stride + phase;
+    const int      lane  = threadIdx.x & 63;
+    const bool     valid = qi < nq * (uint64_t) L.n;
+    int            l     = 0;
+    uint32_t       gp    = 0;
+    uint8_t        ls    = LS_SKIPPED;
+    if (valid) {
+        l  = (int) (qi / nq);
+        gp = lists[(size_t) l * lstride + (qi - (uint64_t) l * nq)];
+        ls = lstat[(size_t) l * npix + gp];
+    }
+    /* one slot reservation per wave (a per-sample atomic on one counter
+     * would queue every sample behind the others) */
+    const bool     keep = ls == LS_LIT || ls == LS_OCCLUDED;
+    const uint64_t m    = __ballot(keep);
+    if (m == 0ull) return; /* uniform over the wave */
+    const int          leader = __ffsll((unsigned long long) m) - 1;
+    unsigned long long base   = 0;
+    if (lane == leader) base = atomicAdd(&ctr[0], (unsigned long long) __popcll(m));
+    base                          = __shfl(base, leader, 64);
+    const unsigned long long slot = base + (unsigned long long) __popcll(m & ((1ull << lane) - 1ull));
+    const uint64_t           lit  = __ballot(keep && ls == LS_LIT && slot < cap);
+    if (lane == leader && lit) atomicAdd(&ctr[1], (unsigned long long) __popcll(lit));
+    if (!keep || slot >= cap) return;
     RawSample    rs;
     const float4 ip = isp[gp];
     rs.isp[0] = ip.x, rs.isp[1] = ip.y, rs.isp[2] = ip.z;
@@ -591,7 +607,6 @@ __global__ void k_collect(SceneDev s, LightsDev L, const uint32_t* __restrict__ 
     rs.light[0] = L.p[l][0], rs.light[1] = L.p[l][1], rs.light[2] = L.p[l][2];
     rs.label    = ls == LS_LIT ? 1.0f : 0.0f;
     out[slot]   = rs;
-    if (ls == LS_LIT) atomicAdd(&ctr[1], 1ull);
 }
 
 __global__ void k_unpack_samples(const RawSample* __restrict__ S, size_t start, size_t n, float* __restrict__ out)
