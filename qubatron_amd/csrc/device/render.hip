@@ -5,14 +5,14 @@
  * The pixel program of octree_fsh.c:399-464 is split into stages so that
  * every traversal kernel runs one kind of ray and keeps its SIMD lanes busy:
  *
- *   k_sprite_batch  (qbt_core.hip) per view and light, the sprite trace of
- *              :452-459, the same for every pixel of a view
- *   k_primary  camera rays of every view of the batch, one per pixel, pixels
- *              in 8x8-tile order, views in order; writes the hit record
- *              (entry point, model indices, leaf, flags) and, per work item,
- *              the batch pixel index when the pixel is shaded
- *   k_compact  list of the shaded pixels (one atomic per wave; a wave covers
- *              one tile, so tiles stay contiguous)
+ *   k_primary  first the sprite traces (:452-459; one per view and light,
+ *              the same for every pixel of a view), then the camera rays of
+ *              every view of the batch, one per pixel, pixels in 8x8-tile
+ *              order, views in order; writes the hit record (entry point,
+ *              model indices, leaf, flags) and, per work item, the batch
+ *              pixel index when the pixel is shaded
+ *   k_sel_*    list of the shaded pixels in item order: per-block counts, a
+ *              scan of the counts, scatter (no same-address atomics)
  *   k_oct_*    per light, the list reordered by the direction octant of the
  *              shadow ray (tuning reorder = 1)
  *   (k_mlp_shadow, surrogate.hip: QBT_RENDER_SURROGATE decisions)
@@ -30,7 +30,8 @@
  * contiguous segments, one per XCD (workgroups are dispatched to the eight
  * XCDs round-robin, so block b starts on segment b % 8 and moves on when its
  * own runs dry): neighbouring pixels, which share most of their node path,
- * stay on one XCD's L2.
+ * stay on one XCD's L2. The segment counters sit on separate 256-byte lines,
+ * since device-scope atomics on one line queue behind each other.
  *
  * The whole sequence contains kernels only (no memset, copy or
  * allocation), so it is captured once into a hipGraph and replayed for later
