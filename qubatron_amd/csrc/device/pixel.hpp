@@ -121,30 +121,31 @@ __device__ __forceinline__ unsigned wave_sum(unsigned x)
     return x;
 }
 
-/* Per-lane stacks in global memory (gstack: L * (dual ? 3 : 2) words per
- * thread of the grid), interleaved across lanes like the LDS stacks. */
+/* Per-lane stacks. Level l of a thread is one slot of W words (W = 2: the
+ * entry word and the node word; W = 3 with the dynamic tree's node word),
+ * slots interleaved across threads: word j of level l of thread i is at
+ * base[(l * nt + i) * W + j]. The two words a push writes and a pop reads sit
+ * side by side (one 64-bit LDS access, one address), and consecutive lanes
+ * touch consecutive slots. gstack: L * W words per thread of the grid. */
+__device__ __forceinline__ Stack stack_at(uint32_t* base, uint32_t tid, uint32_t nt, int dual)
+{
+    Stack     st;
+    const int W = dual ? 3 : 2;
+    st.stride   = W * (int) nt;
+    st.pw       = base + (size_t) W * tid;
+    st.ns       = reinterpret_cast<int*>(st.pw) + 1;
+    st.nd       = dual ? reinterpret_cast<int*>(st.pw) + 2 : st.ns;
+    return st;
+}
+
 __device__ __forceinline__ Stack make_stack_global(uint32_t* gstack, int L, int dual)
 {
-    Stack          st;
-    const uint32_t nt  = gridDim.x * blockDim.x;
-    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    st.stride          = (int) nt;
-    st.pw              = gstack + tid;
-    st.ns              = reinterpret_cast<int*>(gstack) + (size_t) L * nt + tid;
-    st.nd              = dual ? reinterpret_cast<int*>(gstack) + 2 * (size_t) L * nt + tid : st.ns;
-    return st;
+    return stack_at(gstack, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x, dual);
 }
 
 __device__ __forceinline__ Stack make_stack(uint32_t* lds, int L, int dual)
 {
-    Stack st;
-    int   nt  = blockDim.x;
-    int   tid = threadIdx.x;
-    st.stride = nt;
-    st.pw     = lds + tid;
-    st.ns     = reinterpret_cast<int*>(lds) + L * nt + tid;
-    st.nd     = dual ? reinterpret_cast<int*>(lds) + 2 * L * nt + tid : st.ns;
-    return st;
+    return stack_at(lds, threadIdx.x, blockDim.x, dual);
 }
 
 } // namespace qbt

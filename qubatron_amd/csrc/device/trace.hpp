@@ -224,6 +224,28 @@ __device__ __forceinline__ float4 midplane(int src, const Ray& r, float x, float
     return plane_k<SEM>(k, c, r);
 }
 
+/* midplane() of the cube at `level` whose cell coordinates are (ix, iy,
+ * iz) >> sh. On the closed-form grid only the plane's own coordinate of the
+ * cube is formed, by the two FMAs that cube_at and cube_mid apply to it (the
+ * sign folded into the exact product), so the bits are the same. */
+template <int SEM, bool GRID>
+__device__ __forceinline__ float4 midplane_cell(const SceneDev& s, const Ray& r, int src, int level, int ix, int iy,
+                                                int iz, int sh)
+{
+    if constexpr (GRID) {
+        const int   k  = src == 1 ? 2 : src - 2; /* src 1 z, 2 x, 3 y */
+        const int   ci = (k == 0 ? ix : (k == 1 ? iy : iz)) >> sh;
+        const float w  = ldexpf(s.bw, -level);
+        const float sg = k == 0 ? 1.0f : -1.0f; /* x grows along +x, y and z along -y, -z */
+        const float c0 = __builtin_fmaf(sg * (float) ci, w, axis3(k, s.bx, s.by, s.bz));
+        return plane_k<SEM>(k, __builtin_fmaf(sg * 0.5f, w, c0), r);
+    } else {
+        float x, y, z, ww;
+        cube_at<GRID>(s, level, ix >> sh, iy >> sh, iz >> sh, x, y, z, ww);
+        return midplane<SEM, GRID>(src, r, x, y, z, ww);
+    }
+}
+
 /* component selects by value (a conditional over lvalues selects an address
  * and forces the operands into scratch memory) */
 __device__ __forceinline__ int sel(bool c, int a, int b) { return c ? a : b; }
@@ -309,7 +331,8 @@ __device__ __forceinline__ int child_slot(const TreeDev& t, int i, int k, int le
     return b.child(k);
 }
 
-/* Stack columns in LDS: word j of the calling lane is at base[j * stride]. */
+/* The calling lane's stack: the entry word, the node word and the dynamic
+ * node word of level l are pw[l * stride], ns[l * stride], nd[l * stride]. */
 struct Stack {
     uint32_t* pw;
     int* ns;
@@ -327,12 +350,7 @@ __device__ __forceinline__ float4 entry_of(const SceneDev& s, const Ray& r, cons
         uint32_t w   = st.pw[(m - 1) * st.stride];
         int      k   = (int) ((w >> 3) & 7u) - 1;
         int      src = (int) ((w >> (8 + 5 * k + 3)) & 3u);
-        if (src != 0) {
-            int   sh = level - (m - 1);
-            float x, y, z, ww;
-            cube_at<GRID>(s, m - 1, ix >> sh, iy >> sh, iz >> sh, x, y, z, ww);
-            return midplane<SEM, GRID>(src, r, x, y, z, ww);
-        }
+        if (src != 0) return midplane_cell<SEM, GRID>(s, r, src, m - 1, ix, iy, iz, level - (m - 1));
     }
     return start;
 }
@@ -569,7 +587,9 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
         w                     = st.pw[lv * st.stride] >> 8; /* the next entry becomes the current one */
         st.pw[lv * st.stride] = w;
         st.ns[lv * st.stride] = (int) (an - (1u << 30));
-        t.pend &= (an >> 30) == 1u ? ~(1u << lv) : ~0u;
+        /* bit lv is set; clear it when this was the level's last pending
+         * entry (count field 1: bit 31 of the node word clear) */
+        t.pend ^= (1u << lv) & ~(uint32_t) ((int) an >> 31);
         a = an & 0x3FFFFFFFu;
         /* its entry point: the mid-plane hit of the nearest node on the path
          * (this one or an ancestor) entered through a mid-plane, else the
@@ -579,13 +599,7 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
             j--;
             src = (int) ((st.pw[j * st.stride] >> 3) & 3u);
         }
-        isp = t.start;
-        if (src != 0) {
-            const int sh = level - j;
-            float     x, y, z, ww;
-            cube_at<GRID>(s, j, t.ix >> sh, t.iy >> sh, t.iz >> sh, x, y, z, ww);
-            isp = midplane<SEM, GRID>(src, r, x, y, z, ww);
-        }
+        isp = src != 0 ? midplane_cell<SEM, GRID>(s, r, src, j, t.ix, t.iy, t.iz, level - j) : t.start;
     }
     const int sh = level - lv, o = (int) (w & 7u);
     t.ns    = (int) (a + ((w >> 5) & 7u));
@@ -711,9 +725,7 @@ __device__ __forceinline__ int trace_step(const SceneDev& s, const Ray& r, const
             if (src == 0) {
                 isp = entry_of<SEM, GRID>(s, r, st, lv, ix, iy, iz, t.start);
             } else {
-                float x, y, z, ww;
-                cube_at<GRID>(s, lv, ix, iy, iz, x, y, z, ww);
-                isp = midplane<SEM, GRID>(src, r, x, y, z, ww);
+                isp = midplane_cell<SEM, GRID>(s, r, src, lv, ix, iy, iz, 0);
             }
             t.ns = child_slot<CMP, SEM>(s.stat, st.ns[lv * st.stride], o, lv);
             t.nd = DUAL ? child_slot<CMP>(s.dyn, st.nd[lv * st.stride], o, lv) : 0;
