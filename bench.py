@@ -194,7 +194,8 @@ def roofline(args, prof, K, W, H, nl, exps, exps_p, leafhits, copy_gbps, with_tr
         kern[name] = {"alg_bytes_per_launch": b, "avg_ms": km[KIND_PRIMARY] / kl[KIND_PRIMARY],
                       "share": km[KIND_PRIMARY], "launches": kl[KIND_PRIMARY]}
     if kl[KIND_SHADOW]:
-        kern["k_shadow"] = {"alg_bytes_per_launch": 32.0 * e_s / nl, "avg_ms": km[KIND_SHADOW] / kl[KIND_SHADOW],
+        kern["k_shadow"] = {"alg_bytes_per_launch": 32.0 * e_s * K / kl[KIND_SHADOW],
+                            "avg_ms": km[KIND_SHADOW] / kl[KIND_SHADOW],
                             "share": km[KIND_SHADOW], "launches": kl[KIND_SHADOW]}
     traffic = load_traffic(args.traffic) if with_traffic else {}
     for name, e in kern.items():

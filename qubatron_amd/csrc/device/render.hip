@@ -16,10 +16,10 @@
  *   k_oct_*    per light, the list reordered by the direction octant of the
  *              shadow ray (tuning reorder = 1)
  *   (k_mlp_shadow, surrogate.hip: QBT_RENDER_SURROGATE decisions)
- *   k_shadow   one launch per light: the ray from the light to each shaded
- *              pixel's entry point (:431-434); writes the per-light status;
- *              pixels discarded by an earlier light are skipped, as the
- *              shader stops there
+ *   k_shadow   one launch for all lights: the ray from each light to each
+ *              shaded pixel's entry point (:431-434), light after light in
+ *              one work list; writes the per-light status; pixels that an
+ *              earlier light discarded are skipped (the shader stops there)
  *   k_shade    per pixel of the batch: attributes, shading (:436-449),
  *              sprite (:452-459), outputs into each view's frame
  *
@@ -66,7 +66,7 @@ constexpr int kNL      = QBT_MAX_LIGHTS_RENDER;
 /* batch counters (uint32), zeroed before each batch: the number of shaded
  * pixels, the per-light exact-trace counts of the surrogate mode, the
  * per-(light, octant) counts and cursors of the reordering, then the refill
- * counters of the primary and per-light shadow launches, kCtrPad apart
+ * counters of the primary and shadow launches, kCtrPad apart
  * (every refill is a device-scope atomic on one of them; counters that
  * share a line would queue behind each other) */
 enum {
@@ -76,7 +76,7 @@ enum {
     Q_CUR    = Q_OCT + 8 * kNL,
     Q_PRIM   = ((Q_CUR + 8 * kNL + kCtrPad - 1) / kCtrPad) * kCtrPad,
     Q_SHADOW = Q_PRIM + kSeg * kCtrPad,
-    Q_N      = Q_SHADOW + kSeg * kCtrPad * kNL
+    Q_N      = Q_SHADOW + kSeg * kCtrPad
 };
 
 struct PrimOut {
@@ -442,21 +442,32 @@ __global__ void k_oct_scan(const unsigned* __restrict__ cnt, unsigned* __restric
     }
 }
 
+/* Shadow rays of every light in one persistent launch: the work items are
+ * light 0's list, then light 1's, ... (light l: lists + l * lstride, count
+ * counts[l * cstride]), so the tail of one light's rays overlaps the next
+ * light's instead of draining the GPU once per light. Each lane carries its
+ * ray's light. A fragment that an earlier light discarded is skipped when
+ * that light's ray has already ended; otherwise its ray is traced as well
+ * (shading stops at the first discarding light, so the frame is the same;
+ * the sample collector applies the same rule). */
 template <bool CMP, bool DUAL, bool GRID, bool COUNT, bool GSTK>
-__global__ void __launch_bounds__(256) k_shadow(SceneDev s, float Lx, float Ly, float Lz, int l,
-                                                const uint32_t* __restrict__ list, const unsigned* __restrict__ nsel,
+__global__ void __launch_bounds__(256) k_shadow(SceneDev s, LightsDev L, const uint32_t* __restrict__ lists,
+                                                size_t lstride, const unsigned* __restrict__ counts, int cstride,
                                                 const float4* __restrict__ isp, uint8_t* __restrict__ lstat,
-                                                uint8_t* __restrict__ disc, unsigned* __restrict__ q,
+                                                size_t npix, uint8_t* __restrict__ disc, unsigned* __restrict__ q,
                                                 unsigned refill, uint32_t* gstack, int* __restrict__ exp,
                                                 unsigned long long* __restrict__ counters)
 {
     extern __shared__ uint32_t lds[];
     const Stack st   = lane_stack<DUAL, GSTK>(lds, gstack, s.maxlevel);
     const int   lane = threadIdx.x & 63;
-    Refill      rf(*nsel);
+    uint32_t    total = 0;
+    for (int j = 0; j < L.n; j++) total += counts[j * cstride];
+    Refill      rf(total);
     bool        active = false;
     uint32_t    gp     = 0;
-    Ray         r      = {Lx, Ly, Lz, 0.f, 0.f, 0.f};
+    int         lt     = 0; /* the light of this lane's ray */
+    Ray         r      = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float4      ip     = make_float4(0.f, 0.f, 0.f, 0.f);
     TraceState  t;
     unsigned    n_exp = 0, n_abort = 0, n_ray = 0;
@@ -473,7 +484,7 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, float Lx, float Ly, 
             float  sqr = ix * ix + iy * iy + iz * iz;
             ls         = (15.0f < sqr) ? LS_OCCLUDED : LS_LIT; /* step(sqr, 15.0) */
         }
-        lstat[gp] = ls;
+        lstat[(size_t) lt * npix + gp] = ls;
         if (COUNT) {
             atomicAdd(&exp[gp], (int) t.exps);
             n_exp += t.exps;
@@ -483,16 +494,35 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, float Lx, float Ly, 
     for (;;) {
         const uint64_t idle = __ballot(!active);
         if (rf.left > 0 && (__popcll(idle) >= (int) refill || idle == ~0ull)) {
-            const uint32_t k = rf.fetch(&q[Q_SHADOW + kSeg * kCtrPad * l], idle, lane);
+            const uint32_t k = rf.fetch(&q[Q_SHADOW], idle, lane);
             if (k != UINT32_MAX) {
-                gp = list[k];
+                /* item k -> (light, index in its list); the light's position by
+                 * selects (a per-lane index into the lights would go through
+                 * scratch memory) */
+                uint32_t kk = k;
+                int      l  = 0;
+                float    lx = L.p[0][0], ly = L.p[0][1], lz = L.p[0][2];
+                for (int j = 0; j + 1 < L.n; j++) {
+                    const uint32_t c    = counts[j * cstride];
+                    const bool     past = l == j && kk >= c;
+                    kk -= past ? c : 0u;
+                    l += past ? 1 : 0;
+                    lx = past ? L.p[j + 1][0] : lx;
+                    ly = past ? L.p[j + 1][1] : ly;
+                    lz = past ? L.p[j + 1][2] : lz;
+                }
+                lt = l;
+                gp = lists[(size_t) l * lstride + kk];
                 if (l > 0 && disc[gp]) {
-                    lstat[gp] = LS_SKIPPED; /* an earlier light discarded the fragment */
+                    lstat[(size_t) l * npix + gp] = LS_SKIPPED; /* an earlier light discarded the fragment */
                 } else {
                     ip      = isp[gp];
-                    r.dx    = ip.x - Lx;
-                    r.dy    = ip.y - Ly;
-                    r.dz    = ip.z - Lz;
+                    r.px    = lx;
+                    r.py    = ly;
+                    r.pz    = lz;
+                    r.dx    = ip.x - lx;
+                    r.dy    = ip.y - ly;
+                    r.dz    = ip.z - lz;
                     n_ray++;
                     int stt = trace_begin<false>(s, r, t);
                     if (stt == ST_CONT) active = true;
@@ -703,7 +733,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
      * batches then start fewer waves (less launch and refill-counter
      * contention); shadow items are at most one per batch pixel */
     const int      gp = std::min(persistent_grid(c, kp, bs, lds), (int) nblk((size_t) nitems + nspr, (unsigned) bs));
-    const int      gs = std::min(persistent_grid(c, ks, bs, lds), (int) nblk(npix, (unsigned) bs));
+    const int      gs = std::min(persistent_grid(c, ks, bs, lds), (int) nblk((size_t) npix * (nl > 0 ? nl : 1), (unsigned) bs));
     if (GSTK && (rc = c->dev_alloc(c->wb_gstack, qbt_stack_bytes(s.maxlevel, DUAL, (gp > gs ? gp : gs) * bs))))
         return rc;
 
@@ -745,14 +775,15 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
             compact(st, PredUnsure{lists, lstride, q + Q_NSEL, (const uint8_t*) c->wb_lstat.p, npix}, npix, nl,
                     (uint32_t*) c->wb_bcnt.p, q + Q_FB, 1, (uint32_t*) c->wb_fb.p, npix);
         }
-        for (int l = 0; l < nl; l++) {
-            /* exact mode: every shaded pixel; surrogate mode: this light's undecided ones */
-            const uint32_t* li = surrogate ? (const uint32_t*) c->wb_fb.p + (size_t) l * npix : lists + l * lstride;
-            const unsigned* ln = surrogate ? q + Q_FB + l : q + Q_NSEL;
+        if (nl > 0) {
+            /* exact mode: every shaded pixel per light; surrogate mode: each
+             * light's undecided ones */
+            const uint32_t* li  = surrogate ? (const uint32_t*) c->wb_fb.p : lists;
+            const size_t    lst = surrogate ? npix : lstride;
+            const unsigned* ln  = surrogate ? q + Q_FB : q + Q_NSEL;
             c->prof_mark_begin(QBT_PROF_SHADOW);
-            ks<<<gs, bs, lds, st>>>(s, L.p[l][0], L.p[l][1], L.p[l][2], l, li, ln, po.isp,
-                                    (uint8_t*) c->wb_lstat.p + (size_t) l * npix, (uint8_t*) c->wb_disc.p, q,
-                                    refill_sh, gstk, po.exp, cnt);
+            ks<<<gs, bs, lds, st>>>(s, L, li, lst, ln, surrogate ? 1 : 0, po.isp, (uint8_t*) c->wb_lstat.p, npix,
+                                    (uint8_t*) c->wb_disc.p, q, refill_sh, gstk, po.exp, cnt);
             c->prof_mark_end();
         }
         if (collect && (err = qbt_collect_batch(c, s, DUAL, L, lists, lstride, q + Q_NSEL, npix))) return err;
@@ -764,7 +795,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     if (!use_graph) {
         if ((rc = enqueue())) return rc;
         HIPCHK(hipGetLastError());
-        c->last.launches += 6 + nl + (reorder ? 3 : 0) + (surrogate ? 4 : 0);
+        c->last.launches += 6 + (nl > 0 ? 1 : 0) + (reorder ? 3 : 0) + (surrogate ? 4 : 0);
         return QBT_OK;
     }
 

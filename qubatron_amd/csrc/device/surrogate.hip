@@ -583,6 +583,9 @@ __global__ void k_collect(SceneDev s, LightsDev L, const uint32_t* __restrict__ 
         l  = (int) (qi / nq);
         gp = lists[(size_t) l * lstride + (qi - (uint64_t) l * nq)];
         ls = lstat[(size_t) l * npix + gp];
+        /* a light after one that discarded the fragment does not count (its
+         * ray may have been traced: the lights share one launch) */
+        for (int j = 0; j < l; j++) ls = lstat[(size_t) j * npix + gp] == LS_DISCARD ? (uint8_t) LS_SKIPPED : ls;
     }
     /* one slot reservation per wave (a per-sample atomic on one counter
      * would queue every sample behind the others) */
