@@ -134,8 +134,11 @@ def summarize_pmc(dirs, out_path):
                     if "k_copy16" in name:
                         kern = "copy"
                     m = re.search(r"\b(k_render|k_primary|k_shadow)<([^>]*)>", name)
-                    if m and m.group(2).split(",")[-1].strip() == "false":  # not the counting variant
-                        kern = m.group(1)
+                    if m:  # not the counting variant: COUNT is k_render's 3rd, the others' 4th argument
+                        targs = [a.strip() for a in m.group(2).split(",")]
+                        ci = 2 if m.group(1) == "k_render" else 3
+                        if len(targs) > ci and targs[ci] == "false":
+                            kern = m.group(1)
                     if kern is None:
                         continue
                     key = (kern, row["Counter_Name"])
