@@ -180,7 +180,7 @@ int qbt_synchronize(qbt_ctx* ctx);
  * pair on the context stream (up to 4096 launches). qbt_profile_end
  * synchronises and reports launches and their durations in milliseconds. */
 #define QBT_PROF_PRIMARY 0   /* fused render kernel, primary kernel, ray batches */
-#define QBT_PROF_SHADOW 1    /* per-light shadow kernel of the wavefront path */
+#define QBT_PROF_SHADOW 1    /* shadow kernel of the wavefront path (all lights) */
 #define QBT_PROF_SURROGATE 2 /* surrogate decision kernel */
 typedef struct qbt_profile {
     uint32_t launches;

@@ -214,6 +214,57 @@ def test_surrogate_render(gpu_ctx):
     assert mis_t * frac_u < 0.5 * mis_u * frac_t or mis_u > 0.1, stats
 
 
+def test_surrogate_render_three_lights(gpu_ctx):
+    """Three lights with an untrained network and a band at its 30% and 70%
+    quantiles: every light keeps a different set of undecided queries (the
+    fallback lists differ in length and content), and the one shadow launch
+    serves all three lists. Per light: undecided queries equal the exact
+    render, decided ones follow the network's probability."""
+    sc, nrm, col, nodes = _small_scene()
+    _upload(gpu_ctx, nodes, col, nrm)
+    lights = np.array([sc.lights[0], (300.0, 1500.0, 400.0), (1400.0, 900.0, 1500.0)], np.float32)
+    pos, ang = sc.poses(3, seed=21)
+    views = [q.view(pos[i], ang[i], 160, 120, sc.basecube, DEPTH) for i in range(3)]
+    gpu_ctx.surrogate_init(11)
+    try:
+        _three_lights(gpu_ctx, sc, nrm, lights, views)
+    finally:
+        gpu_ctx.surrogate_set_band(0.1, 0.9)
+
+
+def _three_lights(gpu_ctx, sc, nrm, lights, views):
+    checked = 0
+    for vi, v in enumerate(views):
+        print(f"[case] three-light surrogate render, view {vi}", flush=True)
+        gpu_ctx.render(v, lights, q.OUT_ALL)
+        ex = gpu_ctx.read_frame(0)
+        sh = ((ex["flags"] & q.PX_SHADED) != 0) & ((ex["flags"] & q.PX_DISCARD) == 0)
+        ii = np.nonzero(sh.ravel())[0]
+        isp = ex["isp"].reshape(-1, 4)[ii, :3]
+        pn = nrm[ex["hit"].ravel()[ii]]
+        pr = np.stack([gpu_ctx.surrogate_predict(isp, pn, np.repeat(lights[l:l + 1], ii.size, 0), sc.basecube)
+                       for l in range(3)])
+        lo, hi = (float(x) for x in np.quantile(pr, [0.3, 0.7]))
+        gpu_ctx.surrogate_set_band(lo, hi)
+        gpu_ctx.render(v, lights, q.OUT_ALL | q.RENDER_SURROGATE)
+        su = gpu_ctx.read_frame(0)
+        for k in ("hit", "leaf", "dyn", "isp"):
+            np.testing.assert_array_equal(su[k], ex[k])
+        undecided = []
+        for l in range(3):
+            bit = (su["shadow"].ravel()[ii] >> l) & 1
+            ebit = (ex["shadow"].ravel()[ii] >> l) & 1
+            p = pr[l]
+            eps = 1e-3 * (hi - lo) + 1e-4
+            assert np.all(bit[p < lo - eps] == 0) and np.all(bit[p > hi + eps] == 1), f"light {l}"
+            mid = (p > lo + eps) & (p < hi - eps)
+            np.testing.assert_array_equal(bit[mid], ebit[mid], err_msg=f"light {l}")
+            undecided.append(int(np.count_nonzero(mid)))
+            checked += int(mid.sum())
+        assert len(set(undecided)) > 1, undecided  # the fallback lists differ
+    assert checked > 1000
+
+
 def test_surrogate_requires_parameters(gpu_ctx):
     with q.Context(0) as c2:
         sc, nrm, col, nodes = _small_scene()
