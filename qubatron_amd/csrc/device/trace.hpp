@@ -422,11 +422,11 @@ __device__ __forceinline__ int trace_begin(const SceneDev& s, Ray& r, TraceState
  * different per-level encoding, so that backtracking never re-reads a node:
  *   ns word  first child index of the level's node (bits 0-29) | entries
  *            still pending (bits 30-31, 0..3)
- *   pw word  the entry being visited (bits 0-7), then up to three pending
- *            entries (bits 8-15, 16-23, 24-31); an entry is octet (3 bits) |
- *            source << 3 (2 bits) | rank << 5 (3 bits), rank = the child's
- *            position among the node's children, so the child is
- *            first + rank.
+ *   pw word  the entry being visited (bits 0-4), then up to three pending
+ *            entries (bits 5-9, 10-14, 15-19); an entry is octet (3 bits) |
+ *            source << 3 (2 bits); bits 24-31 hold the node's child mask, so
+ *            the child of octet o is first + (the number of mask bits below
+ *            o), counted when the entry is taken.
  * A per-lane bit mask marks the levels that still hold pending entries, so a
  * backtrack finds its level with one count-leading-zeros instead of a loop
  * over the stack.
@@ -544,8 +544,8 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
 
     /* walk the sorted candidates: an octet equal to the previous valid one
      * is replaced by its flipped octet; those whose child exists are kept,
-     * in order, as entry bytes (the word is assembled from the last one, so
-     * that the first kept one ends in the low byte) */
+     * in order, as 5-bit entries (the word is assembled from the last one, so
+     * that the first kept one ends in the low bits), the child mask on top */
     const bool v1 = w1 == w1, v2 = w2 == w2, v3 = w3 == w3;
     const int  o0 = k0 & 7;
     const int  o1 = (k1 & 7) == o0 ? ((k1 >> 5) & 7) : (k1 & 7);
@@ -553,18 +553,16 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     const int  o2 = (k2 & 7) == p1 ? ((k2 >> 5) & 7) : (k2 & 7);
     const int  p2 = v2 ? o2 : p1;
     const int  o3 = (k3 & 7) == p2 ? ((k3 >> 5) & 7) : (k3 & 7);
-    auto entry = [&](int o, int k) -> uint32_t {
-        const uint32_t rk = (uint32_t) __popc(mask & ((1u << o) - 1u));
-        return (uint32_t) o | ((uint32_t) k & 0x18u) | (rk << 5);
-    };
+    auto entry = [](int o, int k) -> uint32_t { return (uint32_t) o | ((uint32_t) k & 0x18u); };
     const bool t0 = bit_of(mask, o0);
     const bool t1 = v1 && bit_of(mask, o1);
     const bool t2 = v2 && bit_of(mask, o2);
     const bool t3 = v3 && bit_of(mask, o3);
     uint32_t   word = t3 ? entry(o3, k3) : 0u;
-    word            = t2 ? (word << 8) | entry(o2, k2) : word;
-    word            = t1 ? (word << 8) | entry(o1, k1) : word;
-    word            = t0 ? (word << 8) | entry(o0, k0) : word;
+    word            = t2 ? (word << 5) | entry(o2, k2) : word;
+    word            = t1 ? (word << 5) | entry(o1, k1) : word;
+    word            = t0 ? (word << 5) | entry(o0, k0) : word;
+    word |= mask << 24;
     const int n     = (t0 ? 1 : 0) + (t1 ? 1 : 0) + (t2 ? 1 : 0) + (t3 ? 1 : 0);
 
     /* the entry to descend into: the first kept candidate, or, backtracking
@@ -584,7 +582,8 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
         if (below == 0u) return over ? ST_ABORT : ST_MISS;
         lv                    = 31 - __clz((int) below);
         const uint32_t an     = (uint32_t) st.ns[lv * st.stride];
-        w                     = st.pw[lv * st.stride] >> 8; /* the next entry becomes the current one */
+        const uint32_t pw     = st.pw[lv * st.stride];
+        w                     = ((pw & 0xFFFFFu) >> 5) | (pw & 0xFF000000u); /* the next entry becomes current */
         st.pw[lv * st.stride] = w;
         st.ns[lv * st.stride] = (int) (an - (1u << 30));
         /* bit lv is set; clear it when this was the level's last pending
@@ -602,7 +601,7 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
         isp = src != 0 ? midplane_cell<SEM, GRID>(s, r, src, j, t.ix, t.iy, t.iz, level - j) : t.start;
     }
     const int sh = level - lv, o = (int) (w & 7u);
-    t.ns    = (int) (a + ((w >> 5) & 7u));
+    t.ns    = (int) (a + (uint32_t) __popc(__builtin_amdgcn_ubfe(w, 24u, (uint32_t) o))); /* first + rank */
     t.ix    = ((t.ix >> sh) << 1) | (o & 1);
     t.iy    = ((t.iy >> sh) << 1) | ((o >> 1) & 1);
     t.iz    = ((t.iz >> sh) << 1) | (o >> 2);
