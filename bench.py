@@ -29,7 +29,9 @@ bit-exactly against the GPU frame.
 
 More legs, reported beside the headline: the same scene at 3840x2160
 (uhd_3840x2160); the same scene with a populated dynamic tree walked in
-lockstep (dynamic_tree); and batched views (batch_views): the 90M-point scene with 4
+lockstep (dynamic_tree; with the time to rebuild a 2M- and a 10M-point
+dynamic tree from points already on the GPU, as the reference rebuilds its
+dynamic tree every frame); and batched views (batch_views): the 90M-point scene with 4
 lights, 64 views of resolutions drawn from 320x240 ... 3840x2160 at random
 poses rendered as one hipGraph-captured batch with the autotuner's tuning,
 two of its views checked bit for bit against single-view renders.
@@ -358,12 +360,32 @@ def dynamic_leg(args, q, ctx, sc, mask, copy_gbps):
     wall = time.perf_counter() - t0
     prof = ctx.profile_end()
     dual = bool(ctx.stats()["dual"])
+    # per-frame rebuild of the dynamic tree from its resident points (the
+    # reference rebuilds its dynamic tree on the CPU every frame,
+    # qubatron.c:439-452): this figure, then a 10M-point one
+    rebuild = {}
+    for npt in (m, 10000000):
+        if npt != m:
+            u2 = rng.normal(size=(npt, 3))
+            u2 /= np.linalg.norm(u2, axis=1, keepdims=True)
+            ctx.upload_points((np.array([900.0, 700.0, 900.0]) + u2 * np.array([120.0, 300.0, 120.0]))
+                              .astype(np.float32), dynamic=True)
+            del u2
+        ctx.build_octree(sc.basecube, sc.depth, npt, dynamic=True)  # warm
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            nodes_b = ctx.build_octree(sc.basecube, sc.depth, npt, dynamic=True)
+        ctx.synchronize()
+        rebuild[str(npt)] = {"ms": round((time.perf_counter() - t0) * 1e3 / K, 3), "nodes": nodes_b}
+        log(f"dynamic tree rebuild from resident points: {npt} points -> {nodes_b} nodes in "
+            f"{rebuild[str(npt)]['ms']:.2f} ms")
     ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)  # back to the static-only scene
     value = float(sum(rays)) / wall / 1e6
     km, kl = prof["kind_ms"], prof["kind_launches"]
     log(f"dynamic tree ({m} points, {dn} nodes): {value:.1f} Mrays/s, {wall * 1e3 / K:.2f} ms/frame")
     return {"value": round(value, 3), "unit": "Mrays/s", "ms_per_step": round(wall * 1e3 / K, 4), "steps": K,
-            "dynamic_points": m, "dynamic_nodes": dn, "dual_walk": dual,
+            "dynamic_points": m, "dynamic_nodes": dn, "dual_walk": dual, "rebuild_per_frame": rebuild,
             "rays_per_frame": round(float(np.mean(rays))),
             "expansions_per_frame": round(float(np.mean(exps))),
             "kernel_ms_per_frame": {"primary": round(km[KIND_PRIMARY] / K, 4),

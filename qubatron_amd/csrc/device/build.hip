@@ -30,12 +30,20 @@
 
 #pragma clang fp contract(off)
 
+/* Scratch blocks of the builder, kept between builds: a build takes about
+ * 80 scratch buffers (keys, values and groups per level), and allocating and
+ * freeing them each time cost more than the build itself for a few million
+ * points (the reference rebuilds its dynamic tree every frame). Each build
+ * carves its buffers from the blocks in order, adding a block where none
+ * fits; blocks beyond kKeepBytes are released when the build ends. */
 struct BuildState {
-    int unused = 0;
+    std::vector<DevBuf> blocks;
 };
 
 void qbt_ctx::builder_free()
 {
+    if (build)
+        for (auto& b : build->blocks) dev_free(b);
     delete build;
     build = nullptr;
 }
@@ -125,21 +133,40 @@ inline unsigned nblk(size_t n) { return (unsigned) ((n + 255) / 256); }
 
 #define HIPCHK(x) QBT_HIPCHK(x)
 
-/* scratch allocation local to one build, released at the end */
+/* one build's scratch, carved from the context's builder blocks */
+constexpr size_t kBlockBytes = (size_t) 64 << 20;
+constexpr size_t kKeepBytes  = (size_t) 2 << 30;
 struct Scratch {
-    qbt_ctx*            c;
-    std::vector<DevBuf> bufs;
-    explicit Scratch(qbt_ctx* cc) : c(cc) {}
+    qbt_ctx*    c;
+    BuildState* bs;
+    size_t      bi = 0, off = 0;
+    explicit Scratch(qbt_ctx* cc) : c(cc), bs(cc->build) {}
     ~Scratch()
     {
-        for (auto& b : bufs) c->dev_free(b);
+        /* the caller has synchronised the stream (or no kernel uses a block
+         * yet); keep the first blocks up to kKeepBytes for the next build */
+        size_t kept = 0, i = 0;
+        for (; i < bs->blocks.size() && kept + bs->blocks[i].cap <= kKeepBytes; i++) kept += bs->blocks[i].cap;
+        for (size_t j = i; j < bs->blocks.size(); j++) c->dev_free(bs->blocks[j]);
+        bs->blocks.resize(i);
     }
     void* get(size_t bytes, int* rc)
     {
+        bytes = ((bytes ? bytes : 16) + 255) & ~(size_t) 255;
+        for (; bi < bs->blocks.size(); bi++, off = 0) {
+            if (off + bytes <= bs->blocks[bi].cap) {
+                void* p = (char*) bs->blocks[bi].p + off;
+                off += bytes;
+                *rc = QBT_OK;
+                return p;
+            }
+        }
         DevBuf b;
-        *rc = c->dev_alloc(b, bytes ? bytes : 16);
+        *rc = c->dev_alloc(b, bytes > kBlockBytes ? bytes : kBlockBytes);
         if (*rc) return nullptr;
-        bufs.push_back(b);
+        bs->blocks.push_back(b);
+        bi  = bs->blocks.size() - 1;
+        off = bytes;
         return b.p;
     }
 };
@@ -156,6 +183,7 @@ extern "C" int qbt_build_octree(qbt_ctx* c, int tree, const float basecube[4], i
     hipStream_t    s = c->stream;
     const uint32_t L = (uint32_t) levels;
     int            rc = QBT_OK;
+    if (!c->build) c->build = new BuildState();
     Scratch        sc(c);
     size_t         n = npoints;
 
