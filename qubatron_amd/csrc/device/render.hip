@@ -731,7 +731,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     auto           ks     = k_shadow<CMP, DUAL, GRID, COUNT, GSTK>;
     /* resident blocks, but no more than one work item per lane: small
      * batches then start fewer waves (less launch and refill-counter
-     * contention); shadow items are at most one per batch pixel */
+     * contention); shadow items are at most one per batch pixel and light */
     const int      gp = std::min(persistent_grid(c, kp, bs, lds), (int) nblk((size_t) nitems + nspr, (unsigned) bs));
     const int      gs = std::min(persistent_grid(c, ks, bs, lds), (int) nblk((size_t) npix * (nl > 0 ? nl : 1), (unsigned) bs));
     if (GSTK && (rc = c->dev_alloc(c->wb_gstack, qbt_stack_bytes(s.maxlevel, DUAL, (gp > gs ? gp : gs) * bs))))
