@@ -80,6 +80,37 @@ __global__ void k_paths(const float* __restrict__ xyz, size_t n, float edge, int
     vals[i] = (uint32_t) i;
 }
 
+/* Nodes per depth from the sorted keys in one pass: point i starts a new
+ * node at every depth from d0 down to L, where d0 is the first octet in
+ * which its key differs from the previous one (depth 1 is the top 3 bits;
+ * an equal key starts none). bins[d0] counts them; the number of nodes at
+ * depth d is the sum of bins[1..d]. */
+__global__ void k_depth_counts(const uint64_t* __restrict__ keys, size_t n, int L, unsigned* __restrict__ bins)
+{
+    __shared__ unsigned h[32];
+    if (threadIdx.x < 32) h[threadIdx.x] = 0u;
+    __syncthreads();
+    /* grid-stride over the keys (a bounded grid: the global adds at the end
+     * are few); one LDS add per (wave, depth): most points share one bin */
+    const size_t stride = (size_t) gridDim.x * blockDim.x;
+    for (size_t base = (size_t) blockIdx.x * blockDim.x; base < n; base += stride) {
+        const size_t i  = base + threadIdx.x;
+        int          d0 = 0;
+        if (i < n) {
+            d0 = 1;
+            if (i > 0) {
+                const uint64_t x = keys[i] ^ keys[i - 1];
+                d0 = x == 0ull ? 0 : L - (63 - __clzll((long long) x)) / 3;
+            }
+        }
+        for (int d = 1; d <= L; d++) {
+            const uint64_t b = __ballot(d0 == d);
+            if ((threadIdx.x & 63) == 0 && b) atomicAdd(&h[d], (unsigned) __popcll(b));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x >= 1 && threadIdx.x <= (unsigned) L && h[threadIdx.x]) atomicAdd(&bins[threadIdx.x], h[threadIdx.x]);
+}
 __global__ void k_shift_keys(const uint64_t* __restrict__ in, size_t n, uint64_t* __restrict__ out)
 {
     size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
@@ -234,16 +265,23 @@ extern "C" int qbt_build_octree(qbt_ctx* c, int tree, const float basecube[4], i
     std::vector<uint32_t*> ppos(L + 2, nullptr); /* parent position of each group at depth d */
     std::vector<size_t>    cnt(L + 2, 0);
     if (n && L) {
+        /* the group count of every depth at once: one readback for the build */
+        unsigned* bins = (unsigned*) sc.get(64 * 4, &rc);
+        if (rc) return rc;
+        k_fill_u32v<<<1, 64, 0, s>>>(bins, 64, 0u);
+        k_depth_counts<<<std::min<size_t>(nblk(n), 1024), 256, 0, s>>>(skeys, n, (int) L, bins);
+        HIPCHK(hipGetLastError());
+        unsigned hb[64];
+        HIPCHK(hipMemcpyAsync(hb, bins, sizeof(hb), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        size_t acc = 0;
+        for (uint32_t d = 1; d <= L; d++) cnt[d] = acc += hb[d];
         ukey[L]  = (uint64_t*) sc.get(n * 8, &rc);
         first[L] = rc ? nullptr : (uint32_t*) sc.get(n * 4, &rc);
         if (rc) return rc;
         tb = tmp_bytes;
         HIPCHK(rocprim::reduce_by_key(tmp, tb, skeys, svals, n, ukey[L], first[L], ucnt, rocprim::minimum<uint32_t>(),
                                       rocprim::equal_to<uint64_t>(), s));
-        uint32_t u = 0;
-        HIPCHK(hipMemcpyAsync(&u, ucnt, 4, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        cnt[L] = u;
         for (uint32_t d = L; d > 1; d--) {
             size_t    m  = cnt[d];
             uint64_t* pk = (uint64_t*) sc.get(m * 8, &rc);
@@ -261,9 +299,6 @@ extern "C" int qbt_build_octree(qbt_ctx* c, int tree, const float basecube[4], i
             tb = tmp_bytes;
             HIPCHK(rocprim::reduce_by_key(tmp, tb, pk, first[d], m, ukey[d - 1], first[d - 1], ucnt,
                                           rocprim::minimum<uint32_t>(), rocprim::equal_to<uint64_t>(), s));
-            HIPCHK(hipMemcpyAsync(&u, ucnt, 4, hipMemcpyDeviceToHost, s));
-            HIPCHK(hipStreamSynchronize(s));
-            cnt[d - 1] = u;
         }
     }
 
