@@ -90,3 +90,21 @@ def test_build_then_render(gpu_ctx):
     o = H.oracle_scene(1).render((900.0, 900.0, 900.0), (np.pi, 0.0), 320, 240, sc.lights)
     mm = H.compare_frames(g, o)
     assert all(v == 0 for k, v in mm.items() if k != "exp"), mm
+
+
+def test_dynamic_rebuilds_reuse_scratch(gpu_ctx):
+    """Per-frame rebuilds of the dynamic tree: the builder carves its scratch
+    from blocks it keeps between builds, so builds of different sizes in a
+    row (larger, smaller, larger again) must each equal the host build."""
+    rng = np.random.default_rng(17)
+    bc = np.array([0.0, 1800.0, 1800.0, 1800.0], np.float32)
+    for n, levels in ((400000, 12), (30000, 9), (400000, 12), (120000, 11)):
+        u = rng.normal(size=(n, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+        pts = (np.array([900.0, 700.0, 900.0]) + u * np.array([120.0, 300.0, 120.0])).astype(np.float32)
+        print(f"[case] dynamic rebuild n={n} levels={levels}", flush=True)
+        gpu_ctx.upload_points(pts, dynamic=True)
+        nn = gpu_ctx.build_octree(bc, levels, n, dynamic=True)
+        g = gpu_ctx.download_octree(nn, dynamic=True)
+        assert np.array_equal(g, _host_build(pts, bc, levels))
+    gpu_ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
