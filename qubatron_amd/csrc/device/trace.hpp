@@ -500,7 +500,7 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
      * this step (in bounds, like any other), then reports the abort */
     const bool  over  = ++t.it > s.iter_cap;
     const int   level = t.level;
-    const uint2 rec   = s.stat.rec[t.ns];
+    const uint2 rec   = s.stat.rec[(uint32_t) t.ns]; /* node indices are non-negative: no sign extension */
     t.exps++;
     const uint32_t mask = rec.y; /* 0 only for the root of an empty tree: then no candidate is taken */
 
