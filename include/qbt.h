@@ -278,7 +278,9 @@ int qbt_particle_update(qbt_ctx* ctx, const float* pos, const float* spd, size_t
  * QBT_BUFFER_DYNAMIC_OCTREE) on the GPU from the first npoints points of the
  * matching POINTS buffer, giving the node array that inserting the points in
  * index order with octree_insert_point (octree.c:95-147) produces, node for
- * node. *nodes_out receives the node count (the reference's len). */
+ * node. *nodes_out receives the node count (the reference's len). The
+ * context keeps up to 2 GB of the build's scratch memory for the next build
+ * (a per-frame rebuild then allocates nothing); qbt_destroy releases it. */
 int qbt_build_octree(qbt_ctx* ctx, int tree, const float basecube[4], int levels, size_t npoints,
                      size_t* nodes_out);
 
