@@ -570,8 +570,8 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
      * ancestor that has one */
     int      lv  = level;
     uint32_t w   = word, a = rec.x;
-    float4   isp = pick_entry((int) ((word >> 3) & 3u), t.entry, az, ax, ay);
     if (n > 0) {
+        t.entry = pick_entry((int) ((word >> 3) & 3u), t.entry, az, ax, ay);
         st.pw[level * st.stride] = word;
         st.ns[level * st.stride] = rec.x | ((uint32_t) (n - 1) << 30);
         t.pend |= (n > 1 ? 1u : 0u) << level;
@@ -598,14 +598,13 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
             j--;
             src = (int) ((st.pw[j * st.stride] >> 3) & 3u);
         }
-        isp = src != 0 ? midplane_cell<SEM, GRID>(s, r, src, j, t.ix, t.iy, t.iz, level - j) : t.start;
+        t.entry = src != 0 ? midplane_cell<SEM, GRID>(s, r, src, j, t.ix, t.iy, t.iz, level - j) : t.start;
     }
     const int sh = level - lv, o = (int) (w & 7u);
     t.ns    = (int) (a + (uint32_t) __popc(__builtin_amdgcn_ubfe(w, 24u, (uint32_t) o))); /* first + rank */
     t.ix    = ((t.ix >> sh) << 1) | (o & 1);
     t.iy    = ((t.iy >> sh) << 1) | ((o >> 1) & 1);
     t.iz    = ((t.iz >> sh) << 1) | (o >> 2);
-    t.entry = isp;
     t.level = lv + 1;
     return over ? ST_ABORT : (t.level == s.maxlevel ? ST_HIT : ST_CONT);
 }
