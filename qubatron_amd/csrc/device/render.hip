@@ -123,20 +123,23 @@ struct Refill {
 /* Lanes with a ray step until it ends; the wave leaves the loop together
  * once fewer than `keep` lanes are left (wave-uniform), so that the refill
  * test, the ballots and the merge of the ray state run once per refill,
- * not once per step. */
+ * not once per step. A lane whose ray ends keeps its status in `fin` and
+ * its state as it is: the caller writes the results of all such lanes
+ * together at the next refill point (a lane that ends would otherwise run
+ * the output code with one or two lanes of its wave, in most steps). */
 __device__ __forceinline__ int keep_active(const Refill& rf, unsigned refill)
 {
     return rf.left > 0 ? 65 - (int) refill : 1; /* refill when idle lanes >= refill */
 }
 
-template <bool CMP, bool DUAL, bool GRID, class F>
+template <bool CMP, bool DUAL, bool GRID>
 __device__ __forceinline__ void step_run(const SceneDev& s, const Ray& r, const Stack& st, TraceState& t, bool& active,
-                                         int keep, F&& finish)
+                                         int keep, int& fin)
 {
     for (;;) {
         const int stt = trace_step<CMP, DUAL, false, GRID>(s, r, st, t);
         if (stt != ST_CONT) {
-            finish(stt);
+            fin    = stt;
             active = false;
             break;
         }
@@ -216,7 +219,12 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut
         }
     };
 
+    int fin = ST_CONT; /* status of a ray that ended and awaits its outputs */
     for (;;) {
+        if (fin != ST_CONT) { /* rays that ended since the last refill point, together */
+            finish(fin);
+            fin = ST_CONT;
+        }
         const uint64_t idle = __ballot(!active);
         if (rf.left > 0 && (__popcll(idle) >= (int) refill || idle == ~0ull)) {
             const uint32_t fetched = rf.fetch(&q[Q_PRIM], idle, lane);
@@ -262,7 +270,7 @@ __global__ void __launch_bounds__(256) k_primary(SceneDev s, BatchDev b, PrimOut
             if (rf.left <= 0) break;
             continue;
         }
-        if (active) step_run<CMP, DUAL, GRID>(s, r, st, t, active, keep_active(rf, refill), finish);
+        if (active) step_run<CMP, DUAL, GRID>(s, r, st, t, active, keep_active(rf, refill), fin);
     }
     n_hit   = wave_sum(n_hit);
     n_abort = wave_sum(n_abort);
@@ -491,7 +499,12 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, LightsDev L, const u
         }
     };
 
+    int fin = ST_CONT; /* status of a ray that ended and awaits its outputs */
     for (;;) {
+        if (fin != ST_CONT) { /* rays that ended since the last refill point, together */
+            finish(fin);
+            fin = ST_CONT;
+        }
         const uint64_t idle = __ballot(!active);
         if (rf.left > 0 && (__popcll(idle) >= (int) refill || idle == ~0ull)) {
             const uint32_t k = rf.fetch(&q[Q_SHADOW], idle, lane);
@@ -534,7 +547,7 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, LightsDev L, const u
             if (rf.left <= 0) break;
             continue;
         }
-        if (active) step_run<CMP, DUAL, GRID>(s, r, st, t, active, keep_active(rf, refill), finish);
+        if (active) step_run<CMP, DUAL, GRID>(s, r, st, t, active, keep_active(rf, refill), fin);
     }
     n_ray   = wave_sum(n_ray);
     n_abort = wave_sum(n_abort);
