@@ -9,8 +9,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-# QBT_LIBDIR selects another build of the two libraries (A/B measurements)
-LIBDIR = os.environ.get("QBT_LIBDIR") or os.path.join(HERE, "lib")
+LIBDIR = os.path.join(HERE, "lib")
 HOST_LIB = os.path.join(LIBDIR, "libqbt_host.so")
 DEV_LIB = os.path.join(LIBDIR, "libqbt.so")
 
