@@ -40,7 +40,7 @@ def test_build_special_points(gpu_ctx):
     base = rng.uniform(0, 1800, (5000, 3)).astype(np.float32)
     pts = np.concatenate([base, base[:1000], np.array([[1800, 1800, 1800], [0, 0, 0], [-0.1, 5, 5],
                                                         [1799.9, -0.2, 1800]], np.float32), base[::-1]])
-    for levels in (0, 1, 3, 12, 16):
+    for levels in (0, 1, 3, 12, 16, 20, 21):  # 21: 63-bit keys, the top octet at bits 60-62
         print(f"[case] special levels={levels}", flush=True)
         try:
             h = _host_build(pts, bc, levels)
@@ -108,3 +108,43 @@ def test_dynamic_rebuilds_reuse_scratch(gpu_ctx):
         g = gpu_ctx.download_octree(nn, dynamic=True)
         assert np.array_equal(g, _host_build(pts, bc, levels))
     gpu_ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
+
+
+def test_kept_build_scratch_yields_to_the_cap():
+    """A dynamic-tree build keeps its scratch memory for the next per-frame
+    rebuild; under max_device_bytes, a later allocation that would exceed the
+    cap releases it instead of failing. The same build + render sequence under
+    a cap 16 MiB below the uncapped run's final footprint gives the same frame
+    and ends below the cap."""
+    sc = q.Scene(2, seed=0, npoints=2000000)
+    pnt, nrm, col = sc.points()
+    rng = np.random.default_rng(4)
+    m = 3000000
+    u = rng.normal(size=(m, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    dp = (np.array([900.0, 700.0, 900.0]) + u * np.array([120.0, 300.0, 120.0])).astype(np.float32)
+    dc = rng.uniform(0, 1, (m, 3)).astype(np.float32)
+    v = q.view((900.0, 900.0, 300.0), (np.pi, 0.0), 1920, 1080, sc.basecube, sc.depth)
+
+    def run(cap):
+        with q.Context(0, max_device_bytes=cap) as c:
+            c.upload_points(pnt)
+            c.upload_model(col, nrm)
+            c.build_octree(sc.basecube, sc.depth, pnt.shape[0])
+            c.upload_points(dp, dynamic=True)
+            c.upload_model(dc, u.astype(np.float32), dynamic=True)
+            c.build_octree(sc.basecube, sc.depth, m, dynamic=True)
+            held = c.device_bytes()
+            c.render(v, sc.lights, q.OUT_DEFAULT | q.OUT_FLAGS)
+            return held, c.device_bytes(), c.read_frame(0)
+
+    print("[case] uncapped", flush=True)
+    held0, end0, f0 = run(0)
+    cap = end0 - (16 << 20)
+    print(f"[case] cap {cap} (uncapped: {held0} after the builds, {end0} after the render)", flush=True)
+    held1, end1, f1 = run(cap)
+    assert held1 == held0  # the dynamic build kept its scratch under the cap too
+    assert end1 <= cap
+    for k in f0:
+        assert np.array_equal(f0[k], f1[k]), k
+    assert np.count_nonzero(f0["hit"] >= 0) > 100000

@@ -178,6 +178,38 @@ def test_camera_outside_and_discard(ctx):
         _assert_same(g, o, f"outside {cam}")
 
 
+@pytest.mark.parametrize("order", ["discard-second", "discard-first", "discard-middle-of-three"])
+def test_shadow_discard_by_any_light(ctx, order):
+    """A shadow ray that the root rejects discards the fragment
+    (octree_fsh.c:195,198): shading stops at the first such light, later
+    lights are not traced (octree_fsh.c:431-434 run per light in order).
+    Scene: a layer of leaves under the cube's top face, seen from above, so
+    every entry point has y = 1800 exactly; a light at y = 1800 outside the
+    cube sends rays parallel to that face, which no face test accepts. The
+    discarding light sits first, second, or between two others; the frame
+    and the expansion counts must not depend on which rays end first."""
+    levels = 8
+    u = 1800.0 / 2 ** levels
+    g = (np.arange(40, 216) + 0.5) * u
+    xx, zz = np.meshgrid(g, g)
+    pts = np.stack([xx.ravel(), np.full(xx.size, 1800.0 - 0.5 * u), zz.ravel()], 1).astype(np.float32)
+    nrm = np.tile(np.array([[0, 1, 0]], np.float32), (pts.shape[0], 1))
+    col = np.random.default_rng(3).uniform(0.2, 1.0, pts.shape).astype(np.float32)
+    bc = np.array([0.0, 1800.0, 1800.0, 1800.0], np.float32)
+    nodes = q.octree_from_points(pts, bc, levels).nodes()
+    _upload_scene(ctx, nodes, col, nrm)
+    osc = po.OracleScene(nodes, col, nrm, bc, levels)
+    l_in, l_in2, l_disc = [900.0, 1000.0, 900.0], [400.0, 1300.0, 1200.0], [-300.0, 1800.0, 900.0]
+    lights = {"discard-second": [l_in, l_disc], "discard-first": [l_disc, l_in],
+              "discard-middle-of-three": [l_in, l_disc, l_in2]}[order]
+    for rep in range(3):
+        gg, o = _render_both(ctx, osc, (900.0, 2700.0, 1500.0), (0.0, -1.1), 96, 80, lights, bc, levels)
+        _assert_same(gg, o, f"{order} #{rep}")
+        assert ctx.stats()["expansions"] == int(o["exp"].sum())
+    leaf, disc = (o["flags"] & po.PX_LEAF) != 0, (o["flags"] & po.PX_DISCARD) != 0
+    assert np.count_nonzero(leaf & disc) > 2000  # every leaf pixel: its shadow ray discards
+
+
 @pytest.mark.parametrize("maxlevel", [0, 1, 2, 5])
 def test_shallow_trees(ctx, maxlevel):
     sc, pnt, nrm, col, nodes = H.scene(1, npoints=5000)

@@ -1,0 +1,124 @@
+"""The learned shadow surrogate at the config-5 workload, through the C ABI:
+the config-2 scene (10M points, depth 12, one light), trained online on the
+GPU by the library's own forward, backward and Adam kernels with 16M shadow
+samples collected from 256 exact 1920x1080 frames, then evaluated on held-out
+1920x1080 frames with exact-trace fallback for predictions in [0.1, 0.9].
+
+Checked on every evaluated frame: primary outputs (hit, leaf, entry point)
+identical to the exact render; every query the surrogate leaves undecided
+traced exactly (shadow bit and colour equal to the exact frame); every
+decided query following the network's own probability (computed again by
+qbt_surrogate_predict from the hit point, the hit leaf's normal and the
+light). The exact frame itself equals the oracle on one of them. Reported:
+shadow-bit mismatch, decided fraction, and the per-frame time of the shadow
+side (fallback traces + decision pass) against the exact shadow pass on the
+same frames.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+import qubatron_amd as q
+from qubatron_amd import surrogate as S
+from oracle import pyoracle as po
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+NT = min(16, os.cpu_count() or 1)
+W, H_ = 1920, 1080
+KIND_PRIMARY, KIND_SHADOW, KIND_SURROGATE = 0, 1, 2
+
+
+@pytest.fixture(scope="module")
+def trained():
+    t0 = time.time()
+    sc = q.Scene(5)
+    pnt, nrm, col = sc.points(nthreads=NT)
+    ctx = q.Context(0)
+    ctx.upload_points(pnt)
+    ctx.upload_model(col, nrm)
+    ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
+    n = ctx.build_octree(sc.basecube, sc.depth, sc.npoints)
+    print(f"[scene] config 5: {sc.npoints} points, {n} nodes in {time.time() - t0:.1f}s", flush=True)
+    tpos, tang = sc.poses(256, seed=7920)
+    tviews = [q.view(tpos[i], tang[i], W, H_, sc.basecube, sc.depth) for i in range(256)]
+    ctx.render(tviews[0], sc.lights, q.OUT_FLAGS)
+    queries = ctx.stats()["shadow_rays"]
+    stride = max(1, int(round(256 * queries / (1 << 24))))
+    t0 = time.time()
+    info = S.train_online(ctx, tviews, sc.lights, sc.basecube, nsamples=1 << 24, stride=stride,
+                          steps_per_frame=32, batch=4096, lr=2e-3, seed=0, final_steps=2000)
+    ctx.synchronize()
+    print(f"[surrogate] trained in {time.time() - t0:.1f}s: {info}", flush=True)
+    yield sc, ctx, pnt, nrm, col, n, info
+    ctx.close()
+
+
+def _shadow_side_ms(ctx, views, lights, mask):
+    """per-frame traversal time of the shadow side: shadow kernel + decision pass"""
+    for v in views[:1]:
+        ctx.render(v, lights, mask)
+    ctx.profile_begin()
+    for v in views:
+        ctx.render(v, lights, mask)
+    p = ctx.profile_end()
+    k = len(views)
+    return (p["kind_ms"][KIND_SHADOW] + p["kind_ms"][KIND_SURROGATE]) / k, p["kind_ms"][KIND_PRIMARY] / k
+
+
+def test_config5_training_collected_16m_samples(trained):
+    sc, ctx, pnt, nrm, col, n, info = trained
+    assert info["samples"] == 1 << 24
+    assert info["frames"] == 256
+    f = info["lit_fraction"]
+    entropy = -(f * np.log(f) + (1 - f) * np.log(1 - f))  # loss of the best constant predictor
+    assert info["loss_last"] < 0.8 * entropy, info
+
+
+def test_config5_surrogate_frames(trained):
+    sc, ctx, pnt, nrm, col, n, info = trained
+    lights = np.asarray(sc.lights, np.float32)
+    pos, ang = sc.poses(6, seed=1)
+    views = [q.view(pos[i], ang[i], W, H_, sc.basecube, sc.depth) for i in range(6)]
+    ctx.surrogate_set_band(0.1, 0.9)
+    bad = tot = pred = 0
+    for i, v in enumerate(views):
+        print(f"[case] config 5 surrogate frame {i}", flush=True)
+        ctx.render(v, lights, q.OUT_ALL)
+        ex = ctx.read_frame(0)
+        ctx.render(v, lights, q.OUT_ALL | q.RENDER_SURROGATE)
+        su = ctx.read_frame(0)
+        pred += ctx.stats()["predicted"]
+        for k in ("hit", "leaf", "isp"):
+            np.testing.assert_array_equal(su[k], ex[k])
+        keep = (su["flags"] & q.PX_PREDICTED) == 0
+        np.testing.assert_array_equal(su["shadow"][keep], ex["shadow"][keep])
+        np.testing.assert_array_equal(su["rgba8"][keep], ex["rgba8"][keep])
+        sh = ((ex["flags"] & q.PX_SHADED) != 0) & ((ex["flags"] & q.PX_DISCARD) == 0)
+        ii = np.nonzero(sh.ravel())[0]
+        pr = ctx.surrogate_predict(ex["isp"].reshape(-1, 4)[ii, :3], nrm[ex["hit"].ravel()[ii]],
+                                   np.repeat(lights[:1], ii.size, 0), sc.basecube)
+        bit = su["shadow"].ravel()[ii] & 1
+        assert np.all(bit[pr < 0.099] == 0) and np.all(bit[pr > 0.901] == 1)
+        mid = (pr > 0.101) & (pr < 0.899)
+        np.testing.assert_array_equal(bit[mid], ex["shadow"].ravel()[ii][mid] & 1)
+        b, t = S.shadow_mismatch(ex, su, 1)
+        bad, tot = bad + b, tot + t
+        if i == 0:  # the exact frame against the oracle
+            nodes = ctx.download_octree(n)
+            o = po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth).render(pos[i], ang[i], W, H_, lights,
+                                                                               nthreads=NT)
+            mm = H.compare_frames(ex, o)
+            assert all(x == 0 for x in mm.values()), mm
+            del nodes, o
+    mismatch, frac = bad / tot, pred / tot
+    exact_shadow, exact_prim = _shadow_side_ms(ctx, views, lights, q.OUT_DEFAULT)
+    sur_shadow, sur_prim = _shadow_side_ms(ctx, views, lights, q.OUT_DEFAULT | q.RENDER_SURROGATE)
+    print(f"[surrogate] config 5: mismatch {mismatch:.4f}, decided {frac:.3f}; per frame: primary "
+          f"{exact_prim:.3f} / {sur_prim:.3f} ms, shadow side exact {exact_shadow:.3f} ms, with the network "
+          f"{sur_shadow:.3f} ms", flush=True)
+    assert tot > 1_000_000
+    assert mismatch < 0.02 and frac > 0.2, (mismatch, frac)
