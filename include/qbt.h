@@ -155,6 +155,7 @@ int qbt_read_frame(qbt_ctx* ctx, int view, qbt_frame* out);
  * larger frame); members not rendered are NULL. */
 int qbt_frame_device(qbt_ctx* ctx, int view, qbt_frame* out);
 
+
 typedef struct qbt_stats {
     double   kernel_ms;      /* device time of the last qbt_render / qbt_trace_rays */
     double   trace_ms;       /* time of the traversal kernel(s) only */
@@ -278,9 +279,12 @@ int qbt_particle_update(qbt_ctx* ctx, const float* pos, const float* spd, size_t
  * QBT_BUFFER_DYNAMIC_OCTREE) on the GPU from the first npoints points of the
  * matching POINTS buffer, giving the node array that inserting the points in
  * index order with octree_insert_point (octree.c:95-147) produces, node for
- * node. *nodes_out receives the node count (the reference's len). The
- * context keeps up to 2 GB of the build's scratch memory for the next build
- * (a per-frame rebuild then allocates nothing); qbt_destroy releases it. */
+ * node. *nodes_out receives the node count (the reference's len). A
+ * dynamic-tree build keeps up to 2 GB of its scratch memory for the next
+ * build (a per-frame rebuild then allocates nothing); a static build releases
+ * its scratch when it ends. Kept scratch counts against max_device_bytes; it
+ * is released when another allocation of the context would exceed that cap
+ * or find the device full, and by qbt_destroy. */
 int qbt_build_octree(qbt_ctx* ctx, int tree, const float basecube[4], int levels, size_t npoints,
                      size_t* nodes_out);
 

@@ -30,14 +30,18 @@
 
 #pragma clang fp contract(off)
 
-/* Scratch blocks of the builder, kept between builds: a build takes about
- * 80 scratch buffers (keys, values and groups per level), and allocating and
- * freeing them each time cost more than the build itself for a few million
- * points (the reference rebuilds its dynamic tree every frame). Each build
- * carves its buffers from the blocks in order, adding a block where none
- * fits; blocks beyond kKeepBytes are released when the build ends. */
+/* Scratch blocks of the builder: a build takes about 80 scratch buffers
+ * (keys, values and groups per level), and allocating and freeing them each
+ * time cost more than the build itself for a few million points (the
+ * reference rebuilds its dynamic tree every frame). Each build carves its
+ * buffers from the blocks in order, adding a block where none fits. A
+ * dynamic-tree build keeps its blocks, up to kKeepBytes, for the next
+ * per-frame rebuild; a static build releases them when it ends. Kept blocks
+ * are idle between builds, and qbt_ctx::dev_alloc releases them (trim)
+ * before it would report QBT_ENOMEM. */
 struct BuildState {
     std::vector<DevBuf> blocks;
+    bool                active = false; /* a build is carving from the blocks */
 };
 
 void qbt_ctx::builder_free()
@@ -46,6 +50,15 @@ void qbt_ctx::builder_free()
         for (auto& b : build->blocks) dev_free(b);
     delete build;
     build = nullptr;
+}
+
+bool qbt_ctx::builder_trim()
+{
+    if (!build || build->active || build->blocks.empty()) return false;
+    (void) hipStreamSynchronize(stream);
+    for (auto& b : build->blocks) dev_free(b);
+    build->blocks.clear();
+    return true;
 }
 
 namespace {
@@ -171,15 +184,18 @@ struct Scratch {
     qbt_ctx*    c;
     BuildState* bs;
     size_t      bi = 0, off = 0;
-    explicit Scratch(qbt_ctx* cc) : c(cc), bs(cc->build) {}
+    bool        keep; /* keep the blocks for the next build (dynamic tree) */
+    Scratch(qbt_ctx* cc, bool keep_blocks) : c(cc), bs(cc->build), keep(keep_blocks) { bs->active = true; }
     ~Scratch()
     {
         /* the caller has synchronised the stream (or no kernel uses a block
-         * yet); keep the first blocks up to kKeepBytes for the next build */
+         * yet); a dynamic build keeps the first blocks up to kKeepBytes */
         size_t kept = 0, i = 0;
-        for (; i < bs->blocks.size() && kept + bs->blocks[i].cap <= kKeepBytes; i++) kept += bs->blocks[i].cap;
+        if (keep)
+            for (; i < bs->blocks.size() && kept + bs->blocks[i].cap <= kKeepBytes; i++) kept += bs->blocks[i].cap;
         for (size_t j = i; j < bs->blocks.size(); j++) c->dev_free(bs->blocks[j]);
         bs->blocks.resize(i);
+        bs->active = false;
     }
     void* get(size_t bytes, int* rc)
     {
@@ -215,7 +231,7 @@ extern "C" int qbt_build_octree(qbt_ctx* c, int tree, const float basecube[4], i
     const uint32_t L = (uint32_t) levels;
     int            rc = QBT_OK;
     if (!c->build) c->build = new BuildState();
-    Scratch        sc(c);
+    Scratch        sc(c, t == 1);
     size_t         n = npoints;
 
     /* 1. paths and stable sort */

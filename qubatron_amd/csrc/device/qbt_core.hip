@@ -122,14 +122,8 @@ __global__ void __launch_bounds__(256) k_copy16(const int4* __restrict__ src, in
 /* per-view, per-light sprite trace (octree_fsh.c:452-459): the ray from the
  * camera to the light is the same for every pixel of a view. */
 template <bool CMP, bool DUAL, bool GRID>
-__global__ void k_sprite(SceneDev s, BatchDev b, int* __restrict__ sprite)
+__device__ __forceinline__ int sprite_flags(const SceneDev& s, const ViewDev& v, int l, const Stack& st)
 {
-    extern __shared__ uint32_t lds[];
-    const int                  l  = threadIdx.x;
-    const Stack                st = make_stack(lds, s.maxlevel, DUAL);
-    const ViewDev&             v  = b.views[blockIdx.x];
-    int* __restrict__          out = sprite + blockIdx.x * QBT_MAX_LIGHTS_RENDER;
-    if (l >= v.nlights) return;
     Ray r = {v.cam[0], v.cam[1], v.cam[2], v.light[l][0] - v.cam[0], v.light[l][1] - v.cam[1],
              v.light[l][2] - v.cam[2]};
     Hit h;
@@ -140,7 +134,18 @@ __global__ void k_sprite(SceneDev s, BatchDev b, int* __restrict__ sprite)
     if (stt == ST_DISCARD) f |= 1;
     if (resv / r.dx > 1.0f) f |= 2;
     if (stt == ST_ABORT) f |= 4;
-    out[l] = f;
+    return f;
+}
+
+template <bool CMP, bool DUAL, bool GRID>
+__global__ void k_sprite(SceneDev s, BatchDev b, int* __restrict__ sprite)
+{
+    extern __shared__ uint32_t lds[];
+    const int                  l  = threadIdx.x;
+    const Stack                st = make_stack(lds, s.maxlevel, DUAL);
+    const ViewDev&             v  = b.views[blockIdx.x];
+    if (l >= v.nlights) return;
+    sprite[blockIdx.x * QBT_MAX_LIGHTS_RENDER + l] = sprite_flags<CMP, DUAL, GRID>(s, v, l, st);
 }
 
 template <bool DUAL, bool GRID, bool COUNT>
@@ -372,6 +377,7 @@ __global__ void __launch_bounds__(256) k_particles(SceneDev s, const float* pos,
 
 static char g_last_hip[256];
 
+
 void qbt_set_hip_error(hipError_t e, const char* what, const char* file, int line)
 {
     snprintf(g_last_hip, sizeof(g_last_hip), "%s: %s (%s:%d)", hipGetErrorString(e), what, file, line);
@@ -383,7 +389,8 @@ int qbt_ctx::dev_alloc(DevBuf& b, size_t bytes)
 {
     if (bytes <= b.cap) return QBT_OK;
     size_t grow = bytes - b.cap;
-    if (max_bytes && held + grow > max_bytes) return QBT_ENOMEM;
+    /* over the cap: first release the builder's idle kept blocks */
+    if (max_bytes && held + grow > max_bytes && !(builder_trim() && held + grow <= max_bytes)) return QBT_ENOMEM;
     if (b.p) {
         HIPCHK(hipFree(b.p));
         held -= b.cap;
@@ -393,8 +400,12 @@ int qbt_ctx::dev_alloc(DevBuf& b, size_t bytes)
     hipError_t e = hipMalloc(&b.p, bytes);
     if (e != hipSuccess) {
         (void) hipGetLastError();
-        b.p = nullptr;
-        return QBT_ENOMEM;
+        e = builder_trim() ? hipMalloc(&b.p, bytes) : e; /* the device is full: the same, then once more */
+        if (e != hipSuccess) {
+            (void) hipGetLastError();
+            b.p = nullptr;
+            return QBT_ENOMEM;
+        }
     }
     b.cap = bytes;
     held += bytes;
@@ -472,7 +483,7 @@ extern "C" void qbt_destroy(qbt_ctx* c)
     c->graph_free();
     for (DevBuf* b : {&c->wb_isp, &c->wb_ms, &c->wb_md, &c->wb_leaf, &c->wb_flags, &c->wb_disc, &c->wb_lstat,
                       &c->wb_exp, &c->wb_cand, &c->wb_items, &c->wb_q, &c->wb_lists, &c->wb_fb, &c->wb_gstack,
-                      &c->wb_bcnt,
+                      &c->wb_bcnt, &c->wb_lexp,
                       &c->bt_tab, &c->valflag})
         c->dev_free(*b);
     for (int k = 0; k < qbt_ctx::kTabSlots; k++) {
@@ -675,7 +686,7 @@ extern "C" int qbt_download(qbt_ctx* c, int buftype, void* host, size_t start, s
 }
 
 /* Check that the child indices of dirty node ranges stay inside the tree. */
-int qbt_ctx::validate_tree(int t)
+int qbt_ctx::validate_tree(int t, unsigned* flag)
 {
     TreeBuf& tb = tree[t];
     if (tb.validated) return QBT_OK;
@@ -684,18 +695,20 @@ int qbt_ctx::validate_tree(int t)
         return QBT_OK;
     }
     size_t lo = tb.full_check ? 0 : tb.dirty_lo, hi = tb.full_check ? tb.n : (tb.dirty_hi < tb.n ? tb.dirty_hi : tb.n);
-    DevBuf& flag = valflag;
-    if (!flag.p) {
-        int rc = dev_alloc(flag, 256);
-        if (rc) return rc;
+    if (!flag) {
+        if (!valflag.p) {
+            int rc = dev_alloc(valflag, 256);
+            if (rc) return rc;
+        }
+        flag = (unsigned*) valflag.p;
     }
-    k_fill_u32<<<1, 64, 0, stream>>>((uint32_t*) flag.p, 1, 0u);
+    k_fill_u32<<<1, 64, 0, stream>>>((uint32_t*) flag, 1, 0u);
     if (hi > lo)
         k_validate<<<blocks_for(hi - lo, 256), 256, 0, stream>>>((const int4*) tb.child.p, lo, hi, (int) tb.n,
-                                                                 (unsigned int*) flag.p);
+                                                                 (unsigned int*) flag);
     HIPCHK(hipGetLastError());
     unsigned int nbad = 0;
-    HIPCHK(hipMemcpyAsync(&nbad, flag.p, 4, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(&nbad, flag, 4, hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     if (nbad) return QBT_ETREE;
     tb.validated  = true;
@@ -722,7 +735,7 @@ bool qbt_grid_exact(const float b[4], int L)
 }
 
 int qbt_ctx::scene_for(const float basecube[4], int maxlevel, SceneDev& s, bool& dual, bool& grid, bool& cmp,
-                       bool allow_compact)
+                       bool allow_compact, unsigned* valflag_dev)
 {
     if (maxlevel < 0 || maxlevel > kMaxLevel) return QBT_EINVAL;
     if (!(basecube[3] > 0.0f) || !isfinite(basecube[0]) || !isfinite(basecube[1]) || !isfinite(basecube[2]) ||
@@ -731,8 +744,8 @@ int qbt_ctx::scene_for(const float basecube[4], int maxlevel, SceneDev& s, bool&
     if (!(ldexpf(basecube[3], -maxlevel) >= FLT_MIN)) return QBT_EINVAL;
     if (tree[0].n == 0) return QBT_EINVAL; /* no static octree uploaded */
     int rc;
-    if ((rc = validate_tree(0))) return rc;
-    if ((rc = validate_tree(1))) return rc;
+    if ((rc = validate_tree(0, valflag_dev))) return rc;
+    if ((rc = validate_tree(1, valflag_dev))) return rc;
     dual = tree[1].n > 1 || (maxlevel == 0 && tree[1].n >= 1);
     grid = qbt_grid_exact(basecube, maxlevel);
     /* compact layout: rebuilt lazily after the node arrays change */

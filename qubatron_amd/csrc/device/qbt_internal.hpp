@@ -161,6 +161,7 @@ struct qbt_ctx {
     DevBuf       wb_fb;    /* surrogate: per-light lists of the rays traced exactly */
     DevBuf       wb_gstack; /* traversal stacks in global memory (tuning stack = 1) */
     DevBuf       wb_bcnt;   /* per-block counts of the stream compactions */
+    DevBuf       wb_lexp;   /* counting renders: shadow expansions per light and pixel */
     DevBuf       bt_tab; /* batch tables: views, item and pixel offsets, frames */
     static constexpr int kTabSlots = 4;
     void*        tab_host[kTabSlots] = {};      /* pinned staging of the batch tables */
@@ -208,10 +209,12 @@ struct qbt_ctx {
 
     int      dev_alloc(DevBuf& b, size_t bytes);
     void     dev_free(DevBuf& b);
-    int      validate_tree(int t);
+    /* flag: a device word for the check's result (nullptr: the context's own,
+     * allocated on first use) */
+    int      validate_tree(int t, unsigned* flag = nullptr);
     int      relayout(int t);
     int      scene_for(const float basecube[4], int maxlevel, qbt::SceneDev& s, bool& dual, bool& grid, bool& cmp,
-                       bool allow_compact);
+                       bool allow_compact, unsigned* valflag_dev = nullptr);
     bool     use_compact = true; /* traverse the compact layout when both trees have one */
     int      ensure_frame(int view, int w, int h, unsigned mask);
     FrameDev frame_dev(int view);
@@ -228,6 +231,9 @@ struct qbt_ctx {
     void     graph_free();
     void     mlp_free();
     void     builder_free();
+    /* release the builder blocks kept between builds (none while a build
+     * runs); true if any were held */
+    bool     builder_trim();
 };
 
 /* surrogate.hip: the MLP decision pass of a batch. The shaded pixels of

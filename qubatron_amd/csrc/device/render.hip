@@ -20,6 +20,7 @@
  *              shaded pixel's entry point (:431-434), light after light in
  *              one work list; writes the per-light status; pixels that an
  *              earlier light discarded are skipped (the shader stops there)
+ *              once that discard is visible
  *   k_shade    per pixel of the batch: attributes, shading (:436-449),
  *              sprite (:452-459), outputs into each view's frame
  *
@@ -454,16 +455,18 @@ __global__ void k_oct_scan(const unsigned* __restrict__ cnt, unsigned* __restric
  * light 0's list, then light 1's, ... (light l: lists + l * lstride, count
  * counts[l * cstride]), so the tail of one light's rays overlaps the next
  * light's instead of draining the GPU once per light. Each lane carries its
- * ray's light. A fragment that an earlier light discarded is skipped when
- * that light's ray has already ended; otherwise its ray is traced as well
- * (shading stops at the first discarding light, so the frame is the same;
- * the sample collector applies the same rule). */
+ * ray's light. disc[pixel] holds the lowest light whose ray discarded the
+ * fragment (atomicMin; ~0 none). A light's ray is skipped when a lower light
+ * has already discarded; otherwise it is traced as well: k_shade stops at the
+ * first discarding light, for the colour, the bits and the expansion count
+ * (lexp: per light, counting renders), so the frame does not depend on the
+ * order in which rays end (the sample collector applies the same rule). */
 template <bool CMP, bool DUAL, bool GRID, bool COUNT, bool GSTK>
 __global__ void __launch_bounds__(256) k_shadow(SceneDev s, LightsDev L, const uint32_t* __restrict__ lists,
                                                 size_t lstride, const unsigned* __restrict__ counts, int cstride,
                                                 const float4* __restrict__ isp, uint8_t* __restrict__ lstat,
-                                                size_t npix, uint8_t* __restrict__ disc, unsigned* __restrict__ q,
-                                                unsigned refill, uint32_t* gstack, int* __restrict__ exp,
+                                                size_t npix, unsigned* __restrict__ disc, unsigned* __restrict__ q,
+                                                unsigned refill, uint32_t* gstack, int* __restrict__ lexp,
                                                 unsigned long long* __restrict__ counters)
 {
     extern __shared__ uint32_t lds[];
@@ -478,13 +481,13 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, LightsDev L, const u
     Ray         r      = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float4      ip     = make_float4(0.f, 0.f, 0.f, 0.f);
     TraceState  t;
-    unsigned    n_exp = 0, n_abort = 0, n_ray = 0;
+    unsigned    n_abort = 0, n_ray = 0;
 
     auto finish = [&](int stt) {
         uint8_t ls;
         if (stt == ST_DISCARD) {
-            ls       = LS_DISCARD;
-            disc[gp] = 1;
+            ls = LS_DISCARD;
+            atomicMin(&disc[gp], (unsigned) lt);
         } else {
             if (stt == ST_ABORT) n_abort++;
             float4 li  = stt == ST_HIT ? t.entry : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -493,10 +496,7 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, LightsDev L, const u
             ls         = (15.0f < sqr) ? LS_OCCLUDED : LS_LIT; /* step(sqr, 15.0) */
         }
         lstat[(size_t) lt * npix + gp] = ls;
-        if (COUNT) {
-            atomicAdd(&exp[gp], (int) t.exps);
-            n_exp += t.exps;
-        }
+        if (COUNT) lexp[(size_t) lt * npix + gp] = (int) t.exps;
     };
 
     int fin = ST_CONT; /* status of a ray that ended and awaits its outputs */
@@ -526,8 +526,9 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, LightsDev L, const u
                 }
                 lt = l;
                 gp = lists[(size_t) l * lstride + kk];
-                if (l > 0 && disc[gp]) {
-                    lstat[(size_t) l * npix + gp] = LS_SKIPPED; /* an earlier light discarded the fragment */
+                if (disc[gp] < (unsigned) l) { /* a lower light discarded the fragment */
+                    lstat[(size_t) l * npix + gp] = LS_SKIPPED;
+                    if (COUNT) lexp[(size_t) l * npix + gp] = 0;
                 } else {
                     ip      = isp[gp];
                     r.px    = lx;
@@ -551,20 +552,33 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, LightsDev L, const u
     }
     n_ray   = wave_sum(n_ray);
     n_abort = wave_sum(n_abort);
-    if (COUNT) n_exp = wave_sum(n_exp);
     if (lane == 0) {
         if (n_ray) atomicAdd(&counters[CNT_SHADOW], (unsigned long long) n_ray);
         if (n_abort) atomicAdd(&counters[CNT_ABORT], (unsigned long long) n_abort);
-        if (COUNT && n_exp) atomicAdd(&counters[CNT_EXP], (unsigned long long) n_exp);
     }
 }
 
-/* per batch pixel: attributes, shading, sprite, outputs (octree_fsh.c:421-463) */
-template <bool DUAL>
+/* per batch pixel: attributes, shading, sprite, outputs (octree_fsh.c:421-463);
+ * counting renders add the shadow expansions of the lights up to the first
+ * discarding one (lexp) to the pixel's count and to the counters */
+template <bool DUAL, bool COUNT>
 __global__ void __launch_bounds__(256) k_shade(SceneDev s, BatchDev b, PrimOut in, const uint8_t* __restrict__ lstat,
-                                               uint32_t npix, unsigned mask)
+                                               const int* __restrict__ lexp, uint32_t npix, unsigned mask,
+                                               unsigned long long* __restrict__ counters)
 {
     const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned       n_exp = 0;
+    if (COUNT) {
+        if (o < npix && (in.flags[o] & QBT_PX_SHADED) && !(in.flags[o] & QBT_PX_DISCARD)) {
+            const int nl = b.views[batch_view(b.pix_off, b.n, o)].nlights;
+            for (int l = 0; l < nl; l++) {
+                if (lstat[(size_t) l * npix + o] == LS_DISCARD) break;
+                n_exp += (unsigned) lexp[(size_t) l * npix + o];
+            }
+        }
+        const unsigned w = wave_sum(n_exp);
+        if ((threadIdx.x & 63) == 0 && w) atomicAdd(&counters[CNT_EXP], (unsigned long long) w);
+    }
     if (o >= npix) return;
     const int      vi     = batch_view(b.pix_off, b.n, o);
     const ViewDev& v      = b.views[vi];
@@ -649,7 +663,7 @@ __global__ void __launch_bounds__(256) k_shade(SceneDev s, BatchDev b, PrimOut i
     if (mask & QBT_OUT_RGBA8) f.rgba8[lo] = pack_rgba8(col[0], col[1], col[2], col[3]);
     if (mask & QBT_OUT_RGBAF) f.rgbaf[lo] = make_float4(col[0], col[1], col[2], col[3]);
     if (mask & QBT_OUT_ISP) f.isp[lo] = isp;
-    if (mask & QBT_OUT_EXP) f.exp[lo] = in.exp[o];
+    if (COUNT && (mask & QBT_OUT_EXP)) f.exp[lo] = in.exp[o] + (int) n_exp;
 }
 
 __global__ void k_zero_u32(unsigned* p, int n)
@@ -658,10 +672,10 @@ __global__ void k_zero_u32(unsigned* p, int n)
     if (i < n) p[i] = 0u;
 }
 
-__global__ void k_zero_u8(uint8_t* p, size_t n)
+__global__ void k_fill_u32n(unsigned* p, size_t n, unsigned v)
 {
     size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
+    if (i < n) p[i] = v;
 }
 
 inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned) ((n + bs - 1) / bs); }
@@ -706,7 +720,7 @@ int qbt_ctx::ensure_wave_bufs(size_t npix, int nlights)
     int          rc;
     const size_t nl = (size_t) (nlights > 0 ? nlights : 1);
     if ((rc = dev_alloc(wb_isp, npix * 16)) || (rc = dev_alloc(wb_ms, npix * 4)) || (rc = dev_alloc(wb_md, npix * 4)) ||
-        (rc = dev_alloc(wb_leaf, npix * 4)) || (rc = dev_alloc(wb_flags, npix)) || (rc = dev_alloc(wb_disc, npix)) ||
+        (rc = dev_alloc(wb_leaf, npix * 4)) || (rc = dev_alloc(wb_flags, npix)) || (rc = dev_alloc(wb_disc, npix * 4)) ||
         (rc = dev_alloc(wb_lstat, npix * nl)) || (rc = dev_alloc(wb_exp, npix * 4)))
         return rc;
     return QBT_OK;
@@ -732,6 +746,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     const uint32_t nb_max = ((nitems > npix ? nitems : npix) + kSelItems - 1) / kSelItems + 1;
     if ((rc = c->dev_alloc(c->wb_bcnt, (size_t) nb_max * 4 * (nl > 0 ? nl : 1)))) return rc;
     if (surrogate && (rc = c->dev_alloc(c->wb_fb, (size_t) npix * 4 * nl))) return rc;
+    if (COUNT && nl > 0 && (rc = c->dev_alloc(c->wb_lexp, (size_t) npix * 4 * nl))) return rc;
 
     const int      bs     = c->block_threads(s.maxlevel, DUAL);
     const size_t   lds    = GSTK ? 0 : qbt_stack_bytes(s.maxlevel, DUAL, bs);
@@ -771,7 +786,9 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     auto enqueue = [&]() -> int {
         int err;
         k_zero_u32<<<nblk(Q_N), 256, 0, st>>>(q, Q_N);
-        k_zero_u8<<<nblk(npix), 256, 0, st>>>((uint8_t*) c->wb_disc.p, npix);
+        k_fill_u32n<<<nblk(npix), 256, 0, st>>>((unsigned*) c->wb_disc.p, npix, ~0u);
+        if (COUNT && nl > 0) /* lights the surrogate decided add no shadow expansions */
+            k_fill_u32n<<<nblk((size_t) npix * nl), 256, 0, st>>>((unsigned*) c->wb_lexp.p, (size_t) npix * nl, 0u);
         c->prof_mark_begin(QBT_PROF_PRIMARY);
         kp<<<gp, bs, lds, st>>>(s, b, po, nitems, nspr, nl > 0 ? nl : 1, c->tune_strip, (int*) c->sprite.p, q,
                                 refill, gstk, cnt);
@@ -796,11 +813,12 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
             const unsigned* ln  = surrogate ? q + Q_FB : q + Q_NSEL;
             c->prof_mark_begin(QBT_PROF_SHADOW);
             ks<<<gs, bs, lds, st>>>(s, L, li, lst, ln, surrogate ? 1 : 0, po.isp, (uint8_t*) c->wb_lstat.p, npix,
-                                    (uint8_t*) c->wb_disc.p, q, refill_sh, gstk, po.exp, cnt);
+                                    (unsigned*) c->wb_disc.p, q, refill_sh, gstk, (int*) c->wb_lexp.p, cnt);
             c->prof_mark_end();
         }
         if (collect && (err = qbt_collect_batch(c, s, DUAL, L, lists, lstride, q + Q_NSEL, npix))) return err;
-        k_shade<DUAL><<<nblk(npix), 256, 0, st>>>(s, b, po, (const uint8_t*) c->wb_lstat.p, npix, mask);
+        k_shade<DUAL, COUNT><<<nblk(npix), 256, 0, st>>>(s, b, po, (const uint8_t*) c->wb_lstat.p,
+                                                         (const int*) c->wb_lexp.p, npix, mask, cnt);
         return QBT_OK;
     };
 

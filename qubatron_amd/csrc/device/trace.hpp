@@ -98,14 +98,19 @@ __device__ __forceinline__ float recip_or_nan(float d)
  * defaults (0, 0, 0, t = FLT_MAX), the particle shader's (0, 0, 0, 0)
  * (particle_vsh.c:64-65), and NaN in shader semantics (see below).
  *
- * The quotient is correctly rounded, as the reference's division, from
- * y = RN(1 / d_k): q = RN(a y), e = a - d_k q (exact with an FMA),
- * RN(q + e y) = RN(a / d_k) (Markstein's theorem for division with a
- * correctly rounded reciprocal). That holds while q is a normal number well
- * inside the range (no underflow or overflow in the steps); elsewhere, and
- * where y is NaN, the quotient is the division itself, on a branch that
- * rays rarely take. Everything else is selects, so that lanes of a wave
- * do not split on the axis or on d_k == 0. */
+ * The quotient is the correctly rounded one, as the reference's division,
+ * from y = RN(1 / d_k): q = RN(a y), e = RN(a - d_k q) (one FMA),
+ * t = RN(q + e y) (one FMA), a Markstein-style correction. It is taken
+ * where |d_k| is in [2^-100, 2^100] (y not NaN), |a| >= 2^-100 and |q| is in
+ * [2^-100, 2^120]: there every step's result is a normal number or exact
+ * (e is a multiple of 2^(e_a - 47), at least 2^-147), so each step scales
+ * with the exponents of a and d_k, and t is correct for all such operands
+ * exactly when it is for every pair of significands in [1, 2). All 2^46
+ * pairs are checked on the device (tests/test_gpu_quotient.py) and a sample
+ * on the host (tests/test_quotient.py). Elsewhere, and where y is NaN, the
+ * quotient is the division itself, on a branch that rays rarely take.
+ * Everything else is selects, so that lanes of a wave do not split on the
+ * axis or on d_k == 0. */
 /* component k by value (a conditional over the Ray's members would select an
  * address and keep the Ray in scratch memory) */
 __device__ __forceinline__ float axis3(int k, float x, float y, float z)
@@ -125,7 +130,7 @@ __device__ __forceinline__ float4 plane_k(int k, float c, const Ray& r)
     float       t  = __builtin_fmaf(e, y, q); /* = q where e == 0, since |q| >= 2^-100 there */
     const float aq = fabsf(q);
     const bool  ok = d != 0.0f;
-    if (!(aq >= 0x1p-100f && aq <= 0x1p+120f) && ok) {
+    if (!(aq >= 0x1p-100f && aq <= 0x1p+120f && fabsf(a) >= 0x1p-100f) && ok) {
         float aa = a, dd = d;
         QBT_OPAQUE(aa);
         QBT_OPAQUE(dd);
@@ -565,41 +570,43 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     word |= mask << 24;
     const int n     = (t0 ? 1 : 0) + (t1 ? 1 : 0) + (t2 ? 1 : 0) + (t3 ? 1 : 0);
 
-    /* the entry to descend into: the first kept candidate, or, backtracking
-     * (octree_fsh.c:368-375), the next pending entry of the nearest
-     * ancestor that has one */
-    int      lv  = level;
-    uint32_t w   = word, a = rec.x;
-    if (n > 0) {
-        t.entry = pick_entry((int) ((word >> 3) & 3u), t.entry, az, ax, ay);
-        st.pw[level * st.stride] = word;
-        st.ns[level * st.stride] = rec.x | ((uint32_t) (n - 1) << 30);
-        t.pend |= (n > 1 ? 1u : 0u) << level;
-    } else {
-        /* the deepest level with a pending entry, from the per-lane mask
-         * (levels between it and this one have none: no loop over them) */
-        const uint32_t below = t.pend & ((1u << level) - 1u);
-        if (below == 0u) return over ? ST_ABORT : ST_MISS;
-        lv                    = 31 - __clz((int) below);
-        const uint32_t an     = (uint32_t) st.ns[lv * st.stride];
-        const uint32_t pw     = st.pw[lv * st.stride];
-        w                     = ((pw & 0xFFFFFu) >> 5) | (pw & 0xFF000000u); /* the next entry becomes current */
-        st.pw[lv * st.stride] = w;
-        st.ns[lv * st.stride] = (int) (an - (1u << 30));
-        /* bit lv is set; clear it when this was the level's last pending
-         * entry (count field 1: bit 31 of the node word clear) */
-        t.pend ^= (1u << lv) & ~(uint32_t) ((int) an >> 31);
-        a = an & 0x3FFFFFFFu;
-        /* its entry point: the mid-plane hit of the nearest node on the path
-         * (this one or an ancestor) entered through a mid-plane, else the
-         * root entry */
-        int j = lv, src = (int) ((w >> 3) & 3u);
+    /* The entry to descend into: the first kept candidate, or, backtracking
+     * (octree_fsh.c:368-375), the next pending entry of the nearest ancestor
+     * that has one. Descending and backtracking lanes share one path (most
+     * wave-steps hold both kinds): the level they work on is lv, whose two
+     * stack words every lane reads and rewrites, and the new words, the
+     * pending mask and the child block are selects. */
+    const bool     desc  = n > 0;
+    const uint32_t below = t.pend & ((1u << level) - 1u);
+    if (!desc && below == 0u) return over ? ST_ABORT : ST_MISS;
+    /* the deepest level with a pending entry, from the per-lane mask (levels
+     * between it and this one have none: no loop over them) */
+    const int      lv  = desc ? level : 31 - __clz((int) below);
+    const uint32_t an  = (uint32_t) st.ns[lv * st.stride];
+    const uint32_t pwv = st.pw[lv * st.stride];
+    /* backtracking: the next entry of level lv becomes the current one */
+    const uint32_t w   = desc ? word : ((pwv & 0xFFFFFu) >> 5) | (pwv & 0xFF000000u);
+    st.pw[lv * st.stride] = w;
+    st.ns[lv * st.stride] = (int) (desc ? rec.x | (uint32_t) (n - 1) << 30 : an - (1u << 30));
+    /* descending: bit `level` set when more than one entry was kept;
+     * backtracking: bit lv cleared when this was the level's last pending
+     * entry (count field 1: bit 31 of the node word clear) */
+    t.pend = desc ? t.pend | (n > 1 ? 1u : 0u) << level : t.pend ^ ((1u << lv) & ~(uint32_t) ((int) an >> 31));
+    const uint32_t a = desc ? rec.x : an & 0x3FFFFFFFu;
+    /* Its entry point. Source 0 (the node's own entry point): a descending
+     * lane keeps its entry; a backtracking one takes the mid-plane hit of the
+     * nearest node on the path entered through a mid-plane, else the root
+     * entry. Sources 1-3: that mid-plane of the level-j node, recomputed from
+     * the plane (same operations as the candidate, so the same bits). */
+    int j = lv, src = (int) ((w >> 3) & 3u);
+    if (!desc && src == 0) {
         while (src == 0 && j > 0) {
             j--;
             src = (int) ((st.pw[j * st.stride] >> 3) & 3u);
         }
-        t.entry = src != 0 ? midplane_cell<SEM, GRID>(s, r, src, j, t.ix, t.iy, t.iz, level - j) : t.start;
     }
+    if (!desc || src != 0)
+        t.entry = src != 0 ? midplane_cell<SEM, GRID>(s, r, src, j, t.ix, t.iy, t.iz, level - j) : t.start;
     const int sh = level - lv, o = (int) (w & 7u);
     t.ns    = (int) (a + (uint32_t) __popc(__builtin_amdgcn_ubfe(w, 24u, (uint32_t) o))); /* first + rank */
     t.ix    = ((t.ix >> sh) << 1) | (o & 1);
