@@ -155,6 +155,27 @@ int qbt_read_frame(qbt_ctx* ctx, int view, qbt_frame* out);
  * larger frame); members not rendered are NULL. */
 int qbt_frame_device(qbt_ctx* ctx, int view, qbt_frame* out);
 
+/* Size of the static device array behind qbt_render_noalloc, per GPU. */
+#define QBT_NOALLOC_SCRATCH_BYTES 8704
+
+/* qbt_render without any allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). It renders like
+ * qbt_render with tuning path 0 (the fused per-pixel kernel, direct node
+ * layout; same results as every path) into caller-owned DEVICE frames:
+ * frames[i] holds device pointers for view i with w = (int)width,
+ * h = (int)height, one for every output out_mask selects (rgbaf and isp
+ * 16-byte aligned); qbt_read_frame does not see these frames. Everything
+ * else the kernels need (counters, sprite flags, the node-array check word)
+ * lives in one static device array of QBT_NOALLOC_SCRATCH_BYTES on each GPU,
+ * held until the process ends and shared by every call on that GPU: calls
+ * that overlap on one GPU race. nviews above QBT_MAX_VIEWS: QBT_ERANGE;
+ * QBT_RENDER_SURROGATE / QBT_RENDER_COLLECT: QBT_ENOTSUP. The first call
+ * after an octree upload checks the node array (one synchronisation).
+ * Asynchronous on the context stream; qbt_get_stats synchronises and reports
+ * the call's counters. */
+int qbt_render_noalloc(qbt_ctx* ctx, const qbt_view* views, int nviews, const float* lights, int nlights,
+                       unsigned out_mask, const qbt_frame* frames);
 
 typedef struct qbt_stats {
     double   kernel_ms;      /* device time of the last qbt_render / qbt_trace_rays */

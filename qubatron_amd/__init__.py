@@ -281,6 +281,22 @@ class Context:
         for i, v in enumerate(views):
             self._views[i] = (int(v.width), int(v.height), outs)
 
+    def render_noalloc(self, views, lights, frames, mask=OUT_DEFAULT):
+        """qbt_render_noalloc: render into caller-owned device frames, no
+        device allocation. frames: one dict per view mapping output names of
+        _FIELDS to device addresses (ints)."""
+        if isinstance(views, _ffi.View):
+            views = [views]
+        arr = (_ffi.View * len(views))(*views)
+        fr = (_ffi.Frame * len(views))()
+        for i, (v, f) in enumerate(zip(views, frames)):
+            fr[i].w, fr[i].h = int(v.width), int(v.height)
+            for name, ptr in f.items():
+                setattr(fr[i], name, int(ptr))
+        lights = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 3)
+        check(self._d.qbt_render_noalloc(self._h, arr, len(views), fptr(lights), lights.shape[0], int(mask), fr),
+              "qbt_render_noalloc")
+
     def update(self, width, height, position, angle, lighta, quality, maxlevel, basesize, shoot):
         pos = np.asarray(position, dtype=np.float32)
         ang = np.asarray(angle, dtype=np.float32)

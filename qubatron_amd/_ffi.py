@@ -178,7 +178,7 @@ DEV_SYMBOLS = [
     "qbt_surrogate_init", "qbt_surrogate_set_params", "qbt_surrogate_get_params", "qbt_surrogate_set_band",
     "qbt_surrogate_reserve", "qbt_surrogate_samples", "qbt_surrogate_read_samples", "qbt_surrogate_forward",
     "qbt_surrogate_predict", "qbt_surrogate_grad", "qbt_surrogate_step", "qbt_surrogate_train", "qbt_autotune",
-    "qbt_particle_update",
+    "qbt_particle_update", "qbt_render_noalloc",
 ]
 
 
@@ -194,6 +194,7 @@ def _declare_dev(d):
     d.qbt_upload.argtypes = [vp, vp, C.c_int, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_int]
     d.qbt_download.argtypes = [vp, C.c_int, vp, C.c_size_t, C.c_size_t]
     d.qbt_render.argtypes = [vp, C.POINTER(View), C.c_int, f32p, C.c_int, C.c_uint]
+    d.qbt_render_noalloc.argtypes = [vp, C.POINTER(View), C.c_int, f32p, C.c_int, C.c_uint, C.POINTER(Frame)]
     d.qbt_update.argtypes = [vp, C.c_float, C.c_float, f32p, f32p, C.c_float, C.c_uint8, C.c_int, C.c_float,
                              C.c_int]
     d.qbt_read_frame.argtypes = [vp, C.c_int, C.POINTER(Frame)]

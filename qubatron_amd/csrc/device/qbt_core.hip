@@ -16,7 +16,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <mutex>
 #include <new>
+#include <vector>
 
 #include "qbt.h"
 #include "qbt_internal.hpp"
@@ -146,6 +148,17 @@ __global__ void k_sprite(SceneDev s, BatchDev b, int* __restrict__ sprite)
     const ViewDev&             v  = b.views[blockIdx.x];
     if (l >= v.nlights) return;
     sprite[blockIdx.x * QBT_MAX_LIGHTS_RENDER + l] = sprite_flags<CMP, DUAL, GRID>(s, v, l, st);
+}
+
+/* the same for one view whose uniforms come by value (direct layout) */
+template <bool DUAL, bool GRID>
+__global__ void k_sprite_view(SceneDev s, ViewDev v, int* __restrict__ sprite)
+{
+    extern __shared__ uint32_t lds[];
+    const int   l  = threadIdx.x;
+    const Stack st = make_stack(lds, s.maxlevel, DUAL);
+    if (l >= v.nlights) return;
+    sprite[l] = sprite_flags<false, DUAL, GRID>(s, v, l, st);
 }
 
 template <bool DUAL, bool GRID, bool COUNT>
@@ -377,6 +390,47 @@ __global__ void __launch_bounds__(256) k_particles(SceneDev s, const float* pos,
 
 static char g_last_hip[256];
 
+/* ------------------------------------------------------------------------ */
+/* scratch of qbt_render_noalloc                                            */
+/* ------------------------------------------------------------------------ */
+
+/* A static array on every GPU the library runs on holds what the fused
+ * render path needs besides its inputs and outputs: the counters (CNT_N x
+ * u64), the word of the node-array check and the sprite flags of up to
+ * QBT_MAX_VIEWS views (QBT_MAX_LIGHTS_RENDER ints each). It lives until the
+ * process ends, and every qbt_render_noalloc call on a GPU shares that GPU's
+ * copy: calls that overlap on one GPU race. */
+constexpr size_t kScrCounters  = 0;
+constexpr size_t kScrValFlag   = 256;
+constexpr size_t kScrSprite    = 512;
+constexpr size_t kScratchBytes = kScrSprite + sizeof(int) * QBT_MAX_LIGHTS_RENDER * QBT_MAX_VIEWS;
+static_assert(sizeof(unsigned long long) * CNT_N <= kScrValFlag, "counters fit before the check word");
+static_assert(kScratchBytes == QBT_NOALLOC_SCRATCH_BYTES, "qbt.h states the scratch size");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+/* Device address of `scratch` on device `dev`, which must be current: each
+ * GPU holds its own copy at its own address, looked up once per device. On
+ * the host the array's name is only a stand-in, never a device address. */
+static std::mutex         g_scratch_mu;
+static std::vector<void*> g_scratch_ptr; /* by device ordinal */
+
+static int scratch_address(int dev, void** out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int) g_scratch_ptr.size() <= dev) g_scratch_ptr.resize((size_t) dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void*      p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            qbt_set_hip_error(e, "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+            (void) hipGetLastError();
+            return QBT_EDEVICE;
+        }
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return QBT_OK;
+}
 
 void qbt_set_hip_error(hipError_t e, const char* what, const char* file, int line)
 {
@@ -449,6 +503,10 @@ extern "C" int qbt_create(const qbt_opts* opts, qbt_ctx** out)
         hipEventCreate(&c->ev2) != hipSuccess) {
         (void) hipGetLastError();
         delete c;
+        return QBT_EDEVICE;
+    }
+    if (scratch_address(dev, &c->scratch)) {
+        qbt_destroy(c);
         return QBT_EDEVICE;
     }
     if (c->dev_alloc(c->stage, kStageBytes) || c->dev_alloc(c->counters, sizeof(unsigned long long) * CNT_N) ||
@@ -1020,6 +1078,7 @@ extern "C" int qbt_render(qbt_ctx* c, const qbt_view* views, int nviews, const f
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     c->last.primary_rays = prim;
     c->pending_stats     = true;
+    c->cnt_src           = nullptr;
     c->stats_has_trace   = false;
     return QBT_OK;
 }
@@ -1049,6 +1108,77 @@ extern "C" int qbt_update(qbt_ctx* c, float width, float height, const float pos
     return qbt_render(c, &v, 1, light, 1, QBT_OUT_DEFAULT);
 }
 
+/* qbt_render's fused path into caller-owned device frames, with every
+ * buffer beyond the inputs and outputs in the static scratch array: no
+ * allocation of any kind. */
+extern "C" int qbt_render_noalloc(qbt_ctx* c, const qbt_view* views, int nviews, const float* lights, int nlights,
+                                  unsigned out_mask, const qbt_frame* frames)
+{
+    if (!c || !views || !frames || nviews <= 0) return QBT_EINVAL;
+    if (nviews > QBT_MAX_VIEWS) return QBT_ERANGE; /* sprite flags of QBT_MAX_VIEWS views fit the scratch */
+    if (nlights < 0 || nlights > QBT_MAX_LIGHTS_RENDER || (nlights > 0 && !lights)) return QBT_EINVAL;
+    if (out_mask & (QBT_RENDER_SURROGATE | QBT_RENDER_COLLECT)) return QBT_ENOTSUP; /* wavefront stages only */
+    if (!c->scratch) return QBT_EDEVICE;
+    const bool count = (out_mask & QBT_RENDER_COUNT) != 0;
+    unsigned   mask  = out_mask & QBT_OUT_ALL;
+    if (!count) mask &= ~QBT_OUT_EXP;
+    for (int i = 0; i < nviews; i++) {
+        const qbt_view&  q = views[i];
+        const qbt_frame& f = frames[i];
+        if (!(q.width >= 1.0f) || !(q.height >= 1.0f) || q.width > 65536.0f || q.height > 65536.0f) return QBT_EINVAL;
+        if ((double) (int) q.width * (double) (int) q.height > 2147483647.0) return QBT_EINVAL;
+        if (f.w != (int) q.width || f.h != (int) q.height) return QBT_EINVAL;
+        const void* need[FR_N] = {f.hit, f.leaf, f.shadow, f.rgba8, f.rgbaf, f.isp, f.flags, f.dyn, f.exp};
+        for (int k = 0; k < FR_N; k++)
+            if ((mask & (1u << k)) && !need[k]) return QBT_EINVAL; /* a selected output without a buffer */
+        if (((uintptr_t) f.rgbaf | (uintptr_t) f.isp) & 15u) return QBT_EINVAL; /* float4 stores */
+    }
+    HIPCHK(hipSetDevice(c->device));
+    uint8_t* const            scr = (uint8_t*) c->scratch; /* this device's copy of `scratch` */
+    unsigned long long* const cnt = (unsigned long long*) (scr + kScrCounters);
+    unsigned* const           vfl = (unsigned*) (scr + kScrValFlag);
+    int* const                spr = (int*) (scr + kScrSprite);
+    c->last = qbt_stats();
+    k_zero_u64<<<1, 64, 0, c->stream>>>(cnt, CNT_N);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    uint64_t prim = 0;
+    for (int i = 0; i < nviews; i++) {
+        SceneDev s;
+        bool     dual, grid, cmp;
+        /* the fused kernel walks the direct layout: no relayout, no allocation */
+        int rc = c->scene_for(views[i].basecube, views[i].maxlevel, s, dual, grid, cmp, false, vfl);
+        if (rc) return rc;
+        ViewDev v;
+        qbt_make_view(views[i], lights, nlights, v);
+        const qbt_frame& q = frames[i];
+        FrameDev f = {q.hit, q.leaf, q.shadow, q.rgba8, (float4*) q.rgbaf, (float4*) q.isp, q.flags, q.dyn, q.exp};
+        int* const   sp    = spr + i * QBT_MAX_LIGHTS_RENDER;
+        const int    bs    = c->block_threads(s.maxlevel, dual);
+        const int    th    = bs / 16;
+        const int    tiles = ((v.w + 15) / 16) * ((v.h + th - 1) / th);
+        const size_t lds   = qbt_stack_bytes(s.maxlevel, dual, bs);
+        const size_t slds  = qbt_stack_bytes(s.maxlevel, dual, 64);
+        qbt_dispatch3(dual, grid, count, [&](auto D_, auto G_, auto N_) {
+            constexpr bool D = decltype(D_)::value, G = decltype(G_)::value, N = decltype(N_)::value;
+            if (nlights > 0) k_sprite_view<D, G><<<1, 64, slds, c->stream>>>(s, v, sp);
+            k_render<D, G, N><<<tiles, bs, lds, c->stream>>>(s, v, sp, f, mask, cnt);
+            return 0;
+        });
+        HIPCHK(hipGetLastError());
+        c->last.grid_exact = grid;
+        c->last.dual       = dual;
+        c->last.launches += nlights > 0 ? 2 : 1;
+        prim += (uint64_t) v.w * (uint64_t) v.h;
+    }
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    c->last.primary_rays = prim;
+    c->pending_stats     = true;
+    c->stats_has_trace   = false;
+    c->cnt_src           = cnt;
+    return QBT_OK;
+}
+
 extern "C" int qbt_get_stats(qbt_ctx* c, qbt_stats* out)
 {
     if (!c || !out) return QBT_EINVAL;
@@ -1065,7 +1195,7 @@ extern "C" int qbt_get_stats(qbt_ctx* c, qbt_stats* out)
             c->last.trace_ms = tms;
         }
         unsigned long long cnt[CNT_N];
-        HIPCHK(hipMemcpy(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(cnt, c->cnt_src ? c->cnt_src : c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost));
         c->last.shadow_rays = cnt[CNT_SHADOW];
         c->last.hits        = cnt[CNT_HITS];
         c->last.expansions  = cnt[CNT_EXP];
@@ -1374,6 +1504,7 @@ extern "C" int qbt_trace_rays(qbt_ctx* c, const float* origins, const float* dir
     c->last.dual         = dual;
     c->last.launches     = 1;
     c->pending_stats     = true;
+    c->cnt_src           = nullptr;
     c->stats_has_trace   = false;
     qbt_stats st;
     return qbt_get_stats(c, &st);
@@ -1418,6 +1549,7 @@ extern "C" int qbt_particle_update(qbt_ctx* c, const float* pos, const float* sp
     c->last.grid_exact   = grid;
     c->last.launches     = 1;
     c->pending_stats     = true;
+    c->cnt_src           = nullptr;
     c->stats_has_trace   = false;
     qbt_stats st;
     return qbt_get_stats(c, &st);

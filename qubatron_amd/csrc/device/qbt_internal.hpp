@@ -207,6 +207,12 @@ struct qbt_ctx {
         return true;
     }
 
+    /* qbt_render_noalloc: the device address of this device's static
+     * scratch array (qbt_core.hip), and where qbt_get_stats reads the
+     * counters of the last call (nullptr: the context's own) */
+    void*        scratch = nullptr;
+    const void*  cnt_src = nullptr;
+
     int      dev_alloc(DevBuf& b, size_t bytes);
     void     dev_free(DevBuf& b);
     /* flag: a device word for the check's result (nullptr: the context's own,
