@@ -138,27 +138,59 @@ struct Epi {
     MatBF        mask; /* EPI_MASK: keep where mask > 0 (the ReLU output) */
 };
 
-/* C (M x N) = A (M x K) B (K x N), M, N, K multiples of 16. One wave per
- * 16 x 16 output tile and K slice of kchunk; slices combine by atomics. */
-__global__ void __launch_bounds__(256) k_gemm(int M, int N, int K, int kchunk, MatBF A, MatBF B, Epi E)
+/* How k_gemm reads an operand's 16 x 16 tile (host-chosen, see gemm()):
+ * OP_K  the 4 values of k a lane needs are adjacent in memory: one 8-byte
+ *       load per lane and step;
+ * OP_T  the tile's other dimension is contiguous: 8-byte loads along it,
+ *       transposed through LDS, one 8-byte LDS read per lane;
+ * OP_S  4 separate 2-byte loads (any layout). */
+enum { OP_S = 0, OP_K = 1, OP_T = 2 };
+
+/* the MFMA operand of one 16 x 16 tile: element (i, k0 + 4 g + e) of a
+ * matrix whose element (i, k) sits at p[i * si + k * sk] (rows of A, columns
+ * of B), for i = i0 + r */
+__device__ __forceinline__ bf16x4 tile_operand(int mode, const uint16_t* __restrict__ p, long si, long sk, int i0,
+                                               int k0, int lane, uint16_t* __restrict__ lds)
 {
-    const int  lane  = threadIdx.x & 63;
-    const long wave  = ((long) blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    bf16x4    v;
+    if (mode == OP_K) {
+        v = *reinterpret_cast<const bf16x4*>(p + (long) (i0 + r) * si + (long) (k0 + 4 * g) * sk);
+    } else if (mode == OP_T) {
+        /* lane loads i = i0 + 4 (lane & 3) .. + 3 at k = k0 + (lane >> 2), stores them as lds[i][k] */
+        const int    kk = lane >> 2, ii = 4 * (lane & 3);
+        const bf16x4 w  = *reinterpret_cast<const bf16x4*>(p + (long) (i0 + ii) * si + (long) (k0 + kk) * sk);
+        for (int e = 0; e < 4; e++) lds[(ii + e) * 16 + kk] = (uint16_t) w[e];
+        __syncthreads(); /* one wave per block: orders the LDS writes before the reads */
+        v = *reinterpret_cast<const bf16x4*>(lds + r * 16 + 4 * g);
+        __syncthreads(); /* the next step overwrites the tile */
+    } else {
+        for (int e = 0; e < 4; e++) v[e] = (short) p[(long) (i0 + r) * si + (long) (k0 + 4 * g + e) * sk];
+    }
+    return v;
+}
+
+/* C (M x N) = A (M x K) B (K x N), M, N, K multiples of 16. One wave (and
+ * block) per 16 x 16 output tile and K slice of kchunk; slices combine by
+ * atomics. amode / bmode: OP_* of A and B. */
+__global__ void __launch_bounds__(64) k_gemm(int M, int N, int K, int kchunk, MatBF A, MatBF B, Epi E, int amode,
+                                             int bmode)
+{
+    __shared__ __align__(16) uint16_t ta[16 * 16], tb[16 * 16];
+    const int  lane  = threadIdx.x;
+    const long wave  = blockIdx.x;
     const int  tm    = M / 16, tn = N / 16, nsplit = (K + kchunk - 1) / kchunk;
     const long ntile = (long) tm * tn;
-    if (wave >= ntile * nsplit) return;
+    if (wave >= ntile * nsplit) return; /* whole block: the barriers stay uniform */
     const int t = (int) (wave % ntile), split = (int) (wave / ntile);
     const int mt = t % tm, nt = t / tm;
     const int r = lane & 15, g = lane >> 4;
     f32x4     acc = {0.f, 0.f, 0.f, 0.f};
     const int k1  = min(K, (split + 1) * kchunk);
     for (int k0 = split * kchunk; k0 < k1; k0 += 16) {
-        bf16x4 a, b;
-        for (int e = 0; e < 4; e++) {
-            a[e] = A.raw(16 * mt + r, k0 + 4 * g + e);
-            b[e] = B.raw(k0 + 4 * g + e, 16 * nt + r);
-        }
-        acc = mfma16(a, b, acc);
+        const bf16x4 a = tile_operand(amode, A.p, A.rs, A.cs, 16 * mt, k0, lane, ta);
+        const bf16x4 b = tile_operand(bmode, B.p, B.cs, B.rs, 16 * nt, k0, lane, tb);
+        acc            = mfma16(a, b, acc);
     }
     for (int e = 0; e < 4; e++) {
         const long row = 16 * mt + 4 * g + e, col = 16 * nt + r;
@@ -801,10 +833,21 @@ static int mlp_batch_bufs(qbt_ctx* c, int B)
     return QBT_OK;
 }
 
+/* OP_* for an operand whose element (i, k) is at p[i * si + k * sk]: the
+ * 8-byte loads need 8-byte aligned addresses for every i, k multiple of 4 */
+static int operand_mode(const uint16_t* p, long si, long sk)
+{
+    const bool al = ((uintptr_t) p & 7u) == 0;
+    if (al && sk == 1 && si % 4 == 0) return OP_K;
+    if (al && si == 1 && sk % 4 == 0) return OP_T;
+    return OP_S;
+}
+
 static void gemm(hipStream_t st, int M, int N, int K, int kchunk, MatBF A, MatBF B, Epi E)
 {
     const long waves = (long) (M / 16) * (N / 16) * ((K + kchunk - 1) / kchunk);
-    k_gemm<<<(unsigned) ((waves * 64 + 255) / 256), 256, 0, st>>>(M, N, K, kchunk, A, B, E);
+    k_gemm<<<(unsigned) waves, 64, 0, st>>>(M, N, K, kchunk, A, B, E, operand_mode(A.p, A.rs, A.cs),
+                                            operand_mode(B.p, B.cs, B.rs));
 }
 
 /* Forward of the B-column batch in X (n valid columns, labels in io_y):
