@@ -101,10 +101,15 @@ def parse():
     ap.add_argument("--sur-final", type=int, default=2000, help="Adam steps after the last frame")
     ap.add_argument("--sur-batch", type=int, default=4096)
     ap.add_argument("--sur-lr", type=float, default=2e-3)
+    ap.add_argument("--sur-final-batch", type=int, default=0, help="batch of the final steps (0 = --sur-batch)")
+    ap.add_argument("--sur-final-lr", type=float, default=0.0, help="rate of the final steps (0 = --sur-lr)")
+    ap.add_argument("--sur-decay", type=float, default=0.0,
+                    help="final steps: cosine decay of the rate to this fraction (0 = constant)")
     ap.add_argument("--sur-band", type=float, nargs=2, default=(0.1, 0.9))
     ap.add_argument("--uhd-frames", type=int, default=5, help="frames of the 3840x2160 leg (0 = skip)")
     ap.add_argument("--dyn-frames", type=int, default=5, help="frames of the dynamic-tree leg (0 = skip)")
     ap.add_argument("--batch-views", type=int, default=64, help="views of the batched leg (0 = skip)")
+    ap.add_argument("--edit-shots", type=int, default=5, help="shots of the edit leg (0 = skip)")
     ap.add_argument("--batch-reps", type=int, default=3, help="timed batches of the batched leg")
     ap.add_argument("--traffic", default=TRAFFIC_FILE)
     ap.add_argument("--summarize-pmc", nargs="*", default=None, metavar="DIR")
@@ -236,7 +241,8 @@ def surrogate_leg(args, q, ctx, sc, views, Wm, K, mask, exact_wall, exact_rays, 
     t0 = time.time()
     info = S.train_online(ctx, tviews, lights, sc.basecube, nsamples=args.sur_samples, stride=stride,
                           steps_per_frame=args.sur_steps, batch=args.sur_batch, lr=args.sur_lr, seed=args.seed,
-                          final_steps=args.sur_final)
+                          final_steps=args.sur_final, final_batch=args.sur_final_batch or None,
+                          final_lr=args.sur_final_lr or None, decay=args.sur_decay)
     ctx.synchronize()
     train_s = time.time() - t0
     log(f"surrogate: trained on {info['samples']} samples from {args.sur_frames} frames (stride {stride}), "
@@ -282,7 +288,10 @@ def surrogate_leg(args, q, ctx, sc, views, Wm, K, mask, exact_wall, exact_rays, 
            "mlp_decision_launches": kl[KIND_SURROGATE],
            "train": {"frames": args.sur_frames, "samples": info["samples"], "stride": stride,
                      "lit_fraction": round(info["lit_fraction"], 4), "adam_steps": info["steps"],
-                     "batch": args.sur_batch, "lr": args.sur_lr, "loss_last": round(info["loss_last"], 5),
+                     "batch": args.sur_batch, "lr": args.sur_lr, "final_steps": args.sur_final,
+                     "final_batch": args.sur_final_batch or args.sur_batch,
+                     "final_lr": args.sur_final_lr or args.sur_lr, "decay": args.sur_decay,
+                     "loss_last": round(info["loss_last"], 5),
                      "seconds": round(train_s, 2), "pose_seed": args.pose_seed + 7919},
            "workload": f"same scene and timed camera path as the exact form, {W}x{H}"}
     log(f"surrogate: {value:.1f} Mrays/s ({res['speedup_vs_exact']:.3f}x exact), mismatch {res['mismatch']:.5f}, "
@@ -331,6 +340,86 @@ def uhd_leg(args, q, ctx, sc, mask, copy_gbps):
             "roofline": {k: roof[k] for k in ("kernel", "achieved", "frac", "kernel_avg_ms", "frame")}}
 
 
+def edit_leg(args, q, ctx, sc, pnt, nrm, nodes, mask):
+    """The reference's shot edit (modelutil_punch_hole, modelutil.c:345-500) on
+    the static tree, then the next frame: every point within `half` of a hit
+    point is removed (its parent node uploaded, 48 bytes) and re-inserted up
+    to 10 units into the surface along its normal (every node it creates
+    uploaded), as many calls as the reference makes. Timed: the upload calls,
+    and the first frame after them (layout patch included) against a frame
+    without edits and a frame after a whole-array upload (full relayout)."""
+    W, H, K = args.width, args.height, args.edit_shots
+    lights = sc.lights
+    pos, ang = sc.poses(K + 1, seed=args.pose_seed)
+    views = [q.view(pos[i], ang[i], W, H, sc.basecube, sc.depth) for i in range(K + 1)]
+    tree = q.HostOctree.from_nodes(nodes, sc.basecube, sc.depth)
+    bt = q._ffi.BUFFER_STATIC_OCTREE
+
+    def frame_ms(v):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.render(v, lights, mask)
+        ctx.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    ctx.render(views[0], lights, mask | q.OUT_ISP | q.OUT_FLAGS)
+    fr = ctx.read_frame(0, fields=("isp", "flags"))
+    shaded = np.argwhere(fr["flags"] & q.PX_SHADED)
+    plain = [frame_ms(v) for v in views[1:]]
+    rel0, pat0, _ = ctx.layout_stats()
+    shots, upload_ms, uploads, edit_frames = [], 0.0, 0, []
+    half = np.float32(15.0)  # guntype 2: dist up to 30 units, half 15 (modelutil.c:378-379)
+    for k in range(K):
+        cy, cx = shaded[(k * 7919) % len(shaded)]
+        centre = fr["isp"][cy, cx, :3].astype(np.float32)
+        d = pnt - centre
+        near = np.flatnonzero(((d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]) < half * half)
+        ranges, moved = [], []
+        for i in near:  # remove (octree_remove_point), one node upload each
+            m, o = tree.remove_point(pnt[i])
+            if o >= 0:
+                ranges.append((o, tree.len))
+                dist = np.float32(np.sqrt(float(((pnt[i] - centre) ** 2).sum())))
+                rat = (half - dist) / half * np.float32(-10.0)
+                moved.append((m, (pnt[i] + nrm[m] * rat).astype(np.float32)))
+        created = []
+        for m, p_ in moved:  # re-insert (octree_insert_point with octindarr)
+            oa = np.zeros(sc.depth + 1, np.int32)
+            tree.insert_point(m, p_, octindarr=oa)
+            created.append([int(o) for o in oa[oa > 0]])
+        nv = tree.nodes_view()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for o, ln in ranges:
+            ctx.upload(nv, q._ffi.TYPE_INT, ln * 48, 16, o * 48, (o + 1) * 48, bt)
+        for cl in created:
+            for o in cl:
+                ctx.upload(nv, q._ffi.TYPE_INT, tree.len * 48, 16, o * 48, (o + 1) * 48, bt)
+        t1 = time.perf_counter()
+        upload_ms += (t1 - t0) * 1e3
+        uploads += len(ranges) + sum(len(c) for c in created)
+        edit_frames.append(frame_ms(views[1 + k]))
+        shots.append({"removed": len(ranges), "nodes_created": sum(len(c) for c in created)})
+    rel1, pat1, recs = ctx.layout_stats()
+    # the same frame after a whole-array upload: a full relayout of the tree
+    nv = tree.nodes_view()
+    ctx.upload(nv, q._ffi.TYPE_INT, tree.len * 48, 16, 0, tree.len * 48, bt)
+    ctx.synchronize()
+    relayout_frame = frame_ms(views[1])
+    rel2 = ctx.layout_stats()[0]
+    res = {"shots": shots, "upload_calls": uploads, "upload_us_per_call": round(upload_ms * 1e3 / max(uploads, 1), 2),
+           "plain_frame_ms": round(float(np.median(plain)), 4),
+           "edit_frame_ms": [round(x, 4) for x in edit_frames],
+           "frame_after_whole_upload_ms": round(relayout_frame, 3),
+           "layout_patches": pat1 - pat0, "relayouts_during_edits": rel1 - rel0,
+           "relayouts_after_whole_upload": rel2 - rel1, "records": recs,
+           "workload": f"{K} shots (radius {float(half)}) on the static tree, each followed by one {W}x{H} frame"}
+    log(f"edits: {K} shots, {uploads} node uploads at {res['upload_us_per_call']} us per call; frame after a shot "
+        f"{np.median(edit_frames):.2f} ms (plain {res['plain_frame_ms']:.2f} ms, after a whole-array upload "
+        f"{relayout_frame:.1f} ms); patches {pat1 - pat0}, relayouts {rel1 - rel0}")
+    return res
+
+
 def dynamic_leg(args, q, ctx, sc, mask, copy_gbps):
     """The same scene with a populated dynamic tree walked in lockstep with
     the static one (octree_fsh.c:223-241, 313-317): a seeded 2M-point shell
@@ -370,6 +459,7 @@ def dynamic_leg(args, q, ctx, sc, mask, copy_gbps):
             u2 /= np.linalg.norm(u2, axis=1, keepdims=True)
             ctx.upload_points((np.array([900.0, 700.0, 900.0]) + u2 * np.array([120.0, 300.0, 120.0]))
                               .astype(np.float32), dynamic=True)
+            ctx.upload_model(rng.uniform(0.0, 1.0, (npt, 3)).astype(np.float32), u2.astype(np.float32), dynamic=True)
             del u2
         ctx.build_octree(sc.basecube, sc.depth, npt, dynamic=True)  # warm
         ctx.synchronize()
@@ -380,12 +470,26 @@ def dynamic_leg(args, q, ctx, sc, mask, copy_gbps):
         rebuild[str(npt)] = {"ms": round((time.perf_counter() - t0) * 1e3 / K, 3), "nodes": nodes_b}
         log(f"dynamic tree rebuild from resident points: {npt} points -> {nodes_b} nodes in "
             f"{rebuild[str(npt)]['ms']:.2f} ms")
+    # the reference's per-frame sequence with the dynamic tree rebuilt on the
+    # GPU: rebuild from the resident points, then the frame, timed together
+    ctx.build_octree(sc.basecube, sc.depth, npt, dynamic=True)
+    ctx.render(views[0], lights, mask)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for v in views[1:]:
+        ctx.build_octree(sc.basecube, sc.depth, npt, dynamic=True)
+        ctx.render(v, lights, mask)
+    ctx.synchronize()
+    rebuild_frame_ms = (time.perf_counter() - t0) * 1e3 / K
+    log(f"dynamic tree rebuilt every frame ({npt} points): {rebuild_frame_ms:.2f} ms per frame (rebuild + render)")
     ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)  # back to the static-only scene
     value = float(sum(rays)) / wall / 1e6
     km, kl = prof["kind_ms"], prof["kind_launches"]
     log(f"dynamic tree ({m} points, {dn} nodes): {value:.1f} Mrays/s, {wall * 1e3 / K:.2f} ms/frame")
     return {"value": round(value, 3), "unit": "Mrays/s", "ms_per_step": round(wall * 1e3 / K, 4), "steps": K,
             "dynamic_points": m, "dynamic_nodes": dn, "dual_walk": dual, "rebuild_per_frame": rebuild,
+            "frame_with_rebuild_ms": {"points": npt, "ms": round(rebuild_frame_ms, 4),
+                                      "what": "qbt_build_octree of the dynamic tree from resident points + the frame"},
             "rays_per_frame": round(float(np.mean(rays))),
             "expansions_per_frame": round(float(np.mean(exps))),
             "kernel_ms_per_frame": {"primary": round(km[KIND_PRIMARY] / K, 4),
@@ -495,7 +599,8 @@ def main():
     ctx.synchronize()
     t_build = time.time() - t0
     log(f"built octree on GPU: {nnodes} nodes ({nnodes * 48 / 1e9:.2f} GB reference layout) in {t_build:.3f}s")
-    del pnt
+    if args.edit_shots <= 0:
+        del pnt
 
     K, Wm = args.steps, args.warmup
     pos, ang = sc.poses(K + Wm, seed=args.pose_seed)
@@ -541,6 +646,7 @@ def main():
 
     # CPU baseline: the oracle on whole frames of the timed camera path
     cpu = None
+    nodes = None
     if not args.no_cpu:
         from oracle import pyoracle as po
         t0 = time.time()
@@ -570,6 +676,12 @@ def main():
 
     uhd = uhd_leg(args, q, ctx, sc, mask, copy_gbps) if args.uhd_frames > 0 else None
     dyn = dynamic_leg(args, q, ctx, sc, mask, copy_gbps) if args.dyn_frames > 0 else None
+    edits = None
+    if args.edit_shots > 0:
+        if nodes is None:
+            nodes = ctx.download_octree(nnodes)
+        edits = edit_leg(args, q, ctx, sc, pnt, nrm, nodes, mask)
+        del pnt
     batch = batch_leg(args, q, ctx, nthreads, mask) if args.batch_views > 0 else None  # replaces the scene
 
     res = {
@@ -589,6 +701,7 @@ def main():
         "surrogate": sur,
         "uhd_3840x2160": uhd,
         "dynamic_tree": dyn,
+        "edits": edits,
         "batch_views": batch,
         "tuning": tuning,
         "build_s": round(t_build, 3),

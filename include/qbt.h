@@ -194,6 +194,14 @@ typedef struct qbt_stats {
 
 int qbt_get_stats(qbt_ctx* ctx, qbt_stats* out);
 
+/* The compact node layout of tree (QBT_BUFFER_STATIC_OCTREE or
+ * QBT_BUFFER_DYNAMIC_OCTREE): how many times it was rebuilt from the node
+ * array (relayouts) and patched in place after range uploads (patches) since
+ * the context was created, and its record count (patches append records).
+ * The layout is brought up to date at the next render; qbt_build_octree
+ * writes it directly (neither count changes). Any pointer may be NULL. */
+int qbt_layout_stats(qbt_ctx* ctx, int tree, uint64_t* relayouts, uint64_t* patches, uint64_t* records);
+
 /* Wait for all work queued on the context stream. */
 int qbt_synchronize(qbt_ctx* ctx);
 

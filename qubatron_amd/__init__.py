@@ -43,6 +43,16 @@ class HostOctree:
         except Exception:
             pass
 
+    @classmethod
+    def from_nodes(cls, nodes, basecube, levels):
+        """A host octree holding a copy of an existing node array (n x 12)."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32).reshape(-1, NODE_INTS)
+        t = cls(basecube, levels)
+        t.reserve(nodes.shape[0] + 1024)
+        C.memmove(t._t.octs, nodes.ctypes.data, nodes.nbytes)
+        t._t.len = nodes.shape[0]
+        return t
+
     @property
     def len(self):
         return int(self._t.len)
@@ -328,6 +338,13 @@ class Context:
 
     def synchronize(self):
         check(self._d.qbt_synchronize(self._h), "qbt_synchronize")
+
+    def layout_stats(self, dynamic=False):
+        """(relayouts, patches, records) of the compact layout of a tree."""
+        r, p, n = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        bt = _ffi.BUFFER_DYNAMIC_OCTREE if dynamic else _ffi.BUFFER_STATIC_OCTREE
+        check(self._d.qbt_layout_stats(self._h, bt, C.byref(r), C.byref(p), C.byref(n)), "qbt_layout_stats")
+        return int(r.value), int(p.value), int(n.value)
 
     def profile_begin(self):
         check(self._d.qbt_profile_begin(self._h), "qbt_profile_begin")

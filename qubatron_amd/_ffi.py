@@ -178,7 +178,7 @@ DEV_SYMBOLS = [
     "qbt_surrogate_init", "qbt_surrogate_set_params", "qbt_surrogate_get_params", "qbt_surrogate_set_band",
     "qbt_surrogate_reserve", "qbt_surrogate_samples", "qbt_surrogate_read_samples", "qbt_surrogate_forward",
     "qbt_surrogate_predict", "qbt_surrogate_grad", "qbt_surrogate_step", "qbt_surrogate_train", "qbt_autotune",
-    "qbt_particle_update", "qbt_render_noalloc",
+    "qbt_particle_update", "qbt_render_noalloc", "qbt_layout_stats",
 ]
 
 
@@ -205,6 +205,8 @@ def _declare_dev(d):
     d.qbt_build_octree.argtypes = [vp, C.c_int, f32p, C.c_int, C.c_size_t, C.POINTER(C.c_size_t)]
     d.qbt_particle_update.argtypes = [vp, f32p, f32p, C.c_size_t, C.c_int, C.c_float, f32p, f32p]
     d.qbt_synchronize.argtypes = [vp]
+    u64p = C.POINTER(C.c_uint64)
+    d.qbt_layout_stats.argtypes = [vp, C.c_int, u64p, u64p, u64p]
     d.qbt_profile_begin.argtypes = [vp]
     d.qbt_profile_end.argtypes = [vp, C.POINTER(Profile)]
     d.qbt_get_tuning.argtypes = [vp, C.POINTER(Tuning)]

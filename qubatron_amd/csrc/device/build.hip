@@ -171,6 +171,40 @@ __global__ void k_link(const uint64_t* __restrict__ ukey, const uint32_t* __rest
     child[8 * (size_t) par + (ukey[j] & 7u)] = (int) idx;
 }
 
+/* The compact layout (layout.hip's format) of the built tree, straight from
+ * the groups: the groups of a depth in key order are that depth's nodes in
+ * breadth-first hierarchical order, so group j of depth d is record
+ * off_d + j, and the children of a node are the consecutive groups of the
+ * next depth with its position as parent. Per group: its record (children
+ * fill it from the next depth's launch), slot 8, reference index, and its
+ * bit and, for the first child, the block start in the parent's record. */
+__global__ void k_cmp_root(uint2* __restrict__ rec, int* __restrict__ model_c, int* __restrict__ ref,
+                           int* __restrict__ r2c)
+{
+    rec[0]     = make_uint2(0u, 0u);
+    model_c[0] = 0;
+    ref[0]     = 0;
+    r2c[0]     = 0;
+}
+
+__global__ void k_cmp_depth(const uint64_t* __restrict__ ukey, const uint32_t* __restrict__ first,
+                            const uint32_t* __restrict__ node, const uint32_t* __restrict__ parent_pos, size_t n,
+                            uint32_t off_parent, uint32_t off_here, uint2* __restrict__ rec, int* __restrict__ model_c,
+                            int* __restrict__ ref, int* __restrict__ r2c)
+{
+    size_t j = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t ci = off_here + (uint32_t) j;
+    rec[ci]           = make_uint2(0u, 0u);
+    model_c[ci]       = (int) first[j];
+    ref[ci]           = (int) node[j];
+    r2c[node[j]]      = (int) ci;
+    const uint32_t pp = parent_pos ? parent_pos[j] - 1u : 0u; /* parent_pos is 1-based */
+    const uint32_t p  = off_parent + pp;
+    atomicOr(&rec[p].y, 0x101u << (uint32_t) (ukey[j] & 7u)); /* live and alloc */
+    if (j == 0 || (parent_pos && parent_pos[j] != parent_pos[j - 1])) rec[p].x = ci;
+}
+
 inline unsigned nblk(size_t n) { return (unsigned) ((n + 255) / 256); }
 
 } // namespace
@@ -230,6 +264,7 @@ extern "C" int qbt_build_octree(qbt_ctx* c, int tree, const float basecube[4], i
     hipStream_t    s = c->stream;
     const uint32_t L = (uint32_t) levels;
     int            rc = QBT_OK;
+    if ((rc = c->flush_pending(t, true))) return rc; /* the build replaces the array */
     if (!c->build) c->build = new BuildState();
     Scratch        sc(c, t == 1);
     size_t         n = npoints;
@@ -349,6 +384,16 @@ extern "C" int qbt_build_octree(qbt_ctx* c, int tree, const float basecube[4], i
     k_fill_u32v<<<nblk(total * 8), 256, 0, s>>>((uint32_t*) tr.child.p, total * 8, 0u);
     k_fill_u32v<<<nblk(total), 256, 0, s>>>((uint32_t*) tr.model.p, total, 0u);
     HIPCHK(hipGetLastError());
+    /* 5. the compact layout, unless its 30-bit first-child indices overflow */
+    CompactTree& cm  = tr.cmp;
+    const bool   cmp = total < (1u << 30);
+    cm.ok = false, cm.n = 0, cm.ref_n = 0;
+    tr.n  = total; /* compact_reserve sizes the reference map by it */
+    if (cmp) {
+        if ((rc = c->compact_reserve(t, total, total))) return rc;
+        k_cmp_root<<<1, 1, 0, s>>>((uint2*) cm.rec.p, (int*) cm.model.p, (int*) cm.ref.p, (int*) cm.r2c.p);
+        HIPCHK(hipGetLastError());
+    }
     if (n && L) {
         std::vector<uint32_t*> node(L + 2, nullptr);
         for (uint32_t d = 1; d <= L; d++) {
@@ -362,12 +407,24 @@ extern "C" int qbt_build_octree(qbt_ctx* c, int tree, const float basecube[4], i
                                                (int*) tr.child.p, (int*) tr.model.p);
             HIPCHK(hipGetLastError());
         }
+        if (cmp) {
+            uint32_t off_parent = 0, off_here = 1; /* the root is record 0 */
+            for (uint32_t d = 1; d <= L; d++) {
+                k_cmp_depth<<<nblk(cnt[d]), 256, 0, s>>>(ukey[d], first[d], node[d], d == 1 ? nullptr : ppos[d],
+                                                         cnt[d], off_parent, off_here, (uint2*) cm.rec.p,
+                                                         (int*) cm.model.p, (int*) cm.ref.p, (int*) cm.r2c.p);
+                HIPCHK(hipGetLastError());
+                off_parent = off_here;
+                off_here += (uint32_t) cnt[d];
+            }
+        }
     }
     HIPCHK(hipStreamSynchronize(s));
-    tr.n          = total;
     tr.validated  = false;
     tr.full_check = true;
-    tr.cmp_stale  = true;
+    tr.patch.clear();
+    tr.cmp_stale = !cmp;
+    if (cmp) cm.ok = true, cm.n = total, cm.ref_n = total;
     if (nodes_out) *nodes_out = total;
     return QBT_OK;
 }

@@ -47,6 +47,20 @@ __global__ void k_scatter_octree(const int4* __restrict__ stage, size_t t0, size
     else model[node] = v.x;
 }
 
+/* held octree texels: texel idx[i] = val[i] */
+__global__ void k_scatter_texels(const int4* __restrict__ val, const uint64_t* __restrict__ idx, size_t count,
+                                 int4* __restrict__ child, int* __restrict__ model)
+{
+    size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const uint64_t t    = idx[i];
+    const uint64_t node = t / 3;
+    const int      k    = (int) (t - node * 3);
+    const int4     v    = val[i];
+    if (k < 2) child[2 * node + k] = v;
+    else model[node] = v.x;
+}
+
 /* float3 items [i0, i0+count) into the .xyz of ca[2*i + which] */
 __global__ void k_scatter_attr(const float* __restrict__ stage, size_t i0, size_t count, float4* __restrict__ ca,
                                int which)
@@ -529,6 +543,8 @@ extern "C" void qbt_destroy(qbt_ctx* c)
         c->dev_free(c->tree[t].cmp.rec);
         c->dev_free(c->tree[t].cmp.model);
         c->dev_free(c->tree[t].cmp.ref);
+        c->dev_free(c->tree[t].cmp.r2c);
+        c->dev_free(c->tree[t].cmp.scratch);
         c->dev_free(c->attr[t].ca);
         c->dev_free(c->pts[t].xyz);
     }
@@ -580,6 +596,34 @@ extern "C" const char* qbt_version(void) { return "qubatron-mi355x 0.1 (gfx950)"
 
 static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned) ((n + bs - 1) / bs); }
 
+/* octree range uploads of at most this many texels are held on the host
+ * and applied together (the reference uploads one 48-byte node per edited
+ * node, modelutil.c:425-500: hundreds of calls per edit) */
+static const size_t kSmallTexels = 96;
+static const size_t kPendMax     = kStageBytes / 24; /* 16-byte value + 8-byte index each */
+
+int qbt_ctx::flush_pending(int t, bool discard)
+{
+    TreeBuf&     tb = tree[t];
+    const size_t n  = tb.pend_idx.size();
+    if (n && !discard) {
+        /* values, then indices, through the staging buffer in one copy */
+        std::vector<uint8_t> pack(n * 24);
+        memcpy(pack.data(), tb.pend_val.data(), n * 16);
+        memcpy(pack.data() + n * 16, tb.pend_idx.data(), n * 8);
+        HIPCHK(hipMemcpyAsync(stage.p, pack.data(), n * 24, hipMemcpyHostToDevice, stream));
+        k_scatter_texels<<<blocks_for(n, 256), 256, 0, stream>>>((const int4*) stage.p,
+                                                                 (const uint64_t*) ((char*) stage.p + n * 16), n,
+                                                                 (int4*) tb.child.p, (int*) tb.model.p);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream)); /* the staging buffer is free again */
+    }
+    tb.pend_idx.clear();
+    tb.pend_val.clear();
+    tb.pend_pos.clear();
+    return QBT_OK;
+}
+
 /* logical/capacity bookkeeping of one upload target */
 extern "C" int qbt_upload(qbt_ctx* c, const void* data, int type, size_t size, size_t itemsize, size_t start,
                           size_t end, int buftype)
@@ -604,6 +648,11 @@ extern "C" int qbt_upload(qbt_ctx* c, const void* data, int type, size_t size, s
         TreeBuf& tb    = c->tree[t];
         size_t   nodes = items / 3;
         if (nodes > INT32_MAX) return QBT_ENOTSUP;
+        const size_t t0r = start / 16, t1r = end / 16;
+        const bool   small = nodes <= tb.cap_nodes && t1r > t0r && t1r - t0r <= kSmallTexels;
+        /* held uploads go before a direct copy (call order); a regrowth's full
+         * copy supersedes them */
+        if (!small && (rc = c->flush_pending(t, nodes > tb.cap_nodes))) return rc;
         if (nodes > tb.cap_nodes) { /* regrowth: reallocate and copy everything (octree_glc.c:412-428) */
             size_t cap = nodes + nodes / 4 + 1024;
             c->dev_free(tb.child);
@@ -617,8 +666,9 @@ extern "C" int qbt_upload(qbt_ctx* c, const void* data, int type, size_t size, s
             start        = 0;
             end          = size;
             tb.full_check = true;
+            tb.cmp_stale  = true;
         }
-        if (nodes < tb.n) tb.full_check = true;
+        if (nodes < tb.n) tb.full_check = tb.cmp_stale = true;
         tb.n = nodes;
         /* copy texels [start/16, end/16) through the staging buffer */
         size_t t0 = start / 16, t1 = end / 16;
@@ -628,7 +678,29 @@ extern "C" int qbt_upload(qbt_ctx* c, const void* data, int type, size_t size, s
             else tb.dirty_lo = lo < tb.dirty_lo ? lo : tb.dirty_lo, tb.dirty_hi = hi > tb.dirty_hi ? hi : tb.dirty_hi;
         }
         tb.validated = false;
-        tb.cmp_stale = true;
+        /* the compact layout: relayout after a whole-array upload, else patch
+         * the written node range in place at the next render */
+        if (t1 > t0 && !tb.cmp_stale) {
+            const size_t lo = t0 / 3, hi = (t1 + 2) / 3;
+            if (lo == 0 && hi >= nodes) tb.cmp_stale = true;
+            else if (tb.patch.size() >= 65536) tb.cmp_stale = true; /* as much bookkeeping as a relayout */
+            else tb.patch.emplace_back(lo, hi);
+        }
+        if (small) { /* hold the texels (copied now: the caller may change its array) */
+            const int4* src = (const int4*) ((const char*) data + t0 * 16);
+            for (size_t a = t0; a < t1; a++) {
+                auto it = tb.pend_pos.find(a);
+                if (it != tb.pend_pos.end()) {
+                    tb.pend_val[it->second] = src[a - t0];
+                } else {
+                    tb.pend_pos.emplace(a, (uint32_t) tb.pend_idx.size());
+                    tb.pend_idx.push_back(a);
+                    tb.pend_val.push_back(src[a - t0]);
+                }
+            }
+            if (tb.pend_idx.size() >= kPendMax && (rc = c->flush_pending(t))) return rc;
+            return QBT_OK;
+        }
         const size_t per = kStageBytes / 16;
         for (size_t a = t0; a < t1; a += per) {
             size_t cnt = t1 - a < per ? t1 - a : per;
@@ -708,6 +780,8 @@ extern "C" int qbt_download(qbt_ctx* c, int buftype, void* host, size_t start, s
     if (octree) {
         const TreeBuf& tb = c->tree[t];
         if (start % 16 || end % 16 || end > tb.n * 48) return QBT_EINVAL;
+        int rc;
+        if ((rc = c->flush_pending(t))) return rc;
         size_t       t0 = start / 16, t1 = end / 16;
         const size_t per = kStageBytes / 16;
         for (size_t a = t0; a < t1; a += per) {
@@ -802,6 +876,7 @@ int qbt_ctx::scene_for(const float basecube[4], int maxlevel, SceneDev& s, bool&
     if (!(ldexpf(basecube[3], -maxlevel) >= FLT_MIN)) return QBT_EINVAL;
     if (tree[0].n == 0) return QBT_EINVAL; /* no static octree uploaded */
     int rc;
+    if ((rc = flush_pending(0)) || (rc = flush_pending(1))) return rc;
     if ((rc = validate_tree(0, valflag_dev))) return rc;
     if ((rc = validate_tree(1, valflag_dev))) return rc;
     dual = tree[1].n > 1 || (maxlevel == 0 && tree[1].n >= 1);
@@ -809,11 +884,21 @@ int qbt_ctx::scene_for(const float basecube[4], int maxlevel, SceneDev& s, bool&
     /* compact layout: rebuilt lazily after the node arrays change */
     cmp = false;
     if (allow_compact && use_compact) {
-        for (int t = 0; t < (dual ? 2 : 1); t++)
-            if (tree[t].cmp_stale) {
-                if ((rc = relayout(t))) return rc;
-                tree[t].cmp_stale = false;
+        for (int t = 0; t < (dual ? 2 : 1); t++) {
+            /* range uploads since the layout: patched in place; a relayout
+             * when the array was replaced or the patch cannot represent them */
+            TreeBuf& tb   = tree[t];
+            bool     need = tb.cmp_stale;
+            if (!need && !tb.patch.empty()) {
+                bool done = false;
+                if ((rc = patch_layout(t, &done))) return rc;
+                need = !done;
             }
+            if (need) {
+                if ((rc = relayout(t))) return rc;
+                tb.cmp_stale = false;
+            }
+        }
         cmp = tree[0].cmp.ok && (!dual || tree[1].cmp.ok);
     }
     memset(&s, 0, sizeof(s));
@@ -1206,6 +1291,16 @@ extern "C" int qbt_get_stats(qbt_ctx* c, qbt_stats* out)
     }
     *out = c->last;
     return c->aborted ? QBT_EABORT : QBT_OK;
+}
+
+extern "C" int qbt_layout_stats(qbt_ctx* c, int tree, uint64_t* relayouts, uint64_t* patches, uint64_t* records)
+{
+    if (!c || (tree != QBT_BUFFER_STATIC_OCTREE && tree != QBT_BUFFER_DYNAMIC_OCTREE)) return QBT_EINVAL;
+    const TreeBuf& tb = c->tree[tree == QBT_BUFFER_STATIC_OCTREE ? 0 : 1];
+    if (relayouts) *relayouts = tb.relayouts;
+    if (patches) *patches = tb.patches;
+    if (records) *records = tb.cmp.ok ? tb.cmp.n : 0;
+    return QBT_OK;
 }
 
 extern "C" int qbt_synchronize(qbt_ctx* c)

@@ -8,6 +8,8 @@
 #include <stdint.h>
 
 #include <type_traits>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "qbt.h"
@@ -36,9 +38,14 @@ struct DevBuf {
 
 /* breadth-first compact layout of a tree (layout.hip) */
 struct CompactTree {
-    DevBuf rec, model, ref;
-    size_t n  = 0;
-    bool   ok = false; /* built and the node array is a proper tree */
+    DevBuf rec, model, ref; /* cap records each */
+    DevBuf r2c;             /* per reference node: its record, -1 none (ref_n valid, r2c_cap allocated) */
+    DevBuf scratch;         /* relayout and patch work buffers, kept between calls */
+    size_t n       = 0;     /* records in use (in-place patches append) */
+    size_t cap     = 0;
+    size_t ref_n   = 0;     /* reference nodes the layout covers */
+    size_t r2c_cap = 0;
+    bool   ok      = false; /* built and the node array is a proper tree */
 };
 
 struct TreeBuf {
@@ -49,7 +56,16 @@ struct TreeBuf {
     bool   full_check = false;
     size_t dirty_lo = 0, dirty_hi = 0;
     CompactTree cmp;
-    bool        cmp_stale = true; /* node array changed since the last relayout */
+    bool        cmp_stale = true; /* node array replaced since the last layout: relayout */
+    /* node ranges written by range uploads since the last layout: patched in
+     * place (layout.hip), or a relayout when the patch cannot represent them */
+    std::vector<std::pair<size_t, size_t>> patch;
+    uint64_t relayouts = 0, patches = 0;
+    /* small range uploads, held on the host until the tree is next used:
+     * texel index -> its 16 bytes (a later upload of a texel replaces it) */
+    std::vector<uint64_t>                   pend_idx;
+    std::vector<int4>                       pend_val;
+    std::unordered_map<uint64_t, uint32_t>  pend_pos;
 };
 
 struct AttrBuf {
@@ -218,7 +234,15 @@ struct qbt_ctx {
     /* flag: a device word for the check's result (nullptr: the context's own,
      * allocated on first use) */
     int      validate_tree(int t, unsigned* flag = nullptr);
+    /* apply the held small uploads of tree t (discard: drop them) */
+    int      flush_pending(int t, bool discard = false);
     int      relayout(int t);
+    /* apply the range uploads since the last layout to the compact layout of
+     * tree t in place; *done = false when they change what the patch cannot
+     * represent (then the caller relayouts) */
+    int      patch_layout(int t, bool* done);
+    /* compact arrays of tree t for nrec records and nref reference nodes */
+    int      compact_reserve(int t, size_t nrec, size_t nref);
     int      scene_for(const float basecube[4], int maxlevel, qbt::SceneDev& s, bool& dual, bool& grid, bool& cmp,
                        bool allow_compact, unsigned* valflag_dev = nullptr);
     bool     use_compact = true; /* traverse the compact layout when both trees have one */

@@ -61,7 +61,7 @@ struct TreeDev {
     const int4* __restrict__ child; /* direct layout: 2 per node */
     const int* __restrict__ model;  /* slot 8, 1 per node (of the layout traversed) */
     int n;                          /* node count of the layout traversed */
-    const uint2* __restrict__ rec;  /* compact layout: {first child, child mask} */
+    const uint2* __restrict__ rec;  /* compact layout: {first child, live | alloc << 8} */
     const int* __restrict__ ref;    /* compact layout: reference node index */
 };
 
@@ -83,12 +83,13 @@ struct Ray {
     float ix, iy, iz; /* RN(1/d) per axis (set by trace_begin), NaN where not usable */
 };
 
-/* RN(1 / d) when |d| is in [2^-100, 2^100] (the reciprocal is then a
- * normal number), else NaN so that plane_k takes the division */
+/* RN(1 / d) when |d| is in [2^-20, 2^100] (the reciprocal is then a
+ * normal number of magnitude at most 2^20), else NaN so that plane_k takes
+ * the division */
 __device__ __forceinline__ float recip_or_nan(float d)
 {
     const float a = fabsf(d);
-    return (a >= 0x1p-100f && a <= 0x1p+100f) ? 1.0f / d : __builtin_nanf("");
+    return (a >= 0x1p-20f && a <= 0x1p+100f) ? 1.0f / d : __builtin_nanf("");
 }
 
 /* Plane test on axis k (0 x, 1 y, 2 z) at coordinate c: is_cube_xplane,
@@ -101,11 +102,12 @@ __device__ __forceinline__ float recip_or_nan(float d)
  * The quotient is the correctly rounded one, as the reference's division,
  * from y = RN(1 / d_k): q = RN(a y), e = RN(a - d_k q) (one FMA),
  * t = RN(q + e y) (one FMA), a Markstein-style correction. It is taken
- * where |d_k| is in [2^-100, 2^100] (y not NaN), |a| >= 2^-100 and |q| is in
- * [2^-100, 2^120]: there every step's result is a normal number or exact
- * (e is a multiple of 2^(e_a - 47), at least 2^-147), so each step scales
- * with the exponents of a and d_k, and t is correct for all such operands
- * exactly when it is for every pair of significands in [1, 2). All 2^46
+ * where |d_k| is in [2^-20, 2^100] (y not NaN) and |q| is in [2^-80, 2^120].
+ * There |y| <= 2^20, so |a| >= |q| / (|y| (1 + 2^-24)) > 2^-101: y, q and t
+ * are normal numbers and e is exact (a - d_k q is a multiple of
+ * 2^(e_a - 47) >= 2^-148), so each step scales with the exponents of a and
+ * d_k, and t is correct for all such operands exactly when it is for every
+ * pair of significands in [1, 2). All 2^46
  * pairs are checked on the device (tests/test_gpu_quotient.py) and a sample
  * on the host (tests/test_quotient.py). Elsewhere, and where y is NaN, the
  * quotient is the division itself, on a branch that rays rarely take.
@@ -127,10 +129,10 @@ __device__ __forceinline__ float4 plane_k(int k, float c, const Ray& r)
     const float a  = c - p;
     const float q  = a * y;
     const float e  = __builtin_fmaf(-q, d, a);
-    float       t  = __builtin_fmaf(e, y, q); /* = q where e == 0, since |q| >= 2^-100 there */
+    float       t  = __builtin_fmaf(e, y, q);
     const float aq = fabsf(q);
     const bool  ok = d != 0.0f;
-    if (!(aq >= 0x1p-100f && aq <= 0x1p+120f && fabsf(a) >= 0x1p-100f) && ok) {
+    if (!(aq >= 0x1p-80f && aq <= 0x1p+120f) && ok) {
         float aa = a, dd = d;
         QBT_OPAQUE(aa);
         QBT_OPAQUE(dd);
@@ -288,15 +290,20 @@ struct Block<false> {
     __device__ __forceinline__ int      child(int o) const { return pick8(lo, hi, o); }
 };
 
+/* Compact record: x = index of the first child; y bits 0-7 = the children
+ * that exist (live), bits 8-15 = the slots of the child block (alloc): child
+ * o is record x + (the number of alloc bits below o). Live is a subset of
+ * alloc; they differ only where an in-place range update zeroed a slot and
+ * left its record in the block. */
 template <>
 struct Block<true> {
     uint2 r;
     __device__ __forceinline__ void     clear() { r = make_uint2(0u, 0u); }
     __device__ __forceinline__ void     load(const TreeDev& t, int i) { r = t.rec[i]; }
-    __device__ __forceinline__ unsigned mask() const { return r.y; }
+    __device__ __forceinline__ unsigned mask() const { return r.y & 0xFFu; }
     __device__ __forceinline__ int      child(int o) const
     {
-        return ((r.y >> o) & 1u) ? (int) (r.x + (uint32_t) __popc(r.y & ((1u << o) - 1u))) : 0;
+        return ((r.y >> o) & 1u) ? (int) (r.x + (uint32_t) __popc((r.y >> 8) & ((1u << o) - 1u))) : 0;
     }
 };
 
@@ -429,9 +436,9 @@ __device__ __forceinline__ int trace_begin(const SceneDev& s, Ray& r, TraceState
  *            still pending (bits 30-31, 0..3)
  *   pw word  the entry being visited (bits 0-4), then up to three pending
  *            entries (bits 5-9, 10-14, 15-19); an entry is octet (3 bits) |
- *            source << 3 (2 bits); bits 24-31 hold the node's child mask, so
- *            the child of octet o is first + (the number of mask bits below
- *            o), counted when the entry is taken.
+ *            source << 3 (2 bits); bits 24-31 hold the slots of the node's
+ *            child block (alloc), so the child of octet o is first + (the
+ *            number of those bits below o), counted when the entry is taken.
  * A per-lane bit mask marks the levels that still hold pending entries, so a
  * backtrack finds its level with one count-leading-zeros instead of a loop
  * over the stack.
@@ -507,7 +514,7 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     const int   level = t.level;
     const uint2 rec   = s.stat.rec[(uint32_t) t.ns]; /* node indices are non-negative: no sign extension */
     t.exps++;
-    const uint32_t mask = rec.y; /* 0 only for the root of an empty tree: then no candidate is taken */
+    const uint32_t mask = rec.y & 0xFFu; /* live children; 0 for the root of an empty tree: no candidate taken */
 
     /* ---- expansion (octree_fsh.c:251-330), without branches ---- */
     float tx, ty, tz, tw, bxx, byy, bzz, hx, hy, hz;
@@ -567,7 +574,7 @@ __device__ __forceinline__ int trace_step_c1(const SceneDev& s, const Ray& r, co
     word            = t2 ? (word << 5) | entry(o2, k2) : word;
     word            = t1 ? (word << 5) | entry(o1, k1) : word;
     word            = t0 ? (word << 5) | entry(o0, k0) : word;
-    word |= mask << 24;
+    word |= (rec.y >> 8) << 24; /* the block's slots, for the rank of an entry's child */
     const int n     = (t0 ? 1 : 0) + (t1 ? 1 : 0) + (t2 ? 1 : 0) + (t3 ? 1 : 0);
 
     /* The entry to descend into: the first kept candidate, or, backtracking

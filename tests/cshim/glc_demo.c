@@ -135,10 +135,10 @@ int main(int argc, char** argv)
     fseek(out, 0, SEEK_SET);
     fwrite(hdr, 4, 4, out);
     fclose(out);
+    printf("glc_demo: %lld points, %d removed and re-inserted, %dx%d frames, %zu nodes\n", (long long) n, nrem, w,
+           h, t.len);
     qbt_destroy((qbt_ctx*) rc.qbt);
     qbt_octree_free(&t);
     free(pnt), free(col), free(nrm), free(mods), free(mv);
-    printf("glc_demo: %lld points, %d removed and re-inserted, %dx%d frames, %zu nodes\n", (long long) n, nrem, w,
-           h, t.len);
     return 0;
 }

@@ -48,7 +48,7 @@ bool build(const int32_t* nodes, size_t len, Layouts& L)
                 order.push_back(n[o]);
             }
         }
-        L.rec[k] = make_uint2(mask ? first : 0u, mask);
+        L.rec[k] = make_uint2(mask ? first : 0u, mask | mask << 8);
     }
     L.rec.resize(order.size());
     L.cmodel.resize(order.size());
@@ -131,5 +131,54 @@ extern "C" int ht_trace(const int32_t* nodes, size_t len, const int32_t* dyn, si
         else grid ? QBT_RUN(false, false, true) : QBT_RUN(false, false, false);
     }
 #undef QBT_RUN
+    return 0;
+}
+
+/* The compact layout after in-place range updates that zeroed child slots:
+ * the records and child blocks of `nodes` (the tree at the last relayout),
+ * with the live bits and slot 8 of `live_nodes` (the same array with some
+ * slots zeroed). Traces n rays on it (single tree) and on the direct layout
+ * of live_nodes; both must equal the oracle on live_nodes. Returns -1 if a
+ * layout cannot be built or live_nodes sets a slot the block lacks. */
+extern "C" int ht_trace_patched(const int32_t* nodes, const int32_t* live_nodes, size_t len, const float basecube[4],
+                                int maxlevel, int layout, int grid, const float* org, const float* dir, size_t n,
+                                int32_t* status, int32_t* leaf, float* isp, int64_t* exps)
+{
+    static const int32_t root[12] = {0};
+    Layouts              S, D, Lv;
+    if (!build(nodes, len, S) || !build(root, 1, D)) return -1;
+    for (size_t k = 0; k < S.rec.size(); k++) {
+        const int32_t* nl    = live_nodes + 12 * (size_t) S.ref[k];
+        const unsigned alloc = S.rec[k].y & 0xFFu;
+        unsigned       live  = 0;
+        for (int o = 0; o < 8; o++)
+            if (nl[o] > 0) live |= 1u << o;
+        if (live & ~alloc) return -1;
+        S.rec[k].y  = live | alloc << 8;
+        S.cmodel[k] = nl[8];
+    }
+    for (size_t i = 0; i < len; i++) { /* the direct layout is the edited array itself */
+        const int32_t* nl  = live_nodes + 12 * i;
+        S.child[2 * i]     = make_int4(nl[0], nl[1], nl[2], nl[3]);
+        S.child[2 * i + 1] = make_int4(nl[4], nl[5], nl[6], nl[7]);
+        S.model[i]         = nl[8];
+    }
+    const bool cmp = layout == 1;
+    SceneDev   s;
+    memset(&s, 0, sizeof(s));
+    s.stat     = tree_of(S, cmp);
+    s.dyn      = tree_of(D, cmp);
+    s.bx       = basecube[0];
+    s.by       = basecube[1];
+    s.bz       = basecube[2];
+    s.bw       = basecube[3];
+    s.maxlevel = maxlevel;
+    s.iter_cap = 1 << 24;
+    const int* ref = S.ref.data();
+    if (cmp) grid ? run<true, false, true>(s, ref, org, dir, n, status, leaf, isp, exps)
+             : run<true, false, false>(s, ref, org, dir, n, status, leaf, isp, exps);
+    else grid ? run<false, false, true>(s, ref, org, dir, n, status, leaf, isp, exps)
+              : run<false, false, false>(s, ref, org, dir, n, status, leaf, isp, exps);
+    (void) Lv;
     return 0;
 }

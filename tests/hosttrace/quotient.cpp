@@ -20,7 +20,7 @@ inline float plane_t(float a, float d, int* fast)
     r.ix = recip_or_nan(d), r.iy = 1.0f, r.iz = 1.0f;
     if (fast) {
         const float q = a * r.ix;
-        *fast = (std::fabs(q) >= 0x1p-100f && std::fabs(q) <= 0x1p+120f && std::fabs(a) >= 0x1p-100f) ? 1 : 0;
+        *fast = (std::fabs(q) >= 0x1p-80f && std::fabs(q) <= 0x1p+120f) ? 1 : 0;
     }
     return plane_k<kSemShader>(0, a, r).w;
 }

@@ -125,7 +125,7 @@ def test_config2_full_frames(cfg2):
         _compare(g, o, f"config 2 pose {i}")
         assert st["expansions"] == int(o["exp"].sum())
         assert st["hits"] == int(np.count_nonzero(o["flags"] & po.PX_SHADED))
-        assert (o["hit"] >= 0).mean() > 0.2  # depth-12 leaves leave gaps in 10M-point surfaces
+        assert (o["hit"] >= 0).mean() > 0.05  # depth-12 leaves leave gaps in 10M-point surfaces
 
 
 def test_config2_all_64_poses_bands(cfg2):

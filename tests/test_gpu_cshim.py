@@ -55,7 +55,7 @@ def test_flat_files_through_the_c_shim(tmp_path):
     # what model_load_flat reads: raw little-endian float32 triples
     fp, fc, fn = (np.fromfile(x, dtype="<f4").reshape(-1, 3) for x in paths)
     cam, ang, lighta, quality, shoot = (900.0, 900.0, 900.0), (np.pi, 0.1), 0.7, 6, 1
-    centre, radius = np.float32([900.0, 900.0, 1799.0]), np.float32(80.0)
+    centre, radius = np.float32([420.0, 1294.0, 1799.0]), np.float32(80.0)  # on the wall in view
     out = str(tmp_path / "frames.bin")
     args = [DEMO, *paths, out, "640", "480", *(repr(float(x)) for x in cam), repr(float(ang[0])),
             repr(float(ang[1])), repr(lighta), str(quality), str(depth), "1800", str(shoot),
@@ -106,5 +106,5 @@ def test_flat_files_through_the_c_shim(tmp_path):
         o = po.OracleScene(nodes, fc, fn, sc.basecube, depth).render(cam, ang, W, Hh, light, shoot=shoot)
         mm = H.compare_frames(frames[i], o)
         assert all(v == 0 for v in mm.values()), f"frame {i + 1}: {mm}"
-        assert (o["hit"] >= 0).mean() > 0.2
+        assert (o["hit"] >= 0).mean() > 0.03
     assert np.count_nonzero(frames[0]["leaf"] != frames[1]["leaf"]) > 50  # the edit is in view

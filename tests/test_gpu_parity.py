@@ -264,7 +264,10 @@ def test_trace_rays(ctx, semantics):
 
 def test_range_updates(ctx):
     """In-place zeroing and appended octets through partial uploads
-    (modelutil.c:425-500): the device tree equals the edited host tree."""
+    (modelutil.c:425-500): the device tree equals the edited host tree, and
+    the frame rendered after the edits equals the oracle's. The compact layout
+    takes them as an in-place patch (cleared live bits, moved child blocks,
+    appended subtrees), not a relayout."""
     sc, pnt, nrm, col, _ = H.scene(1, npoints=300000)
     tree = q.octree_from_points(pnt, sc.basecube, sc.depth)
     nodes = tree.nodes()
@@ -275,6 +278,12 @@ def test_range_updates(ctx):
     ctx.upload(big, q._ffi.TYPE_INT, tree.len * 48, 16, 0, tree.len * 48, q._ffi.BUFFER_STATIC_OCTREE)
     ctx.upload_model(col, nrm)
     ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
+    osc0 = po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth)
+    g, o = _render_both(ctx, osc0, (900.0, 900.0, 900.0), (np.pi, 0.0), 160, 120, sc.lights, sc.basecube,
+                        sc.depth)
+    _assert_same(g, o, "before edits")
+    leaf_before = o["leaf"]
+    rel0, pat0, _ = ctx.layout_stats()
     rng = np.random.default_rng(3)
     victims = pnt[rng.choice(pnt.shape[0], 3000, replace=False)]
     for p in victims:  # punch: zero slots, upload the one modified node
@@ -298,6 +307,82 @@ def test_range_updates(ctx):
     g, o = _render_both(ctx, osc, (900.0, 900.0, 900.0), (np.pi, 0.0), 160, 120, sc.lights, sc.basecube,
                         sc.depth)
     _assert_same(g, o, "after edits")
+    assert np.count_nonzero(o["leaf"] != leaf_before) > 0  # the edits are in view
+    rel1, pat1, _ = ctx.layout_stats()
+    if ctx.tuning()["layout"] == 0 and ctx.tuning()["path"] == 1:
+        assert (rel1, pat1) == (rel0, pat0 + 1), "the edits were not patched in place"
+    # a second batch on the patched layout: removals of re-inserted points
+    # (single-node uploads) and a range of mostly unchanged nodes
+    for p in newp[:50]:
+        m, o_ = tree.remove_point(p)
+        if o_ >= 0:
+            ctx.upload(tree.nodes_view(), q._ffi.TYPE_INT, tree.len * 48, 16, o_ * 48, (o_ + 1) * 48,
+                       q._ffi.BUFFER_STATIC_OCTREE)
+    nv = tree.nodes()
+    ctx.upload(nv, q._ffi.TYPE_INT, tree.len * 48, 16, 48, min(tree.len, 4000) * 48, q._ffi.BUFFER_STATIC_OCTREE)
+    osc2 = po.OracleScene(nv, col, nrm, sc.basecube, sc.depth)
+    g, o = _render_both(ctx, osc2, (900.0, 900.0, 900.0), (np.pi, 0.0), 160, 120, sc.lights, sc.basecube,
+                        sc.depth)
+    _assert_same(g, o, "second edit batch")
+    if ctx.tuning()["layout"] == 0 and ctx.tuning()["path"] == 1:
+        assert ctx.layout_stats()[:2] == (rel0, pat0 + 2)
+
+
+def test_patch_fallbacks(ctx):
+    """Range uploads the in-place patch does not take go to a relayout: a
+    root slot re-pointed at an appended copy of its child (the copy's
+    children are still laid out under the detached child), and the slot
+    restored (the old child's children are now laid out under the copy).
+    Frames equal the oracle in every case."""
+    sc, pnt, nrm, col, nodes = H.scene(1, npoints=50000)
+    _upload_scene(ctx, nodes, col, nrm)
+    osc = po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth)
+    cam, ang = (900.0, 900.0, 900.0), (np.pi, 0.0)
+    g, o = _render_both(ctx, osc, cam, ang, 96, 64, sc.lights, sc.basecube, sc.depth)
+    _assert_same(g, o, "start")
+    n = nodes.copy()
+    root = n[0]
+    kids = [k for k in range(8) if root[k] > 0]
+    # (a) replace root child kids[0] by an appended copy of that subtree's root node
+    copy_ = n[root[kids[0]]].copy()
+    n2 = np.vstack([n, copy_[None]])
+    n2[0, kids[0]] = n.shape[0]
+    ctx.upload(n2, q._ffi.TYPE_INT, n2.shape[0] * 48, 16, 0, 48, q._ffi.BUFFER_STATIC_OCTREE)
+    ctx.upload(n2, q._ffi.TYPE_INT, n2.shape[0] * 48, 16, n.shape[0] * 48, n2.shape[0] * 48,
+               q._ffi.BUFFER_STATIC_OCTREE)
+    osc2 = po.OracleScene(n2, col, nrm, sc.basecube, sc.depth)
+    g, o = _render_both(ctx, osc2, cam, ang, 96, 64, sc.lights, sc.basecube, sc.depth)
+    _assert_same(g, o, "root child re-pointed at a copy")
+    rel1 = ctx.layout_stats()[0]
+    # (b) restore the original slot: a tree again
+    ctx.upload(n, q._ffi.TYPE_INT, n2.shape[0] * 48, 16, 0, 48, q._ffi.BUFFER_STATIC_OCTREE)
+    n3 = n2.copy()
+    n3[0] = n[0]
+    osc3 = po.OracleScene(n3, col, nrm, sc.basecube, sc.depth)
+    g, o = _render_both(ctx, osc3, cam, ang, 96, 64, sc.lights, sc.basecube, sc.depth)
+    _assert_same(g, o, "restored")
+    if ctx.tuning()["layout"] == 0 and ctx.tuning()["path"] == 1:
+        assert ctx.layout_stats()[0] == rel1 + 1
+
+
+def test_built_tree_needs_no_relayout(ctx):
+    """qbt_build_octree writes the compact layout with the node array: renders
+    use it with no relayout, equal to the oracle on the downloaded array."""
+    sc = q.Scene(1, seed=4, npoints=200000)
+    pnt, nrm, col = sc.points()
+    ctx.upload_points(pnt)
+    ctx.upload_model(col, nrm)
+    ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
+    rel0, pat0, _ = ctx.layout_stats()
+    nn = ctx.build_octree(sc.basecube, sc.depth, pnt.shape[0])
+    nodes = ctx.download_octree(nn)
+    osc = po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth)
+    g, o = _render_both(ctx, osc, (900.0, 900.0, 900.0), (2.2, -0.1), 128, 96, sc.lights, sc.basecube, sc.depth)
+    _assert_same(g, o, "built tree")
+    rel1, pat1, rec = ctx.layout_stats()
+    assert (rel1, pat1) == (rel0, pat0)
+    if ctx.tuning()["layout"] == 0 and ctx.tuning()["path"] == 1:
+        assert ctx.stats()["compact"] == 1 and rec == nn
 
 
 def test_update_entry_point(ctx):

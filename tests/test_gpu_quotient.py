@@ -3,9 +3,9 @@ against the device's IEEE division.
 
 plane_k takes t = (c - p) / d from a per-ray reciprocal y = RN(1/d) as
 q = RN(a y), e = RN(a - d q), t = RN(q + e y) where |d| is in
-[2^-100, 2^100], |a| >= 2^-100 and |q| is in [2^-100, 2^120], and divides
-elsewhere. Inside that range y, q and t are normal numbers and e is exact or
-normal (the exact a - d q is a multiple of 2^(e_a - 47) >= 2^-147), so every
+[2^-20, 2^100] and |q| is in [2^-80, 2^120], and divides elsewhere. Inside
+that range |a| > 2^-101, y, q and t are normal numbers and e is exact (the
+exact a - d q is a multiple of 2^(e_a - 47) >= 2^-148), so every
 step commutes with scaling a and d by powers of two, and signs are symmetric:
 t = RN(a / d) for all such operands exactly when it holds for every pair of
 significands a, d in [1, 2). This test checks all 2^46 pairs on the device

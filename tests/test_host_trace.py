@@ -33,6 +33,8 @@ def ht(tmp_path_factory):
     f32p, i32p, i64p = C.POINTER(C.c_float), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
     lib.ht_trace.argtypes = [i32p, C.c_size_t, i32p, C.c_size_t, f32p, C.c_int, C.c_int, C.c_int, f32p, f32p,
                              C.c_size_t, i32p, i32p, f32p, i64p]
+    lib.ht_trace_patched.argtypes = [i32p, i32p, C.c_size_t, f32p, C.c_int, C.c_int, C.c_int, f32p, f32p,
+                                     C.c_size_t, i32p, i32p, f32p, i64p]
     return lib
 
 
@@ -132,3 +134,45 @@ def test_non_grid_basecube_running_sum(ht):
     nodes = q.octree_from_points(pts, bc, 8).nodes()
     org, dirs = rays(rng, 3000, bc)
     check(ht, nodes, bc, 8, org, dirs, grids=(0,))
+
+
+@pytest.mark.parametrize("seed,frac,leaves_only", [(11, 0.08, False), (12, 0.6, True)])
+def test_zeroed_slots_on_the_compact_layout(ht, seed, frac, leaves_only):
+    """In-place range updates that zero child slots (octree_remove_point,
+    octree.c:182-218) keep the compact records and clear live bits; the
+    child block keeps its slots (alloc), so the ranks of the other children
+    stay. Traces on that layout (and on the direct layout of the edited array)
+    equal the oracle on the edited array: status, leaf, entry point bits and
+    expansion counts. Slots are zeroed at every depth (whole subtrees drop)."""
+    rng = np.random.default_rng(seed)
+    bc = (0.0, 1800.0, 1800.0, 1800.0)
+    levels = 8
+    pts = rng.uniform(1.0, 1799.0, (20000, 3)).astype(np.float32)
+    nodes = q.octree_from_points(pts, bc, levels).nodes()
+    live = nodes.copy()
+    nz = np.argwhere(live[:, :8] > 0)
+    if leaves_only:  # slots that point at leaves, as a punched hole removes them
+        nz = nz[(nodes[live[nz[:, 0], nz[:, 1]], :8] == 0).all(1)]
+    pick = nz[rng.uniform(size=len(nz)) < frac]
+    live[pick[:, 0], pick[:, 1]] = 0
+    live[rng.integers(0, live.shape[0], 500), 8] += 7  # slot 8 changes travel too
+    org, dirs = rays(rng, 3000)
+    ref = oracle_trace(live, bc, levels, org, dirs)
+    assert (ref["status"] == po.HIT).sum() > 300
+    i32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))
+    f32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    n = org.shape[0]
+    bca = np.asarray(bc, np.float32)
+    for layout in (0, 1):
+        for grid in (0, 1):
+            st, lf = np.empty(n, np.int32), np.empty(n, np.int32)
+            isp, ex = np.empty((n, 4), np.float32), np.empty(n, np.int64)
+            rc = ht.ht_trace_patched(i32(nodes), i32(live), nodes.shape[0], f32(bca), levels, layout, grid,
+                                     f32(org), f32(dirs), n, i32(st), i32(lf), f32(isp),
+                                     ex.ctypes.data_as(C.POINTER(C.c_int64)))
+            assert rc == 0
+            what = f"layout {layout} grid {grid}"
+            np.testing.assert_array_equal(st, ref["status"], err_msg=what)
+            np.testing.assert_array_equal(lf, ref["leaf"], err_msg=what)
+            np.testing.assert_array_equal(isp.view(np.uint32), ref["isp"].view(np.uint32), err_msg=what)
+            np.testing.assert_array_equal(ex, ref["exps"], err_msg=what)

@@ -4,8 +4,9 @@ IEEE division, on the CPU.
 The reference divides: t = (c - p) / d (octree_fsh.c:67, 80, 93). The
 traversal computes t from a per-ray reciprocal y = RN(1/d) as q = RN(a y),
 e = RN(a - d q), t = RN(q + e y) (FMAs), and divides only outside
-|d| in [2^-100, 2^100], |a| >= 2^-100, |q| in [2^-100, 2^120]. Inside that
-range every step is exact or lands on a normal number, so the result scales
+|d| in [2^-20, 2^100], |q| in [2^-80, 2^120] (which bound |a| below by
+2^-101). Inside that range every step is exact or lands on a normal number,
+so the result scales
 with the exponents of a and d and depends only on their significands; the
 GPU test tests/test_gpu_quotient.py checks all 2^46 significand pairs on the
 device build. Here the host build of the same code (g++, -ffp-contract=off,
@@ -81,8 +82,9 @@ def _f(bits):
 
 def test_hard_operands(qc):
     """Quotient significands near 2 and 1, divisors at and next to powers of
-    two, |q| at both switch points (2^-100, 2^120), |a| and |d| at their
-    switch points, subnormal and tiny a, and exact coordinate differences."""
+    two, |q| at both switch points (2^-80, 2^120), |d| at its switch points
+    (2^-20, 2^100), |a| at the smallest the fast path admits (2^-101 with
+    |d| = 2^-20), subnormal and tiny a, and exact coordinate differences."""
     one_m = np.float32(2.0) - np.float32(2.0 ** -23)  # 2 - ulp
     sig = np.concatenate([1 + np.arange(0, 64) * 2.0 ** -23, 2 - np.arange(1, 65) * 2.0 ** -23,
                           1 + np.arange(0, 64) * 2.0 ** -12, [1.5, 1.25, 1.75, 4.0 / 3, 1.1]]).astype(np.float32)
@@ -90,18 +92,23 @@ def test_hard_operands(qc):
     A, D = A.ravel(), D.ravel()
     cases = [(A, D), (-A, D), (A, -D), (A * np.float32(2.0 ** 40), D * np.float32(2.0 ** -30)),
              (A * np.float32(2.0 ** -90), D * np.float32(2.0 ** 10))]
-    # |q| around 2^-100 and 2^120; |d| around 2^-100 and 2^100; |a| around 2^-100
+    # |q| around 2^-80 and 2^120; |d| around 2^-20 and 2^100; |a| around 2^-101
     e = np.float32(1.0 + 2.0 ** -23)
     for s in (np.float32(1.0), one_m, e):
-        for q in (2.0 ** -100, 2.0 ** 120):
+        for q in (2.0 ** -80, 2.0 ** 120):
             for lo in (0.5, 1.0, 2.0):
                 a = np.float32(q * lo) * s
                 cases.append((np.array([a, -a]), np.array([np.float32(1.0), np.float32(1.0)])))
                 cases.append((np.array([a * np.float32(2.0 ** 5)]), np.array([np.float32(2.0 ** 5) * e])))
-        for dd in (2.0 ** -101, 2.0 ** -100, 2.0 ** -99, 2.0 ** 99, 2.0 ** 100, 2.0 ** 101):
+        for dd in (2.0 ** -21, 2.0 ** -20, 2.0 ** -19, 2.0 ** 99, 2.0 ** 100, 2.0 ** 101):
             cases.append((np.float32([1.0, 1.5, 1e-20, 1e20]) * s, np.full(4, np.float32(dd) * s)))
-        for aa in (2.0 ** -101, 2.0 ** -100, 2.0 ** -99, 2.0 ** -126, 2.0 ** -140):
-            cases.append((np.full(4, np.float32(aa) * s), np.float32([2.0 ** -60, 2.0 ** -30, 1.0, 3.0])))
+        for aa in (2.0 ** -102, 2.0 ** -101, 2.0 ** -100, 2.0 ** -99, 2.0 ** -126, 2.0 ** -140):
+            cases.append((np.full(4, np.float32(aa) * s), np.float32([2.0 ** -60, 2.0 ** -30, 2.0 ** -20, 1.0])))
+        # q just above 2^-80 from the smallest a: d at 2^-20 and just above
+        for dd in (2.0 ** -20, 2.0 ** -20 * (1 + 2.0 ** -23), 2.0 ** -19 * (1 - 2.0 ** -24)):
+            a = (np.float32(2.0 ** -100) * sig[:32] * s).astype(np.float32)
+            a = np.concatenate([a, -a] + [a * np.float32(2.0 ** -k) for k in range(1, 8)])
+            cases.append((a, np.full(a.size, np.float32(dd))))
     sub = _f(np.concatenate([np.arange(1, 64), np.arange(0x7FFFC0, 0x800040)]))  # subnormals, the normal edge
     cases.append((sub, np.full(sub.size, np.float32(2.0 ** -60))))
     cases.append((sub, np.full(sub.size, np.float32(3.0 * 2.0 ** -90))))
@@ -135,7 +142,7 @@ def test_random_scene_range(qc):
     rng = np.random.default_rng(2)
     n = 1 << 22
     a = (rng.uniform(-3000, 3000, n) * 2.0 ** rng.integers(-30, 1, n)).astype(np.float32)
-    d = (rng.normal(size=n) * 2.0 ** rng.integers(-20, 12, n)).astype(np.float32)
+    d = (rng.normal(size=n) * 2.0 ** rng.integers(-8, 12, n)).astype(np.float32)
     fast = _assert_ieee(qc, a, d, "scene range")
     assert fast.mean() > 0.99
 
