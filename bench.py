@@ -113,6 +113,8 @@ def parse():
     ap.add_argument("--batch-reps", type=int, default=3, help="timed batches of the batched leg")
     ap.add_argument("--traffic", default=TRAFFIC_FILE)
     ap.add_argument("--summarize-pmc", nargs="*", default=None, metavar="DIR")
+    ap.add_argument("--summarize-mfma", nargs="*", default=None, metavar="DIR",
+                    help="print the MLP kernels' matrix-core counters (rocprofv3 output folders)")
     ap.add_argument("--path", type=int, default=None, help="render path (0 fused, 1 wavefront); default: library's")
     ap.add_argument("--block", type=int, default=None, help="threads per block (64/128/256)")
     ap.add_argument("--refill", type=int, default=None, help="idle lanes that trigger a wave refill (1..64)")
@@ -179,6 +181,58 @@ def summarize_pmc(dirs, out_path):
     with open(out_path, "w") as fh:
         json.dump(res, fh, indent=1, sort_keys=True)
     print(json.dumps(res, indent=1, sort_keys=True))
+
+
+MLP_KERNELS = ("k_mlp_shadow", "k_mlp_infer", "k_gemm")
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+SIMDS = 1024  # 256 CUs x 4 SIMDs
+
+
+def summarize_mfma(dirs):
+    """Matrix-core counters of the MLP kernels: per-dispatch averages of
+    SQ_INSTS_VALU_MFMA_MOPS_BF16 (units of 512 flops), SQ_INSTS_MFMA,
+    SQ_VALU_MFMA_BUSY_CYCLES and GRBM_GUI_ACTIVE from counter_collection.csv,
+    against the average durations in kernel_stats.csv of a --kernel-trace
+    --stats run. Prints a markdown table."""
+    vals, durs = {}, {}
+    pat = re.compile(r"\b(%s)\b" % "|".join(MLP_KERNELS))
+    for d in dirs:
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    m = pat.search(row.get("Kernel_Name", ""))
+                    if m:
+                        key = (m.group(1), row["Counter_Name"])
+                        vals.setdefault(key, {})[(f, row.get("Dispatch_Id"))] = float(row["Counter_Value"])
+        for f in glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    m = pat.search(row.get("Name", ""))
+                    if m:
+                        c, tot = int(row["Calls"]), float(row["TotalDurationNs"])
+                        a = durs.setdefault(m.group(1), [0, 0.0])
+                        a[0], a[1] = a[0] + c, a[1] + tot
+    avg = {k: float(np.mean(list(v.values()))) for k, v in vals.items()}
+    n = {k: len(v) for k, v in vals.items()}
+    lines = ["| kernel | dispatches | bf16 flops / dispatch (MOPS x 512) | MFMA insts / dispatch | avg duration (us) "
+             "| achieved TFLOP/s | frac of 2.5 PFLOP/s | MFMA busy (busy / (GUI_ACTIVE/8 x 1024 SIMDs)) |",
+             "|---|---|---|---|---|---|---|---|"]
+    for k in MLP_KERNELS:
+        mops = avg.get((k, "SQ_INSTS_VALU_MFMA_MOPS_BF16"))
+        if mops is None:
+            continue
+        flops = mops * 512.0
+        ins = avg.get((k, "SQ_INSTS_MFMA"))
+        busy, gui = avg.get((k, "SQ_VALU_MFMA_BUSY_CYCLES")), avg.get((k, "GRBM_GUI_ACTIVE"))
+        dc = durs.get(k)
+        us = dc[1] / dc[0] / 1e3 if dc and dc[0] else None
+        tf = flops / (us * 1e-6) / 1e12 if us else None
+        util = busy / (gui / 8.0 * SIMDS) if busy is not None and gui else None
+        f = lambda x, fmt: (fmt % x) if x is not None else "n/a"
+        lines.append(f"| {k} | {n[(k, 'SQ_INSTS_VALU_MFMA_MOPS_BF16')]} | {flops:.4g} | {f(ins, '%.4g')} | "
+                     f"{f(us, '%.1f')} | {f(tf, '%.1f')} | {f(tf / MFMA_BF16_PEAK_TFLOPS if tf else None, '%.4f')} | "
+                     f"{f(util, '%.4f')} |")
+    print("\n".join(lines))
 
 
 def load_traffic(path):
@@ -275,6 +329,19 @@ def surrogate_leg(args, q, ctx, sc, views, Wm, K, mask, exact_wall, exact_rays, 
         resolved += st["primary_rays"] + st["predicted"] + st["shadow_rays"]
         b, t = S.shadow_mismatch(ex, su, len(lights))
         bad, tot = bad + b, tot + t
+    # the standalone inference entry point (k_mlp_infer) on 4 M queries: hit
+    # points uniform in the cube, random unit normals, the scene's lights
+    rng = np.random.default_rng(args.seed + 5)
+    nq = 1 << 22
+    bx, by, bz, bw = (float(v) for v in sc.basecube)
+    ip = (np.array([bx, by - bw, bz - bw], np.float32) + rng.random((nq, 3), np.float32) * np.float32(bw))
+    nv = rng.normal(size=(nq, 3)).astype(np.float32)
+    nv /= np.linalg.norm(nv, axis=1, keepdims=True)
+    lq = np.asarray(lights, np.float32)[np.arange(nq) % len(lights)]
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    pq = ctx.surrogate_predict(ip, nv, lq, sc.basecube)
+    infer_s = time.perf_counter() - t0
     value = exact_rays / wall / 1e6
     km, kl = prof["kind_ms"], prof["kind_launches"]
     res = {"value": round(value, 3), "unit": "Mrays/s", "speedup_vs_exact": round(exact_wall / wall, 4),
@@ -286,6 +353,8 @@ def surrogate_leg(args, q, ctx, sc, views, Wm, K, mask, exact_wall, exact_rays, 
                                    "shadow": round(km[KIND_SHADOW] / K, 4),
                                    "mlp_decision": round(km[KIND_SURROGATE] / K, 4)},
            "mlp_decision_launches": kl[KIND_SURROGATE],
+           "predict_entry": {"queries": nq, "seconds_with_copies": round(infer_s, 4),
+                             "mean_p": round(float(np.mean(pq)), 4)},
            "train": {"frames": args.sur_frames, "samples": info["samples"], "stride": stride,
                      "lit_fraction": round(info["lit_fraction"], 4), "adam_steps": info["steps"],
                      "batch": args.sur_batch, "lr": args.sur_lr, "final_steps": args.sur_final,
@@ -563,6 +632,9 @@ def main():
     args = parse()
     if args.summarize_pmc is not None:
         summarize_pmc(args.summarize_pmc, args.traffic)
+        return
+    if args.summarize_mfma is not None:
+        summarize_mfma(args.summarize_mfma)
         return
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for --gpus {args.gpus}); "

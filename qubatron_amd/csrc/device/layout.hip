@@ -94,19 +94,6 @@ constexpr int kNew = 1 << 30;
 /* patch state words: failure, records in use, grow-list and new-list lengths */
 enum { PS_FAIL = 0, PS_LEN = 1, PS_GROW = 2, PS_NEW = 3, PS_WORDS = 64 };
 
-/* the dirty node of item g: ranges[r].x + g - roff[r] for the last r with roff[r] <= g */
-__device__ __forceinline__ size_t range_item(const uint2* __restrict__ ranges, const uint32_t* __restrict__ roff,
-                                             int nr, uint32_t g)
-{
-    int lo = 0, hi = nr - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (roff[mid] <= g) lo = mid;
-        else hi = mid - 1;
-    }
-    return (size_t) ranges[lo].x + (g - roff[lo]);
-}
-
 /* link reference node v at record p: claim r2c[v] (v must not be in the
  * tree), fill the record's ref and slot 8, and queue v for k_patch_new */
 __device__ __forceinline__ void link_new(int v, uint32_t p, size_t n, const int* __restrict__ model,

@@ -113,6 +113,21 @@ __global__ void k_validate(const int4* __restrict__ child, size_t lo, size_t hi,
     if (!ok) atomicAdd(bad, 1u);
 }
 
+/* the same over a list of node ranges (total nodes in all of them) */
+__global__ void k_validate_ranges(const int4* __restrict__ child, const uint2* __restrict__ ranges,
+                                  const uint32_t* __restrict__ roff, int nr, uint32_t total, int n,
+                                  unsigned int* __restrict__ bad)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const size_t i = range_item(ranges, roff, nr, g);
+    int4         a = child[2 * i], b = child[2 * i + 1];
+    bool ok = (unsigned) a.x < (unsigned) n && (unsigned) a.y < (unsigned) n && (unsigned) a.z < (unsigned) n &&
+              (unsigned) a.w < (unsigned) n && (unsigned) b.x < (unsigned) n && (unsigned) b.y < (unsigned) n &&
+              (unsigned) b.z < (unsigned) n && (unsigned) b.w < (unsigned) n;
+    if (!ok) atomicAdd(bad, 1u);
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n)
 {
     for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
@@ -558,7 +573,7 @@ extern "C" void qbt_destroy(qbt_ctx* c)
     for (DevBuf* b : {&c->wb_isp, &c->wb_ms, &c->wb_md, &c->wb_leaf, &c->wb_flags, &c->wb_disc, &c->wb_lstat,
                       &c->wb_exp, &c->wb_cand, &c->wb_items, &c->wb_q, &c->wb_lists, &c->wb_fb, &c->wb_gstack,
                       &c->wb_bcnt, &c->wb_lexp,
-                      &c->bt_tab, &c->valflag})
+                      &c->bt_tab, &c->valflag, &c->valrg})
         c->dev_free(*b);
     for (int k = 0; k < qbt_ctx::kTabSlots; k++) {
         if (c->tab_host[k]) (void) hipHostFree(c->tab_host[k]);
@@ -600,6 +615,7 @@ static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned) ((n
  * and applied together (the reference uploads one 48-byte node per edited
  * node, modelutil.c:425-500: hundreds of calls per edit) */
 static const size_t kSmallTexels = 96;
+static const size_t kValRanges   = 65536; /* written node ranges checked one by one */
 static const size_t kPendMax     = kStageBytes / 24; /* 16-byte value + 8-byte index each */
 
 int qbt_ctx::flush_pending(int t, bool discard)
@@ -676,6 +692,8 @@ extern "C" int qbt_upload(qbt_ctx* c, const void* data, int type, size_t size, s
             size_t lo = t0 / 3, hi = (t1 + 2) / 3;
             if (tb.dirty_hi <= tb.dirty_lo) tb.dirty_lo = lo, tb.dirty_hi = hi;
             else tb.dirty_lo = lo < tb.dirty_lo ? lo : tb.dirty_lo, tb.dirty_hi = hi > tb.dirty_hi ? hi : tb.dirty_hi;
+            if (tb.vdirty.size() < kValRanges) tb.vdirty.emplace_back(lo, hi);
+            else tb.vdirty_over = true;
         }
         tb.validated = false;
         /* the compact layout: relayout after a whole-array upload, else patch
@@ -827,6 +845,7 @@ int qbt_ctx::validate_tree(int t, unsigned* flag)
         return QBT_OK;
     }
     size_t lo = tb.full_check ? 0 : tb.dirty_lo, hi = tb.full_check ? tb.n : (tb.dirty_hi < tb.n ? tb.dirty_hi : tb.n);
+    const bool own_flag_given = flag != nullptr;
     if (!flag) {
         if (!valflag.p) {
             int rc = dev_alloc(valflag, 256);
@@ -835,17 +854,55 @@ int qbt_ctx::validate_tree(int t, unsigned* flag)
         flag = (unsigned*) valflag.p;
     }
     k_fill_u32<<<1, 64, 0, stream>>>((uint32_t*) flag, 1, 0u);
-    if (hi > lo)
+    /* scattered writes (an edit touches nodes all over the array): check the
+     * written ranges only, when that is much less than their bounding range
+     * (not for a caller's flag word: that path allocates nothing) */
+    std::vector<uint2>    rg;
+    std::vector<uint32_t> roff;
+    uint64_t              total = 0;
+    if (!tb.full_check && !tb.vdirty_over && !own_flag_given && tb.vdirty.size() > 1) {
+        std::sort(tb.vdirty.begin(), tb.vdirty.end());
+        size_t a = 0, b = 0;
+        for (size_t k = 0; k <= tb.vdirty.size(); k++) {
+            const bool flush = k == tb.vdirty.size() || tb.vdirty[k].first > b;
+            if (flush && b > a) {
+                const size_t bb = b < tb.n ? b : tb.n;
+                if (bb > a) {
+                    rg.push_back(make_uint2((uint32_t) a, (uint32_t) bb));
+                    roff.push_back((uint32_t) total);
+                    total += bb - a;
+                }
+            }
+            if (k == tb.vdirty.size()) break;
+            if (flush) a = tb.vdirty[k].first, b = tb.vdirty[k].second;
+            else b = tb.vdirty[k].second > b ? tb.vdirty[k].second : b;
+        }
+        if (total * 4 > hi - lo) rg.clear(); /* not worth a second kernel shape */
+    }
+    if (!rg.empty()) {
+        const size_t bytes = rg.size() * 12;
+        if (int rc = dev_alloc(valrg, bytes)) return rc;
+        std::vector<uint8_t> pack(bytes);
+        memcpy(pack.data(), rg.data(), rg.size() * 8);
+        memcpy(pack.data() + rg.size() * 8, roff.data(), rg.size() * 4);
+        HIPCHK(hipMemcpyAsync(valrg.p, pack.data(), bytes, hipMemcpyHostToDevice, stream));
+        k_validate_ranges<<<blocks_for(total, 256), 256, 0, stream>>>(
+            (const int4*) tb.child.p, (const uint2*) valrg.p, (const uint32_t*) ((char*) valrg.p + rg.size() * 8),
+            (int) rg.size(), (uint32_t) total, (int) tb.n, (unsigned int*) flag);
+    } else if (hi > lo) {
         k_validate<<<blocks_for(hi - lo, 256), 256, 0, stream>>>((const int4*) tb.child.p, lo, hi, (int) tb.n,
                                                                  (unsigned int*) flag);
+    }
     HIPCHK(hipGetLastError());
     unsigned int nbad = 0;
     HIPCHK(hipMemcpyAsync(&nbad, flag, 4, hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
+    HIPCHK(hipStreamSynchronize(stream)); /* also frees the host copy of the ranges */
     if (nbad) return QBT_ETREE;
     tb.validated  = true;
     tb.full_check = false;
     tb.dirty_lo = tb.dirty_hi = 0;
+    tb.vdirty.clear();
+    tb.vdirty_over = false;
     return QBT_OK;
 }
 

@@ -31,6 +31,21 @@ enum : uint8_t {
     LS_UNSURE   = 6  /* surrogate: inside the band, to be traced exactly */
 };
 
+/* Item g of a list of node ranges [ranges[r].x, ranges[r].y) numbered
+ * consecutively (roff[r] = items before range r): the node
+ * ranges[r].x + g - roff[r] for the last r with roff[r] <= g */
+__device__ __forceinline__ size_t range_item(const uint2* __restrict__ ranges, const uint32_t* __restrict__ roff,
+                                             int nr, uint32_t g)
+{
+    int lo = 0, hi = nr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (roff[mid] <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    return (size_t) ranges[lo].x + (g - roff[lo]);
+}
+
 struct DevBuf {
     void*  p   = nullptr;
     size_t cap = 0; /* bytes */
@@ -54,7 +69,10 @@ struct TreeBuf {
     size_t cap_nodes = 0;
     bool   validated = true;
     bool   full_check = false;
-    size_t dirty_lo = 0, dirty_hi = 0;
+    size_t dirty_lo = 0, dirty_hi = 0; /* bounding range of the node ranges written since the last check */
+    /* the written ranges themselves (up to kValRanges; vdirty_over: more) */
+    std::vector<std::pair<size_t, size_t>> vdirty;
+    bool                                   vdirty_over = false;
     CompactTree cmp;
     bool        cmp_stale = true; /* node array replaced since the last layout: relayout */
     /* node ranges written by range uploads since the last layout: patched in
@@ -157,7 +175,7 @@ struct qbt_ctx {
     size_t       held      = 0;
     hipStream_t  stream    = nullptr;
     hipEvent_t   ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
-    DevBuf       stage, counters, sprite, rays, valflag;
+    DevBuf       stage, counters, sprite, rays, valflag, valrg;
     TreeBuf      tree[2];
     AttrBuf      attr[2];
     PointsBuf    pts[2];

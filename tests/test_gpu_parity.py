@@ -417,6 +417,33 @@ def test_invalid_tree_rejected(ctx):
     assert e.value.code == q._ffi.QBT_ETREE
 
 
+def test_scattered_range_uploads_checked(ctx):
+    """Single-node uploads scattered over the array are checked node range by
+    node range: a bad child index in one of them is QBT_ETREE; re-uploading
+    that node restores the frame exactly."""
+    sc, pnt, nrm, col, nodes = H.scene(1, npoints=20000)
+    _upload_scene(ctx, nodes, col, nrm)
+    v = q.view((900, 900, 900), (0.3, 0.1), 48, 32, sc.basecube, sc.depth)
+    ctx.render(v, sc.lights, q.OUT_ALL)
+    ref = ctx.read_frame(0)
+    n = nodes.shape[0]
+    T = q._ffi
+    bad = nodes.copy()
+    k = n // 2
+    bad[k, 5] = n + 5
+    print(f"[case] scattered uploads: n={n}, bad node {k}", flush=True)
+    for o in (3, k, n - 2):
+        ctx.upload(bad, T.TYPE_INT, n * 48, 16, o * 48, (o + 1) * 48, T.BUFFER_STATIC_OCTREE)
+    with pytest.raises(q.QbtError) as e:
+        ctx.render(v, sc.lights, q.OUT_ALL)
+    assert e.value.code == q._ffi.QBT_ETREE
+    ctx.upload(nodes, T.TYPE_INT, n * 48, 16, k * 48, (k + 1) * 48, T.BUFFER_STATIC_OCTREE)
+    ctx.render(v, sc.lights, q.OUT_ALL)
+    got = ctx.read_frame(0)
+    mm = H.compare_frames(got, ref)
+    assert all(x == 0 for x in mm.values()), mm
+
+
 def test_upload_argument_checks(ctx):
     nodes = np.zeros((2, 12), np.int32)
     T = q._ffi
