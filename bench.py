@@ -110,6 +110,8 @@ def parse():
     ap.add_argument("--dyn-frames", type=int, default=5, help="frames of the dynamic-tree leg (0 = skip)")
     ap.add_argument("--batch-views", type=int, default=64, help="views of the batched leg (0 = skip)")
     ap.add_argument("--edit-shots", type=int, default=5, help="shots of the edit leg (0 = skip)")
+    ap.add_argument("--c5-frames", type=int, default=10,
+                    help="timed frames of the config-5 surrogate leg (0 = skip)")
     ap.add_argument("--batch-reps", type=int, default=3, help="timed batches of the batched leg")
     ap.add_argument("--traffic", default=TRAFFIC_FILE)
     ap.add_argument("--summarize-pmc", nargs="*", default=None, metavar="DIR")
@@ -183,7 +185,7 @@ def summarize_pmc(dirs, out_path):
     print(json.dumps(res, indent=1, sort_keys=True))
 
 
-MLP_KERNELS = ("k_mlp_shadow", "k_mlp_infer", "k_gemm")
+MLP_KERNELS = ("k_mlp_shadow", "k_mlp_infer", "k_mlp_train")
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 SIMDS = 1024  # 256 CUs x 4 SIMDs
 
@@ -565,6 +567,53 @@ def dynamic_leg(args, q, ctx, sc, mask, copy_gbps):
                                     "shadow": round(km[KIND_SHADOW] / K, 4)}}
 
 
+def config5_leg(args, q, ctx, nthreads, mask):
+    """The learned shadow surrogate on its own workload: the config-2 scene
+    (10 M points, depth 12, one light) at the bench resolution; a seeded camera
+    path of c5_frames exact frames timed, then the network trained online on
+    16 M samples from 256 exact frames of a separate path and the same frames
+    timed with it (surrogate_leg), with the mismatch against the exact frames."""
+    sc = q.Scene(5, seed=args.seed)
+    t0 = time.time()
+    pnt, nrm, col = sc.points(nthreads=nthreads)
+    ctx.upload_points(pnt)
+    ctx.upload_model(col, nrm)
+    ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
+    nnodes = ctx.build_octree(sc.basecube, sc.depth, sc.npoints)
+    ctx.synchronize()
+    del pnt, nrm, col
+    log(f"config 5 leg: config-2 scene ({sc.npoints} points, {nnodes} nodes, {len(sc.lights)} light) in "
+        f"{time.time() - t0:.2f}s")
+    W, H, K, Wm = args.width, args.height, args.c5_frames, args.warmup
+    lights = sc.lights
+    pos, ang = sc.poses(K + Wm, seed=args.pose_seed)
+    views = [q.view(pos[i], ang[i], W, H, sc.basecube, sc.depth) for i in range(K + Wm)]
+    rays, exps, exps_p, leafhits = count_frames(q, ctx, views[Wm:], lights, mask)
+    for i in range(Wm):
+        ctx.render(views[i], lights, mask)
+    ctx.synchronize()
+    ctx.profile_begin()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for i in range(Wm, K + Wm):
+        ctx.render(views[i], lights, mask)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    prof = ctx.profile_end()
+    total_rays = float(sum(rays))
+    km = prof["kind_ms"]
+    log(f"config 5 leg: exact {total_rays / wall / 1e6:.1f} Mrays/s, {wall * 1e3 / K:.3f} ms/frame")
+    qpf = (total_rays - K * W * H) / K
+    res = surrogate_leg(args, q, ctx, sc, views, Wm, K, mask, wall, total_rays, qpf)
+    res["exact"] = {"value": round(total_rays / wall / 1e6, 3), "ms_per_step": round(wall * 1e3 / K, 4),
+                    "kernel_ms_per_frame": {"primary": round(km[KIND_PRIMARY] / K, 4),
+                                            "shadow": round(km[KIND_SHADOW] / K, 4)}}
+    res["workload"] = (f"config 5: the config-2 scene ({sc.npoints} points, depth {sc.depth}, {len(lights)} light), "
+                       f"{W}x{H}, {K} frames of a seeded camera path, exact and with the network")
+    res["points"], res["nodes"], res["steps"] = sc.npoints, nnodes, K
+    return res
+
+
 BATCH_RES = ((320, 240), (640, 480), (1280, 720), (1920, 1080), (3840, 2160))
 
 
@@ -755,6 +804,9 @@ def main():
         edits = edit_leg(args, q, ctx, sc, pnt, nrm, nodes, mask)
         del pnt
     batch = batch_leg(args, q, ctx, nthreads, mask) if args.batch_views > 0 else None  # replaces the scene
+    c5 = None
+    if args.c5_frames > 0 and not args.no_surrogate:
+        c5 = config5_leg(args, q, ctx, nthreads, mask)  # replaces the scene
 
     res = {
         "metric": "primary+shadow Mrays/s @1080p, 90M-pt depth-12 octree; surrogate vs exact",
@@ -775,6 +827,7 @@ def main():
         "dynamic_tree": dyn,
         "edits": edits,
         "batch_views": batch,
+        "surrogate_config5": c5,
         "tuning": tuning,
         "build_s": round(t_build, 3),
         "total_s": round(time.time() - t_all, 1),

@@ -374,7 +374,10 @@ int qbt_ctx::patch_layout(int t, bool* done)
     hipStream_t  s = stream;
     int          rc;
     const size_t n = tb.n;
-    if ((rc = compact_reserve(t, cm.n, n))) return rc;
+    /* the layout's arrays must already hold every record and reference node
+     * (growing them would drop their contents; the array only outgrows them
+     * by a regrowth, which relays out) */
+    if (cm.n > cm.cap || n > cm.r2c_cap) return QBT_OK;
     if (cm.cap >= (1u << 30)) return QBT_OK; /* first-child indices are 30-bit */
     /* the ranges (merged on upload) below ref_n and their prefix offsets */
     std::vector<uint2>    rg;

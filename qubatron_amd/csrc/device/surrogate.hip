@@ -3,11 +3,12 @@
  * counterpart: the reference traces every shadow ray,
  * src/qubatron/shaders/octree_fsh.c:426-434).
  *
- *   training   feature kernel -> four MFMA GEMMs (bias + ReLU + bf16 in the
- *              epilogue) -> loss / output gradient -> per layer: weight
- *              gradient GEMM (split-K, fp32 atomics), bias row sums, the
- *              masked input gradient GEMM -> Adam on fp32 master weights ->
- *              bf16 repack
+ *   training   feature kernel -> one fused kernel per minibatch: forward
+ *              (bias + ReLU + bf16 between layers), loss and output
+ *              gradient, backward through the masked transposed weights and
+ *              the weight and bias gradients, 64 samples per wave on the
+ *              matrix cores -> sum of the per-block partial gradients ->
+ *              Adam on fp32 master weights -> bf16 repack
  *   inference  one fused kernel: weights in LDS, 16 queries per wave, the
  *              four layers chained in registers (mlp.hpp)
  *   render     the decision pass of QBT_RENDER_SURROGATE: predicted bits
@@ -54,7 +55,7 @@ constexpr int kLB1 = 0, kLB2 = 64, kLB3 = 128, kLB4 = 192, kLBN = 196;
 struct MlpState {
     DevBuf   params, m, v, grads, packed, ctr; /* ctr: collected, lit (u64) */
     DevBuf   samples, perm, keys, vals, sort_tmp;
-    DevBuf   X, H1, H2, H3, L16, dZ4, dZ3, dZ2, dZ1, lossbuf, io_x, io_p, io_y, io_a, io_c, flat;
+    DevBuf   X, gpart, lossbuf, io_x, io_p, io_y, io_a, io_c, flat; /* gpart: k_mlp_train's partial gradients */
     size_t   cap = 0;
     uint32_t stride = 1, phase = 0;
     int      t     = 0; /* Adam steps taken */
@@ -120,92 +121,245 @@ __global__ void k_mlp_pack(const float* __restrict__ P, uint16_t* __restrict__ W
     W[i] = f2bf(v);
 }
 
-/* matrix view: element (r, c) at p[r * rs + c * cs] */
-struct MatBF {
-    const uint16_t* p;
-    long            rs, cs;
-    __device__ __forceinline__ short raw(long r, long c) const { return (short) p[r * rs + c * cs]; }
+/* ------------------------------------------------------------------------ */
+/* fused training forward and backward                                      */
+/* ------------------------------------------------------------------------ */
+
+constexpr int kChunk = 64; /* samples per chunk: four fragment groups of 16, one per wave */
+
+/* one chunk's activations and gradients, [feature][kChunk] bf16 */
+struct TrainLds {
+    uint16_t x[kMlpInPad * kChunk];
+    uint16_t h1[64 * kChunk], h2[64 * kChunk], h3[64 * kChunk];
+    uint16_t d1[64 * kChunk], d2[64 * kChunk], d3[64 * kChunk], d4[16 * kChunk];
+    float    bias[kLBN];
 };
 
-enum { EPI_F32 = 0, EPI_ATOMIC = 1, EPI_BIAS_RELU = 2, EPI_MASK = 3 };
-
-struct Epi {
-    int          mode;
-    float*       of; /* EPI_F32 / EPI_ATOMIC: fp32 output */
-    uint16_t*    ob; /* EPI_BIAS_RELU / EPI_MASK: bf16 output */
-    long         ors, ocs;
-    const float* bias; /* EPI_BIAS_RELU: per row */
-    MatBF        mask; /* EPI_MASK: keep where mask > 0 (the ReLU output) */
-};
-
-/* How k_gemm reads an operand's 16 x 16 tile (host-chosen, see gemm()):
- * OP_K  the 4 values of k a lane needs are adjacent in memory: one 8-byte
- *       load per lane and step;
- * OP_T  the tile's other dimension is contiguous: 8-byte loads along it,
- *       transposed through LDS, one 8-byte LDS read per lane;
- * OP_S  4 separate 2-byte loads (any layout). */
-enum { OP_S = 0, OP_K = 1, OP_T = 2 };
-
-/* the MFMA operand of one 16 x 16 tile: element (i, k0 + 4 g + e) of a
- * matrix whose element (i, k) sits at p[i * si + k * sk] (rows of A, columns
- * of B), for i = i0 + r */
-__device__ __forceinline__ bf16x4 tile_operand(int mode, const uint16_t* __restrict__ p, long si, long sk, int i0,
-                                               int k0, int lane, uint16_t* __restrict__ lds)
+/* A fragment of W (row-major, `in` columns): rows 16 mt.., columns 16 kb.. */
+__device__ __forceinline__ bf16x4 wfrag(const uint16_t* __restrict__ W, int in, int mt, int kb, int r, int g)
 {
-    const int r = lane & 15, g = lane >> 4;
-    bf16x4    v;
-    if (mode == OP_K) {
-        v = *reinterpret_cast<const bf16x4*>(p + (long) (i0 + r) * si + (long) (k0 + 4 * g) * sk);
-    } else if (mode == OP_T) {
-        /* lane loads i = i0 + 4 (lane & 3) .. + 3 at k = k0 + (lane >> 2), stores them as lds[i][k] */
-        const int    kk = lane >> 2, ii = 4 * (lane & 3);
-        const bf16x4 w  = *reinterpret_cast<const bf16x4*>(p + (long) (i0 + ii) * si + (long) (k0 + kk) * sk);
-        for (int e = 0; e < 4; e++) lds[(ii + e) * 16 + kk] = (uint16_t) w[e];
-        __syncthreads(); /* one wave per block: orders the LDS writes before the reads */
-        v = *reinterpret_cast<const bf16x4*>(lds + r * 16 + 4 * g);
-        __syncthreads(); /* the next step overwrites the tile */
-    } else {
-        for (int e = 0; e < 4; e++) v[e] = (short) p[(long) (i0 + r) * si + (long) (k0 + 4 * g + e) * sk];
-    }
+    return *reinterpret_cast<const bf16x4*>(W + (16 * mt + r) * in + 16 * kb + 4 * g);
+}
+
+/* A fragment of W^T: element (i, k) = W[16 kb + k][16 mt + i] */
+__device__ __forceinline__ bf16x4 wtfrag(const uint16_t* __restrict__ W, int in, int mt, int kb, int r, int g)
+{
+    bf16x4 v;
+    for (int e = 0; e < 4; e++) v[e] = (short) W[(16 * kb + 4 * g + e) * in + 16 * mt + r];
     return v;
 }
 
-/* C (M x N) = A (M x K) B (K x N), M, N, K multiples of 16. One wave (and
- * block) per 16 x 16 output tile and K slice of kchunk; slices combine by
- * atomics. amode / bmode: OP_* of A and B. */
-__global__ void __launch_bounds__(64) k_gemm(int M, int N, int K, int kchunk, MatBF A, MatBF B, Epi E, int amode,
-                                             int bmode)
+/* D fragment (features 16 t + 4 g + e of sample r) of group j into a
+ * [feature][kChunk] LDS array */
+__device__ __forceinline__ void put_frag(uint16_t* __restrict__ a, int t, int j, bf16x4 v, int r, int g)
 {
-    __shared__ __align__(16) uint16_t ta[16 * 16], tb[16 * 16];
-    const int  lane  = threadIdx.x;
-    const long wave  = blockIdx.x;
-    const int  tm    = M / 16, tn = N / 16, nsplit = (K + kchunk - 1) / kchunk;
-    const long ntile = (long) tm * tn;
-    if (wave >= ntile * nsplit) return; /* whole block: the barriers stay uniform */
-    const int t = (int) (wave % ntile), split = (int) (wave / ntile);
-    const int mt = t % tm, nt = t / tm;
-    const int r = lane & 15, g = lane >> 4;
-    f32x4     acc = {0.f, 0.f, 0.f, 0.f};
-    const int k1  = min(K, (split + 1) * kchunk);
-    for (int k0 = split * kchunk; k0 < k1; k0 += 16) {
-        const bf16x4 a = tile_operand(amode, A.p, A.rs, A.cs, 16 * mt, k0, lane, ta);
-        const bf16x4 b = tile_operand(bmode, B.p, B.cs, B.rs, 16 * nt, k0, lane, tb);
-        acc            = mfma16(a, b, acc);
+    for (int e = 0; e < 4; e++) a[(16 * t + 4 * g + e) * kChunk + 16 * j + r] = (uint16_t) v[e];
+}
+
+/* one 16 x 16 tile of D = A B^T over the chunk's 64 samples (K): A rows
+ * 16 mt.. of a, B rows 16 nt.. of b, both [row][kChunk] in LDS */
+__device__ __forceinline__ f32x4 chunk_tile(const uint16_t* __restrict__ a, const uint16_t* __restrict__ b, int mt,
+                                            int nt, int r, int g)
+{
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < kChunk / 16; j++) {
+        const bf16x4 av = *reinterpret_cast<const bf16x4*>(a + (16 * mt + r) * kChunk + 16 * j + 4 * g);
+        const bf16x4 bv = *reinterpret_cast<const bf16x4*>(b + (16 * nt + r) * kChunk + 16 * j + 4 * g);
+        acc             = mfma16(av, bv, acc);
     }
+    return acc;
+}
+
+/* the gradient tile (rows 16 mt + 4 g + e, column 16 nt + r of a matrix
+ * with `in` columns at offset off) into the block's row of partial sums */
+__device__ __forceinline__ void add_tile(float* __restrict__ gp, bool first, int off, int in, int mt, int nt, f32x4 v,
+                                         int r, int g)
+{
     for (int e = 0; e < 4; e++) {
-        const long row = 16 * mt + 4 * g + e, col = 16 * nt + r;
-        const long o   = row * E.ors + col * E.ocs;
-        float      v   = acc[e];
-        switch (E.mode) {
-        case EPI_F32: E.of[o] = v; break;
-        case EPI_ATOMIC: atomicAdd(&E.of[o], v); break;
-        case EPI_BIAS_RELU:
-            v       = v + E.bias[row];
-            E.ob[o] = f2bf(v > 0.0f ? v : 0.0f);
-            break;
-        default: E.ob[o] = E.mask.raw(row, col) > 0 ? f2bf(v) : (uint16_t) 0; break;
-        }
+        float* q = gp + off + (16 * mt + 4 * g + e) * in + 16 * nt + r;
+        *q       = first ? v[e] : *q + v[e];
     }
+}
+
+__device__ __forceinline__ bf16x4 relu_pack(f32x4 acc, const float* __restrict__ b)
+{
+    return pack4(acc[0] + b[0] > 0.0f ? acc[0] + b[0] : 0.0f, acc[1] + b[1] > 0.0f ? acc[1] + b[1] : 0.0f,
+                 acc[2] + b[2] > 0.0f ? acc[2] + b[2] : 0.0f, acc[3] + b[3] > 0.0f ? acc[3] + b[3] : 0.0f);
+}
+
+/* dZ = (W^T dZ_next) where the layer's ReLU output is positive, bf16 */
+__device__ __forceinline__ bf16x4 mask_pack(f32x4 acc, bf16x4 h)
+{
+    return pack4(h[0] > 0 ? acc[0] : 0.0f, h[1] > 0 ? acc[1] : 0.0f, h[2] > 0 ? acc[2] : 0.0f,
+                 h[3] > 0 ? acc[3] : 0.0f);
+}
+
+/* the weights as matrix-core A fragments, forward (W) and backward (W^T),
+ * held in registers for a whole launch */
+struct TrainFrags {
+    bf16x4 w1[4][3], w2[4][4], w3[4][4], w4[4]; /* W:   rows 16 mt.., columns 16 kb.. */
+    bf16x4 t2[4][4], t3[4][4], t4[4];           /* W^T: rows 16 mt.., columns 16 kb.. */
+};
+
+__device__ __forceinline__ void load_train_frags(const uint16_t* __restrict__ W, int r, int g, TrainFrags& F)
+{
+    for (int mt = 0; mt < 4; mt++) {
+        for (int kb = 0; kb < 3; kb++) F.w1[mt][kb] = wfrag(W + kP1, 48, mt, kb, r, g);
+        for (int kb = 0; kb < 4; kb++) {
+            F.w2[mt][kb] = wfrag(W + kP2, 64, mt, kb, r, g);
+            F.w3[mt][kb] = wfrag(W + kP3, 64, mt, kb, r, g);
+            F.t2[mt][kb] = wtfrag(W + kP2, 64, mt, kb, r, g);
+            F.t3[mt][kb] = wtfrag(W + kP3, 64, mt, kb, r, g);
+        }
+        F.w4[mt] = wfrag(W + kP4, 64, 0, mt, r, g);  /* the output row block, K block mt */
+        F.t4[mt] = wtfrag(W + kP4, 64, mt, 0, r, g); /* W4^T rows 16 mt.., its 16 columns */
+    }
+}
+
+/* Forward and backward of the B columns of X (n valid; labels Y), four
+ * waves per block. Block b takes the 64-sample chunks b, b + gridDim.x, ...;
+ * per chunk wave w runs fragment group w (16 samples) through the four
+ * layers as matrix-core fragments (bias, ReLU and bf16 rounding between
+ * layers, the rounding points of the oracle), the loss and output gradient
+ * (bf16((p - y) / n)), and the backward pass through the transposed weights
+ * masked by the ReLU outputs, writing every activation and gradient to LDS;
+ * then the four waves share the chunk's 48 weight-gradient tiles (16 x 16,
+ * the 64 samples as K) and the bias row sums, added to the block's own row
+ * of gp (kGradN floats, no atomics; k_grad_rows sums the rows). The summed
+ * BCE goes to *loss. */
+__global__ void __launch_bounds__(256) k_mlp_train(const uint16_t* __restrict__ W, const float* __restrict__ P,
+                                                   const uint16_t* __restrict__ X, const float* __restrict__ Y, int B,
+                                                   int n, float* __restrict__ gp, double* __restrict__ loss)
+{
+    __shared__ TrainLds L;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63, r = lane & 15, g = lane >> 4;
+    for (int i = tid; i < kLBN; i += blockDim.x) {
+        float b = 0.0f;
+        if (i < 64) b = P[kB1 + i];
+        else if (i < 128) b = P[kB2 + i - 64];
+        else if (i < 192) b = P[kB3 + i - 128];
+        else if (i == kLB4) b = P[kB4];
+        L.bias[i] = b;
+    }
+    TrainFrags F;
+    load_train_frags(W, r, g, F);
+    float* const  row    = gp + (size_t) blockIdx.x * kGradN;
+    const int     chunks = (B + kChunk - 1) / kChunk;
+    double        lsum   = 0.0;
+    const bf16x4  zero4  = {0, 0, 0, 0};
+    for (int c = blockIdx.x; c < chunks; c += gridDim.x) {
+        const bool first = c == (int) blockIdx.x;
+        const int  base  = c * kChunk;
+        for (int i = tid; i < kMlpInPad * kChunk; i += blockDim.x) {
+            const int f = i / kChunk, s = base + i % kChunk;
+            L.x[i]      = s < B ? X[(size_t) f * B + s] : (uint16_t) 0;
+        }
+        __syncthreads();
+        const int j = wv; /* this wave's fragment group */
+        if (base + 16 * j >= B) { /* past the batch: zeros, which add nothing below */
+            for (int t = 0; t < 4; t++) {
+                put_frag(L.h1, t, j, zero4, r, g), put_frag(L.h2, t, j, zero4, r, g);
+                put_frag(L.h3, t, j, zero4, r, g), put_frag(L.d1, t, j, zero4, r, g);
+                put_frag(L.d2, t, j, zero4, r, g), put_frag(L.d3, t, j, zero4, r, g);
+            }
+            put_frag(L.d4, 0, j, zero4, r, g);
+        } else {
+            /* forward */
+            bf16x4 xin[3], h1[4], h2[4], h3[4];
+            for (int kb = 0; kb < 3; kb++)
+                for (int e = 0; e < 4; e++) xin[kb][e] = (short) L.x[(16 * kb + 4 * g + e) * kChunk + 16 * j + r];
+            for (int mt = 0; mt < 4; mt++) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                for (int kb = 0; kb < 3; kb++) acc = mfma16(F.w1[mt][kb], xin[kb], acc);
+                h1[mt] = relu_pack(acc, L.bias + kLB1 + 16 * mt + 4 * g);
+                put_frag(L.h1, mt, j, h1[mt], r, g);
+            }
+            for (int mt = 0; mt < 4; mt++) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                for (int kb = 0; kb < 4; kb++) acc = mfma16(F.w2[mt][kb], h1[kb], acc);
+                h2[mt] = relu_pack(acc, L.bias + kLB2 + 16 * mt + 4 * g);
+                put_frag(L.h2, mt, j, h2[mt], r, g);
+            }
+            for (int mt = 0; mt < 4; mt++) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                for (int kb = 0; kb < 4; kb++) acc = mfma16(F.w3[mt][kb], h2[kb], acc);
+                h3[mt] = relu_pack(acc, L.bias + kLB3 + 16 * mt + 4 * g);
+                put_frag(L.h3, mt, j, h3[mt], r, g);
+            }
+            f32x4 lg = {0.f, 0.f, 0.f, 0.f};
+            for (int kb = 0; kb < 4; kb++) lg = mfma16(F.w4[kb], h3[kb], lg);
+            /* loss and output gradient: row 0 of the output tile, lanes 0..15 */
+            const int s  = base + 16 * j + r;
+            float     dz = 0.0f;
+            if (g == 0 && s < n) {
+                const float p  = 1.0f / (1.0f + expf(-(lg[0] + L.bias[kLB4])));
+                const float y  = Y[s];
+                const float pc = fminf(fmaxf(p, 1e-7f), 1.0f - 1e-7f);
+                lsum += y > 0.5f ? -log((double) pc) : -log(1.0 - (double) pc);
+                dz = (p - y) / (float) n;
+            }
+            bf16x4 d4 = zero4;
+            d4[0]     = g == 0 ? (short) f2bf(dz) : (short) 0; /* row 0 of dZ4; rows 1..15 zero */
+            put_frag(L.d4, 0, j, d4, r, g);
+            /* backward */
+            bf16x4 d3[4], d2[4];
+            for (int mt = 0; mt < 4; mt++) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                acc       = mfma16(F.t4[mt], d4, acc);
+                d3[mt]    = mask_pack(acc, h3[mt]);
+                put_frag(L.d3, mt, j, d3[mt], r, g);
+            }
+            for (int mt = 0; mt < 4; mt++) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                for (int kb = 0; kb < 4; kb++) acc = mfma16(F.t3[mt][kb], d3[kb], acc);
+                d2[mt] = mask_pack(acc, h2[mt]);
+                put_frag(L.d2, mt, j, d2[mt], r, g);
+            }
+            for (int mt = 0; mt < 4; mt++) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                for (int kb = 0; kb < 4; kb++) acc = mfma16(F.t2[mt][kb], d2[kb], acc);
+                put_frag(L.d1, mt, j, mask_pack(acc, h1[mt]), r, g);
+            }
+        }
+        __syncthreads();
+        /* weight gradients of the chunk, dW_l = dZ_l H_{l-1}^T: 48 tiles, wave w takes w, w + 4, ... */
+        for (int t = wv; t < 48; t += 4) {
+            if (t < 12) {
+                const int mt = t / 3, nt = t % 3;
+                add_tile(row, first, kP1, 48, mt, nt, chunk_tile(L.d1, L.x, mt, nt, r, g), r, g);
+            } else if (t < 28) {
+                const int mt = (t - 12) / 4, nt = (t - 12) % 4;
+                add_tile(row, first, kP2, 64, mt, nt, chunk_tile(L.d2, L.h1, mt, nt, r, g), r, g);
+            } else if (t < 44) {
+                const int mt = (t - 28) / 4, nt = (t - 28) % 4;
+                add_tile(row, first, kP3, 64, mt, nt, chunk_tile(L.d3, L.h2, mt, nt, r, g), r, g);
+            } else {
+                const int nt = t - 44;
+                add_tile(row, first, kP4, 64, 0, nt, chunk_tile(L.d4, L.h3, 0, nt, r, g), r, g);
+            }
+        }
+        /* bias gradients: row sums, wave w takes layer w + 1 (lane = feature) */
+        if (wv < 3 || lane == 0) {
+            const uint16_t* d   = wv == 0 ? L.d1 : wv == 1 ? L.d2 : wv == 2 ? L.d3 : L.d4;
+            const int       off = wv == 0 ? kGB1 : wv == 1 ? kGB2 : wv == 2 ? kGB3 : kGB4;
+            float           acc = 0.0f;
+            for (int k = 0; k < kChunk; k++) acc += bf2f(d[lane * kChunk + k]);
+            row[off + lane] = first ? acc : row[off + lane] + acc;
+        }
+        __syncthreads(); /* the next chunk overwrites the LDS arrays */
+    }
+    for (int o = 32; o > 0; o >>= 1) lsum += __shfl_xor(lsum, o, 64);
+    if (lane == 0 && lsum != 0.0) atomicAdd(loss, lsum);
+}
+
+/* G[i] = sum over the rows of the partial gradients */
+__global__ void k_grad_rows(const float* __restrict__ gp, int rows, float* __restrict__ G)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kGradN) return;
+    float acc = 0.0f;
+    for (int b = 0; b < rows; b++) acc += gp[(size_t) b * kGradN + i];
+    G[i] = acc;
 }
 
 /* the 48 (padded) features of a query, bf16, feature-major X[48][B] */
@@ -239,44 +393,6 @@ __global__ void k_copy_features(const float* __restrict__ x, int B, int n, uint1
     for (int f = 0; f < kMlpInPad; f++)
         X[(size_t) f * B + s] = f2bf((s < n && f < kMlpIn) ? x[(size_t) s * kMlpIn + f] : 0.0f);
     if (Y) Y[s] = (s < n && yin) ? yin[s] : 0.0f;
-}
-
-/* output layer: logit = L16[0][s] + b4, p = sigmoid; summed BCE into *loss;
- * dZ4 row 0 = bf16((p - y) / n), rows 1..15 zero */
-__global__ void k_mlp_loss(const float* __restrict__ L16, const float* __restrict__ P, const float* __restrict__ Y,
-                           int B, int n, uint16_t* __restrict__ dZ4, float* __restrict__ pout, double* __restrict__ loss)
-{
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= B) return;
-    for (int r = 1; r < 16; r++) dZ4[(size_t) r * B + s] = 0;
-    if (s >= n) {
-        dZ4[s] = 0;
-        return;
-    }
-    float logit = L16[s] + P[kB4];
-    float p     = 1.0f / (1.0f + expf(-logit));
-    if (pout) pout[s] = p;
-    float  y  = Y[s];
-    float  pc = fminf(fmaxf(p, 1e-7f), 1.0f - 1e-7f);
-    double l  = y > 0.5f ? -log((double) pc) : -log(1.0 - (double) pc);
-    atomicAdd(loss, l);
-    dZ4[s] = f2bf((p - y) / (float) n);
-}
-
-/* g[r] += sum over s of Z[r][s] (bf16 rows of length B), one block per row */
-__global__ void k_rowsum(const uint16_t* __restrict__ Z, int B, float* __restrict__ g)
-{
-    __shared__ float red[256];
-    const int        r   = blockIdx.x;
-    float            acc = 0.0f;
-    for (int s = threadIdx.x; s < B; s += blockDim.x) acc += bf2f(Z[(size_t) r * B + s]);
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
-        if ((int) threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) g[r] += red[0];
 }
 
 /* flat parameter index -> gradient-layout index */
@@ -665,8 +781,8 @@ void qbt_ctx::mlp_free()
     if (!mlp) return;
     MlpState* m = mlp;
     for (DevBuf* b : {&m->params, &m->m, &m->v, &m->grads, &m->packed, &m->ctr, &m->samples, &m->perm, &m->keys,
-                      &m->vals, &m->sort_tmp, &m->X, &m->H1, &m->H2, &m->H3, &m->L16, &m->dZ4, &m->dZ3, &m->dZ2,
-                      &m->dZ1, &m->lossbuf, &m->io_x, &m->io_p, &m->io_y, &m->io_a, &m->io_c, &m->flat})
+                      &m->vals, &m->sort_tmp, &m->X, &m->gpart, &m->lossbuf, &m->io_x, &m->io_p, &m->io_y,
+                      &m->io_a, &m->io_c, &m->flat})
         dev_free(*b);
     delete mlp;
     mlp = nullptr;
@@ -817,84 +933,38 @@ extern "C" int qbt_surrogate_read_samples(qbt_ctx* c, size_t start, size_t n, fl
     return QBT_OK;
 }
 
-/* activation buffers for batch width B (multiple of 16) */
+constexpr int kTrainBlocks = 256; /* blocks of k_mlp_train, rows of partial gradients, at most */
+
+/* batch buffers for batch width B (multiple of 16) */
 static int mlp_batch_bufs(qbt_ctx* c, int B)
 {
-    MlpState*    m = c->mlp;
-    const size_t b = (size_t) B;
+    MlpState*    m      = c->mlp;
+    const size_t b      = (size_t) B;
+    const size_t chunks = (b + kChunk - 1) / kChunk;
+    const size_t rows   = chunks < (size_t) kTrainBlocks ? chunks : (size_t) kTrainBlocks;
     int          rc;
-    if ((rc = c->dev_alloc(m->X, kMlpInPad * b * 2)) || (rc = c->dev_alloc(m->H1, 64 * b * 2)) ||
-        (rc = c->dev_alloc(m->H2, 64 * b * 2)) || (rc = c->dev_alloc(m->H3, 64 * b * 2)) ||
-        (rc = c->dev_alloc(m->L16, 16 * b * 4)) || (rc = c->dev_alloc(m->dZ4, 16 * b * 2)) ||
-        (rc = c->dev_alloc(m->dZ3, 64 * b * 2)) || (rc = c->dev_alloc(m->dZ2, 64 * b * 2)) ||
-        (rc = c->dev_alloc(m->dZ1, 64 * b * 2)) || (rc = c->dev_alloc(m->io_y, b * 4)) ||
-        (rc = c->dev_alloc(m->io_p, b * 4)))
+    if ((rc = c->dev_alloc(m->X, kMlpInPad * b * 2)) || (rc = c->dev_alloc(m->gpart, rows * kGradN * 4)) ||
+        (rc = c->dev_alloc(m->io_y, b * 4)) || (rc = c->dev_alloc(m->io_p, b * 4)))
         return rc;
     return QBT_OK;
 }
 
-/* OP_* for an operand whose element (i, k) is at p[i * si + k * sk]: the
- * 8-byte loads need 8-byte aligned addresses for every i, k multiple of 4 */
-static int operand_mode(const uint16_t* p, long si, long sk)
+/* Forward and backward of the B-column batch in X (n valid columns, labels
+ * in io_y): the summed loss into lossbuf, the gradient (layout kGradN) into
+ * grads. Three launches: loss reset, k_mlp_train, k_grad_rows. */
+static int mlp_fwd_bwd(qbt_ctx* c, int B, int n, double* loss_out)
 {
-    const bool al = ((uintptr_t) p & 7u) == 0;
-    if (al && sk == 1 && si % 4 == 0) return OP_K;
-    if (al && si == 1 && sk % 4 == 0) return OP_T;
-    return OP_S;
-}
-
-static void gemm(hipStream_t st, int M, int N, int K, int kchunk, MatBF A, MatBF B, Epi E)
-{
-    const long waves = (long) (M / 16) * (N / 16) * ((K + kchunk - 1) / kchunk);
-    k_gemm<<<(unsigned) waves, 64, 0, st>>>(M, N, K, kchunk, A, B, E, operand_mode(A.p, A.rs, A.cs),
-                                            operand_mode(B.p, B.cs, B.rs));
-}
-
-/* Forward of the B-column batch in X (n valid columns, labels in io_y):
- * probabilities into io_p, summed loss into lossbuf; with backward, the
- * gradient (layout kGradN) into grads. */
-static int mlp_fwd_bwd(qbt_ctx* c, int B, int n, bool backward, double* loss_out)
-{
-    MlpState*       m  = c->mlp;
-    hipStream_t     st = c->stream;
-    const uint16_t* W  = (const uint16_t*) m->packed.p;
-    const float*    P  = (const float*) m->params.p;
-    uint16_t *      X = (uint16_t*) m->X.p, *H1 = (uint16_t*) m->H1.p, *H2 = (uint16_t*) m->H2.p,
-             *H3 = (uint16_t*) m->H3.p;
-    gemm(st, 64, B, 48, 48, {W + kP1, 48, 1}, {X, B, 1}, {EPI_BIAS_RELU, nullptr, H1, B, 1, P + kB1, {}});
-    gemm(st, 64, B, 64, 64, {W + kP2, 64, 1}, {H1, B, 1}, {EPI_BIAS_RELU, nullptr, H2, B, 1, P + kB2, {}});
-    gemm(st, 64, B, 64, 64, {W + kP3, 64, 1}, {H2, B, 1}, {EPI_BIAS_RELU, nullptr, H3, B, 1, P + kB3, {}});
-    gemm(st, 16, B, 64, 64, {W + kP4, 64, 1}, {H3, B, 1}, {EPI_F32, (float*) m->L16.p, nullptr, B, 1, nullptr, {}});
-    HIPCHK(hipGetLastError());
-    double* lossd = (double*) m->lossbuf.p;
+    MlpState*   m      = c->mlp;
+    hipStream_t st     = c->stream;
+    const int   chunks = (B + kChunk - 1) / kChunk;
+    const int   rows   = chunks < kTrainBlocks ? chunks : kTrainBlocks;
+    double*     lossd  = (double*) m->lossbuf.p;
     k_fill_f32<<<1, 64, 0, st>>>((float*) m->lossbuf.p, 2, 0.0f);
-    k_mlp_loss<<<nblk(B), 256, 0, st>>>((const float*) m->L16.p, P, (const float*) m->io_y.p, B, n,
-                                        (uint16_t*) m->dZ4.p, (float*) m->io_p.p, lossd);
+    k_mlp_train<<<rows, 256, 0, st>>>((const uint16_t*) m->packed.p, (const float*) m->params.p,
+                                      (const uint16_t*) m->X.p, (const float*) m->io_y.p, B, n, (float*) m->gpart.p,
+                                      lossd);
+    k_grad_rows<<<nblk(kGradN), 256, 0, st>>>((const float*) m->gpart.p, rows, (float*) m->grads.p);
     HIPCHK(hipGetLastError());
-    if (backward) {
-        float*    G   = (float*) m->grads.p;
-        uint16_t *dZ4 = (uint16_t*) m->dZ4.p, *dZ3 = (uint16_t*) m->dZ3.p, *dZ2 = (uint16_t*) m->dZ2.p,
-                 *dZ1 = (uint16_t*) m->dZ1.p;
-        /* K split of the weight-gradient GEMMs (K = batch) */
-        const int kc = B > 512 ? 512 : B;
-        k_fill_f32<<<nblk(kGradN), 256, 0, st>>>(G, kGradN, 0.0f);
-        /* layer 4: dW4 = dZ4 H3^T, db4 = rowsum dZ4, dZ3 = (W4^T dZ4) where H3 > 0 */
-        gemm(st, 16, 64, B, kc, {dZ4, B, 1}, {H3, 1, B}, {EPI_ATOMIC, G + kP4, nullptr, 64, 1, nullptr, {}});
-        k_rowsum<<<1, 256, 0, st>>>(dZ4, B, G + kGB4);
-        gemm(st, 64, B, 16, 16, {W + kP4, 1, 64}, {dZ4, B, 1}, {EPI_MASK, nullptr, dZ3, B, 1, nullptr, {H3, B, 1}});
-        /* layer 3 */
-        gemm(st, 64, 64, B, kc, {dZ3, B, 1}, {H2, 1, B}, {EPI_ATOMIC, G + kP3, nullptr, 64, 1, nullptr, {}});
-        k_rowsum<<<64, 256, 0, st>>>(dZ3, B, G + kGB3);
-        gemm(st, 64, B, 64, 64, {W + kP3, 1, 64}, {dZ3, B, 1}, {EPI_MASK, nullptr, dZ2, B, 1, nullptr, {H2, B, 1}});
-        /* layer 2 */
-        gemm(st, 64, 64, B, kc, {dZ2, B, 1}, {H1, 1, B}, {EPI_ATOMIC, G + kP2, nullptr, 64, 1, nullptr, {}});
-        k_rowsum<<<64, 256, 0, st>>>(dZ2, B, G + kGB2);
-        gemm(st, 64, B, 64, 64, {W + kP2, 1, 64}, {dZ2, B, 1}, {EPI_MASK, nullptr, dZ1, B, 1, nullptr, {H1, B, 1}});
-        /* layer 1 */
-        gemm(st, 64, 48, B, kc, {dZ1, B, 1}, {X, 1, B}, {EPI_ATOMIC, G + kP1, nullptr, 48, 1, nullptr, {}});
-        k_rowsum<<<64, 256, 0, st>>>(dZ1, B, G + kGB1);
-        HIPCHK(hipGetLastError());
-    }
     if (loss_out) {
         double l = 0.0;
         HIPCHK(hipMemcpyAsync(&l, lossd, 8, hipMemcpyDeviceToHost, st));
@@ -991,7 +1061,7 @@ extern "C" int qbt_surrogate_grad(qbt_ctx* c, const float* x, const float* y, si
     int B, rc;
     if ((rc = mlp_load_host(c, x, y, n, &B))) return rc;
     double l = 0.0;
-    if ((rc = mlp_fwd_bwd(c, B, (int) n, true, &l))) return rc;
+    if ((rc = mlp_fwd_bwd(c, B, (int) n, &l))) return rc;
     if (grads) {
         k_grads_to_flat<<<nblk(kMlpParams), 256, 0, c->stream>>>((const float*) c->mlp->grads.p,
                                                                   (float*) c->mlp->flat.p);
@@ -1073,7 +1143,7 @@ extern "C" int qbt_surrogate_train(qbt_ctx* c, const float basecube[4], int step
                                                              (uint16_t*) m->X.p, (float*) m->io_y.p);
         HIPCHK(hipGetLastError());
         m->pos += (size_t) batch;
-        if ((rc = mlp_fwd_bwd(c, batch, batch, true, it == steps - 1 ? &last : nullptr))) return rc;
+        if ((rc = mlp_fwd_bwd(c, batch, batch, it == steps - 1 ? &last : nullptr))) return rc;
         if ((rc = mlp_adam_step(c, lr))) return rc;
     }
     HIPCHK(hipStreamSynchronize(c->stream));
