@@ -680,15 +680,31 @@ __global__ void k_fill_u32n(unsigned* p, size_t n, unsigned v)
 
 inline unsigned nblk(size_t n, unsigned bs = 256) { return (unsigned) ((n + bs - 1) / bs); }
 
+/* Blocks of a persistent traversal launch over `items` rays. All the blocks
+ * that fit (6 waves per SIMD for the traversal kernels: VGPRs and LDS
+ * stacks) when there are at least kFullGridRays rays per lane of that grid;
+ * fewer, 4 waves per SIMD, for shorter launches, where the end of a launch
+ * (each wave stepping its last few long rays with most lanes idle once the
+ * queue runs dry) is a large share: there each wave serves half as many rays
+ * again before the queue runs dry. Measured: the 1080p camera launch of the
+ * 90M-point scene (5 rays per lane) -2.6% with 4, the one-light shadow launch
+ * of the 10M-point scene (1.3 rays per lane) -8%; 3840x2160 frames (20-30
+ * rays per lane) and 64-view batches 5-8% slower with 4. */
+constexpr int kSmallGridWavesPerSimd = 4;
+constexpr int kFullGridRays          = 16;
+
 template <class K>
-int persistent_grid(qbt_ctx* c, K kernel, int bs, size_t lds)
+int persistent_grid(qbt_ctx* c, K kernel, int bs, size_t lds, size_t items)
 {
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, bs, lds) != hipSuccess || per <= 0) {
         (void) hipGetLastError();
         per = 1;
     }
-    return per * c->num_cus;
+    const size_t lanes = (size_t) per * (size_t) c->num_cus * (size_t) bs;
+    if (items >= (size_t) kFullGridRays * lanes) return per * c->num_cus;
+    const int cap = std::max(1, 4 * kSmallGridWavesPerSimd / std::max(1, bs / 64)); /* blocks per CU */
+    return std::min(per, cap) * c->num_cus;
 }
 
 /* the launch configuration a captured graph depends on (kernel arguments) */
@@ -760,8 +776,9 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     /* resident blocks, but no more than one work item per lane: small
      * batches then start fewer waves (less launch and refill-counter
      * contention); shadow items are at most one per batch pixel and light */
-    const int      gp = std::min(persistent_grid(c, kp, bs, lds), (int) nblk((size_t) nitems + nspr, (unsigned) bs));
-    const int      gs = std::min(persistent_grid(c, ks, bs, lds), (int) nblk((size_t) npix * (nl > 0 ? nl : 1), (unsigned) bs));
+    const size_t   np = (size_t) nitems + nspr, ns = (size_t) npix * (nl > 0 ? nl : 1);
+    const int      gp = std::min(persistent_grid(c, kp, bs, lds, np), (int) nblk(np, (unsigned) bs));
+    const int      gs = std::min(persistent_grid(c, ks, bs, lds, ns), (int) nblk(ns, (unsigned) bs));
     if (GSTK && (rc = c->dev_alloc(c->wb_gstack, qbt_stack_bytes(s.maxlevel, DUAL, (gp > gs ? gp : gs) * bs))))
         return rc;
 
