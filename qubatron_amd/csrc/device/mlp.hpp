@@ -57,7 +57,9 @@ __device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d)
 
 /* Feature f (0..47) of one query: f = 6k + 2a + {0 sin, 1 cos} for band k
  * (frequency 2^k pi) and axis a of the hit position scaled to [0,1]; 36..38
- * normal; 39..41 light scaled to [0,1]; 42..47 zero padding. */
+ * normal; 39..41 light scaled to [0,1]; 42..47 zero padding. Training and
+ * inference compute them with the same code, so the network sees the same
+ * features in both. */
 /* element a (0..2) of a 3-vector, selected by value (an indexed access with
  * a lane-dependent index would put the vector in scratch memory) */
 __device__ __forceinline__ float pick3(const float v[3], int a)
@@ -66,12 +68,23 @@ __device__ __forceinline__ float pick3(const float v[3], int a)
     return a == 0 ? x : (a == 1 ? y : z);
 }
 
+/* sin and cos of 2^k pi x (band k of coordinate x), by the hardware's
+ * transcendental unit: v_sin_f32 / v_cos_f32 take revolutions, and
+ * 2^k pi x = 2 pi (2^(k-1) x) with 2^(k-1) x exact */
+__device__ __forceinline__ void band_sincos(int k, float x, float* s, float* c)
+{
+    const float rev = ldexpf(x, k - 1);
+    *s              = __builtin_amdgcn_sinf(rev);
+    *c              = __builtin_amdgcn_cosf(rev);
+}
+
 __device__ __forceinline__ float mlp_feature(int f, const float u[3], const float n[3], const float lu[3])
 {
     if (f < 36) {
         int   k = f / 6, a = (f % 6) >> 1;
-        float arg = ((float) (1 << k) * 3.14159265358979f) * pick3(u, a);
-        return (f & 1) ? cosf(arg) : sinf(arg);
+        float s, c;
+        band_sincos(k, pick3(u, a), &s, &c);
+        return (f & 1) ? c : s;
     }
     if (f < 39) return pick3(n, f - 36);
     if (f < 42) return pick3(lu, f - 39);

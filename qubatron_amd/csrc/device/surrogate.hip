@@ -483,7 +483,7 @@ __device__ __forceinline__ void load_frags(const uint16_t* __restrict__ W, int l
 }
 
 /* B fragments of the input of 16 queries: lane l holds features 16 kb +
- * 4 (l / 16) + e of query l % 16; a sin/cos pair shares one sincosf */
+ * 4 (l / 16) + e of query l % 16 */
 __device__ __forceinline__ void query_frag(const float u[3], const float nrm[3], const float lu[3], int g,
                                            bf16x4 xin[3])
 {
@@ -492,9 +492,8 @@ __device__ __forceinline__ void query_frag(const float u[3], const float nrm[3],
         for (int p = 0; p < 2; p++) {
             const int fi = 16 * kb + 4 * g + 2 * p; /* even */
             if (fi < 36) {
-                const int   k = fi / 6, a = (fi % 6) >> 1;
-                const float arg = ((float) (1 << k) * 3.14159265358979f) * pick3(u, a);
-                sincosf(arg, &f[2 * p], &f[2 * p + 1]);
+                const int k = fi / 6, a = (fi % 6) >> 1;
+                band_sincos(k, pick3(u, a), &f[2 * p], &f[2 * p + 1]);
             } else {
                 f[2 * p]     = mlp_feature(fi, u, nrm, lu);
                 f[2 * p + 1] = mlp_feature(fi + 1, u, nrm, lu);
