@@ -2,7 +2,7 @@
 traversal (``src/qubatron/shaders/octree_fsh.c`` / ``src/qubatron/octree.c``).
 
 TEST INFRASTRUCTURE: only tests/, ``__graft_entry__.smoke()`` and bench.py's
-cpu_baseline leg use it, as the checker. The product never imports it.
+cpu_baseline leg use it, to compare against. The product never imports it.
 """
 import ctypes as C
 import os

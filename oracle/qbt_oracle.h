@@ -1,6 +1,6 @@
 /*
  * qbt_oracle.h - CPU restatement of Qubatron's octree ray traversal, shadow
- * test and shading, used ONLY as the parity checker of the HIP path.
+ * test and shading, used ONLY as the parity reference of the HIP path.
  *
  * TEST INFRASTRUCTURE. Only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load liboracle.so; the product library never links it.

@@ -1,5 +1,5 @@
-"""The reference's renderer connector replaced by the C shim of
-INTEGRATION.md (tests/cshim/octree_glc_qbt.c), driven by a compiled C caller
+"""The reference's renderer connector replaced by the C shim a maintainer
+adds to the reference's build (tests/cshim/octree_glc_qbt.c), driven by a compiled C caller
 (tests/cshim/glc_demo.c) that replays the reference's static-scene calls:
 load .pnt/.col/.nrm flat files (model.c:53-111), insert the points in order
 (modelutil.c:203-217), upload colours, normals and the octree
