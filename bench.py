@@ -59,7 +59,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 COPY_BYTES = 1 << 30
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traversal_config3_1080p.json")
-KIND_PRIMARY, KIND_SHADOW, KIND_SURROGATE = 0, 1, 2
+KIND_PRIMARY, KIND_SHADOW, KIND_SURROGATE, KIND_FRAME = 0, 1, 2, 3
 
 
 def host_cpu():
@@ -125,6 +125,8 @@ def parse():
     ap.add_argument("--reorder", type=int, default=None, help="1 shadow rays in direction-octant order")
     ap.add_argument("--graph", type=int, default=None, help="1 replay captured hipGraphs")
     ap.add_argument("--strip", type=int, default=None, help="tile order: 0 row-major, S strips of S tiles")
+    ap.add_argument("--merge", type=int, default=None,
+                    help="1 camera and shadow rays of exact renders in one launch (k_frame), 0 two launches")
     return ap.parse_args()
 
 
@@ -263,6 +265,10 @@ def roofline(args, prof, K, W, H, nl, exps, exps_p, leafhits, copy_gbps, with_tr
         kern["k_shadow"] = {"alg_bytes_per_launch": 32.0 * e_s * K / kl[KIND_SHADOW],
                             "avg_ms": km[KIND_SHADOW] / kl[KIND_SHADOW],
                             "share": km[KIND_SHADOW], "launches": kl[KIND_SHADOW]}
+    if kl[KIND_FRAME]:  # camera and shadow rays in one launch: both kernels' bytes
+        kern["k_frame"] = {"alg_bytes_per_launch": (32.0 * (e_p + e_s) + 4.0 * h) * K / kl[KIND_FRAME],
+                           "avg_ms": km[KIND_FRAME] / kl[KIND_FRAME],
+                           "share": km[KIND_FRAME], "launches": kl[KIND_FRAME]}
     traffic = load_traffic(args.traffic) if with_traffic else {}
     for name, e in kern.items():
         e["achieved_gbs"] = e["alg_bytes_per_launch"] / (e["avg_ms"] * 1e-3) / 1e9
@@ -273,7 +279,7 @@ def roofline(args, prof, K, W, H, nl, exps, exps_p, leafhits, copy_gbps, with_tr
     dom = max(kern, key=lambda k: kern[k]["share"])
     d = kern[dom]
     frame_bytes = 32.0 * (e_p + e_s) + 28.0 * h + W * H * (12 + nl)
-    frame_ms = (km[KIND_PRIMARY] + km[KIND_SHADOW]) / K
+    frame_ms = (km[KIND_PRIMARY] + km[KIND_SHADOW] + km[KIND_FRAME]) / K
     roof = {"bound": "hbm", "achieved": round(d["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(d["frac"], 4), "traffic": d["traffic"], "kernel": dom,
             "alg_bytes_per_launch": round(d["alg_bytes_per_launch"]), "kernel_avg_ms": round(d["avg_ms"], 4),
@@ -564,7 +570,8 @@ def dynamic_leg(args, q, ctx, sc, mask, copy_gbps):
             "rays_per_frame": round(float(np.mean(rays))),
             "expansions_per_frame": round(float(np.mean(exps))),
             "kernel_ms_per_frame": {"primary": round(km[KIND_PRIMARY] / K, 4),
-                                    "shadow": round(km[KIND_SHADOW] / K, 4)}}
+                                    "shadow": round(km[KIND_SHADOW] / K, 4),
+                                    "camera_and_shadow": round(km[KIND_FRAME] / K, 4)}}
 
 
 def config5_leg(args, q, ctx, nthreads, mask):
@@ -607,7 +614,8 @@ def config5_leg(args, q, ctx, nthreads, mask):
     res = surrogate_leg(args, q, ctx, sc, views, Wm, K, mask, wall, total_rays, qpf)
     res["exact"] = {"value": round(total_rays / wall / 1e6, 3), "ms_per_step": round(wall * 1e3 / K, 4),
                     "kernel_ms_per_frame": {"primary": round(km[KIND_PRIMARY] / K, 4),
-                                            "shadow": round(km[KIND_SHADOW] / K, 4)}}
+                                            "shadow": round(km[KIND_SHADOW] / K, 4),
+                                            "camera_and_shadow": round(km[KIND_FRAME] / K, 4)}}
     res["workload"] = (f"config 5: the config-2 scene ({sc.npoints} points, depth {sc.depth}, {len(lights)} light), "
                        f"{W}x{H}, {K} frames of a seeded camera path, exact and with the network")
     res["points"], res["nodes"], res["steps"] = sc.npoints, nnodes, K
@@ -728,7 +736,7 @@ def main():
 
     ctx = q.Context(0)
     ctx.set_tuning(path=args.path, block_threads=args.block, refill=args.refill, layout=args.layout,
-                   stack=args.stack, reorder=args.reorder, graph=args.graph, strip=args.strip)
+                   stack=args.stack, reorder=args.reorder, graph=args.graph, strip=args.strip, merge=args.merge)
     tuning = ctx.tuning()
     copy_ms, copy_gbps = ctx.copy_bandwidth(COPY_BYTES, 5)
     log(f"copy kernel: {copy_gbps:.0f} GB/s (read+write) on {COPY_BYTES >> 20} MiB")
@@ -777,7 +785,7 @@ def main():
     value = total_rays / wall / 1e6
     km, kl = prof["kind_ms"], prof["kind_launches"]
     log(f"timed {K} frames: {wall * 1e3:.2f} ms wall; primary {km[0] / max(kl[0], 1):.3f} ms x {kl[0]}, "
-        f"shadow {km[1] / max(kl[1], 1):.3f} ms x {kl[1]}")
+        f"shadow {km[1] / max(kl[1], 1):.3f} ms x {kl[1]}, camera and shadow {km[3] / max(kl[3], 1):.3f} ms x {kl[3]}")
     roof = roofline(args, prof, K, W, H, nl, exps, exps_p, leafhits, copy_gbps)
     log(f"roofline: {roof['kernel']} {roof['achieved']} GB/s = {roof['frac']:.4f} of peak; frame "
         f"{roof['frame']['achieved']} GB/s")

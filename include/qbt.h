@@ -190,6 +190,7 @@ typedef struct qbt_stats {
     uint32_t dual;           /* the dynamic tree took part */
     uint32_t compact;        /* the compact node layout was traversed */
     uint64_t expansions_primary; /* of which by primary rays (QBT_RENDER_COUNT only) */
+    uint64_t shadow_deferred;    /* of shadow_rays, traced by their pixel's camera lane (tuning merge) */
 } qbt_stats;
 
 int qbt_get_stats(qbt_ctx* ctx, qbt_stats* out);
@@ -212,6 +213,7 @@ int qbt_synchronize(qbt_ctx* ctx);
 #define QBT_PROF_PRIMARY 0   /* fused render kernel, primary kernel, ray batches */
 #define QBT_PROF_SHADOW 1    /* shadow kernel of the wavefront path (all lights) */
 #define QBT_PROF_SURROGATE 2 /* surrogate decision kernel */
+#define QBT_PROF_FRAME 3     /* camera and shadow rays in one launch (tuning merge) */
 typedef struct qbt_profile {
     uint32_t launches;
     uint32_t reserved;
@@ -240,7 +242,10 @@ int qbt_profile_end(qbt_ctx* ctx, qbt_profile* out);
  * capture the batch's launches into a hipGraph and replay it while the
  * launch configuration is unchanged. strip: order of the 8x8 pixel tiles,
  * 0 = row-major, S (1..1024) = vertical strips S tiles wide, each row-major.
- * Results never depend on these. */
+ * merge: 1 = an exact render (no surrogate, no sample collection, reorder 0)
+ * traces its camera and shadow rays in one persistent launch, the shadow
+ * rays of finished pixels taken by waves whose camera work has run dry; 0 =
+ * a camera launch, then a shadow launch. Results never depend on these. */
 typedef struct qbt_tuning {
     int path;
     int block_threads;
@@ -250,6 +255,7 @@ typedef struct qbt_tuning {
     int reorder;
     int graph;
     int strip;
+    int merge; /* 1: camera and shadow rays of an exact render in one launch (k_frame) */
 } qbt_tuning;
 
 int qbt_get_tuning(qbt_ctx* ctx, qbt_tuning* out);
@@ -258,7 +264,8 @@ int qbt_set_tuning(qbt_ctx* ctx, const qbt_tuning* in);
 /* Autotuner: times qbt_render of the given views and lights (QBT_RENDER_*
  * counting/collection flags are dropped) under candidate tunings of the
  * wavefront path - coordinate descent over block_threads {64, 128, 256},
- * refill {defaults, 8, 16, 24, 32, 48}, stack {LDS, global} and reorder {0, 1}, two
+ * refill {defaults, 8, 16, 24, 32, 48}, stack {LDS, global}, reorder {0, 1}
+ * and merge {0, 1} (merge takes effect with reorder 0), two
  * sweeps, a candidate kept only if it is at least 1% faster - and leaves the
  * fastest set on the context. results (up to max_results) receive every
  * timed candidate, *nresults how many were timed. reps: timed renders per

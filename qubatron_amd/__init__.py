@@ -357,7 +357,7 @@ class Context:
         out["kind_ms"] = list(p.kind_ms)
         return out
 
-    TUNING_FIELDS = ("path", "block_threads", "refill", "layout", "stack", "reorder", "graph", "strip")
+    TUNING_FIELDS = ("path", "block_threads", "refill", "layout", "stack", "reorder", "graph", "strip", "merge")
 
     def tuning(self):
         t = _ffi.Tuning()

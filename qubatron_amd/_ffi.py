@@ -78,7 +78,8 @@ class Profile(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("path", C.c_int), ("block_threads", C.c_int), ("refill", C.c_int), ("layout", C.c_int),
-                ("stack", C.c_int), ("reorder", C.c_int), ("graph", C.c_int), ("strip", C.c_int)]
+                ("stack", C.c_int), ("reorder", C.c_int), ("graph", C.c_int), ("strip", C.c_int),
+                ("merge", C.c_int)]
 
 
 class TuneResult(C.Structure):
@@ -89,7 +90,8 @@ class Stats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("trace_ms", C.c_double), ("primary_rays", C.c_uint64),
                 ("shadow_rays", C.c_uint64), ("hits", C.c_uint64), ("expansions", C.c_uint64),
                 ("predicted", C.c_uint64), ("launches", C.c_uint32), ("grid_exact", C.c_uint32),
-                ("dual", C.c_uint32), ("compact", C.c_uint32), ("expansions_primary", C.c_uint64)]
+                ("dual", C.c_uint32), ("compact", C.c_uint32), ("expansions_primary", C.c_uint64),
+                ("shadow_deferred", C.c_uint64)]
 
 
 BUFFER_STATIC_COLOR, BUFFER_STATIC_NORMAL, BUFFER_STATIC_OCTREE = 0, 1, 2

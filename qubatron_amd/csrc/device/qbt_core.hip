@@ -572,7 +572,7 @@ extern "C" void qbt_destroy(qbt_ctx* c)
     c->graph_free();
     for (DevBuf* b : {&c->wb_isp, &c->wb_ms, &c->wb_md, &c->wb_leaf, &c->wb_flags, &c->wb_disc, &c->wb_lstat,
                       &c->wb_exp, &c->wb_cand, &c->wb_items, &c->wb_q, &c->wb_lists, &c->wb_fb, &c->wb_gstack,
-                      &c->wb_bcnt, &c->wb_lexp,
+                      &c->wb_bcnt, &c->wb_lexp, &c->wb_slots,
                       &c->bt_tab, &c->valflag, &c->valrg})
         c->dev_free(*b);
     for (int k = 0; k < qbt_ctx::kTabSlots; k++) {
@@ -1342,6 +1342,7 @@ extern "C" int qbt_get_stats(qbt_ctx* c, qbt_stats* out)
         c->last.hits        = cnt[CNT_HITS];
         c->last.expansions  = cnt[CNT_EXP];
         c->last.expansions_primary = cnt[CNT_EXP_PRIM];
+        c->last.shadow_deferred    = cnt[CNT_DEFER];
         c->last.predicted   = cnt[CNT_PRED];
         c->aborted          = cnt[CNT_ABORT];
         c->pending_stats    = false;
@@ -1414,6 +1415,7 @@ extern "C" int qbt_get_tuning(qbt_ctx* c, qbt_tuning* out)
     out->reorder       = c->tune_reorder;
     out->graph         = c->tune_graph;
     out->strip         = c->tune_strip;
+    out->merge         = c->tune_merge;
     return QBT_OK;
 }
 
@@ -1426,9 +1428,10 @@ extern "C" int qbt_set_tuning(qbt_ctx* c, const qbt_tuning* in)
     if (in->refill < 0 || in->refill > 64) return QBT_EINVAL;
     if (in->layout < 0 || in->layout > 1) return QBT_EINVAL;
     if (in->stack < 0 || in->stack > 1 || in->reorder < 0 || in->reorder > 1 || in->graph < 0 || in->graph > 1 ||
-        in->strip < 0 || in->strip > 1024)
+        in->strip < 0 || in->strip > 1024 || in->merge < 0 || in->merge > 1)
         return QBT_EINVAL;
     c->tune_strip   = in->strip;
+    c->tune_merge   = in->merge;
     c->path         = in->path;
     c->tune_stack   = in->stack;
     c->tune_reorder = in->reorder;
@@ -1495,12 +1498,16 @@ extern "C" int qbt_autotune(qbt_ctx* c, const qbt_view* views, int nviews, const
     static const int refills[] = {0, 8, 16, 24, 32, 48}; /* 0: the per-launch defaults */
     for (int sweep = 0; sweep < 2; sweep++) {
         bool improved = false;
-        for (int knob = 0; knob < 4; knob++) {
+        for (int knob = 0; knob < 5; knob++) {
             const int* vals = knob == 0 ? blocks : knob == 1 ? refills : nullptr;
             const int  nv   = knob == 0 ? 3 : knob == 1 ? 6 : 2;
             for (int k = 0; k < nv; k++) {
                 qbt_tuning t = best;
-                int*       f = knob == 0 ? &t.block_threads : knob == 1 ? &t.refill : knob == 2 ? &t.stack : &t.reorder;
+                int*       f = knob == 0   ? &t.block_threads
+                               : knob == 1 ? &t.refill
+                               : knob == 2 ? &t.stack
+                               : knob == 3 ? &t.reorder
+                                           : &t.merge;
                 const int  v = vals ? vals[k] : k;
                 if (v == *f) continue;
                 *f = v;

@@ -17,7 +17,7 @@
 
 #define QBT_OUT_ALL 0x1FFu
 
-enum { CNT_SHADOW = 0, CNT_HITS, CNT_EXP, CNT_ABORT, CNT_PRED, CNT_QUEUE, CNT_EXP_PRIM, CNT_Q3, CNT_N };
+enum { CNT_SHADOW = 0, CNT_HITS, CNT_EXP, CNT_ABORT, CNT_PRED, CNT_QUEUE, CNT_EXP_PRIM, CNT_DEFER, CNT_N };
 enum { FR_HIT = 0, FR_LEAF, FR_SHADOW, FR_RGBA8, FR_RGBAF, FR_ISP, FR_FLAGS, FR_DYN, FR_EXP, FR_N };
 
 /* per-(light, pixel) shadow status of the wavefront path */
@@ -196,6 +196,7 @@ struct qbt_ctx {
     DevBuf       wb_gstack; /* traversal stacks in global memory (tuning stack = 1) */
     DevBuf       wb_bcnt;   /* per-block counts of the stream compactions */
     DevBuf       wb_lexp;   /* counting renders: shadow expansions per light and pixel */
+    DevBuf       wb_slots;  /* k_frame: shadow hand-over slots, 16 B per (light, camera work item) */
     DevBuf       bt_tab; /* batch tables: views, item and pixel offsets, frames */
     static constexpr int kTabSlots = 4;
     void*        tab_host[kTabSlots] = {};      /* pinned staging of the batch tables */
@@ -205,6 +206,7 @@ struct qbt_ctx {
     int          tune_reorder = 0; /* shadow rays in direction-octant order (measured: no gain) */
     int          tune_graph   = 1; /* replay batched renders from a captured hipGraph */
     int          tune_strip   = 0; /* tile order: 0 row-major, S > 0 vertical strips of S tiles */
+    int          tune_merge   = 1; /* exact renders: camera and shadow rays in one launch (k_frame) */
     /* captured render graph and the launch configuration it was captured for */
     hipGraph_t           graph      = nullptr;
     hipGraphExec_t       graph_exec = nullptr;

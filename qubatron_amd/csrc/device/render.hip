@@ -24,6 +24,11 @@
  *   k_shade    per pixel of the batch: attributes, shading (:436-449),
  *              sprite (:452-459), outputs into each view's frame
  *
+ * With tuning merge (exact renders without reordering), k_frame replaces
+ * k_primary, the list and k_shadow: one persistent launch whose waves take
+ * camera rays first and then the shadow rays of finished pixels, handed
+ * over through per-(light, work item) slots (see k_frame).
+ *
  * k_primary and k_shadow are persistent: each wave keeps up to 64 rays in
  * flight and refills idle lanes from a work counter with one atomic per
  * refill, so a lane that finishes early starts the next ray instead of
@@ -106,7 +111,9 @@ struct Refill {
         const uint32_t lo = seg_bound(n, seg), hi = seg_bound(n, seg + 1);
         uint32_t       base = 0;
         if (lane == leader) base = atomicAdd(&ctr[seg * kCtrPad], nidle);
-        base          = __shfl(base, leader, 64);
+        /* readlane of a uniform lane: a scalar, so seg and left stay in
+         * scalar registers */
+        base          = __builtin_amdgcn_readlane(base, leader);
         uint32_t item = UINT32_MAX;
         if ((idle >> lane) & 1ull) {
             uint32_t rank = (uint32_t) __popcll(idle & ((1ull << lane) - 1ull));
@@ -558,6 +565,315 @@ __global__ void __launch_bounds__(256) k_shadow(SceneDev s, LightsDev L, const u
     }
 }
 
+/* ---- camera and shadow rays in one persistent launch ------------------------
+ *
+ * k_frame runs k_primary's work items, then the shadow rays of k_shadow, in
+ * one launch: a wave whose camera work has run dry takes shadow rays while
+ * the last camera rays of other waves are still being traced, so the end of
+ * the camera pass no longer leaves the SIMDs with a few lanes each, and no
+ * compaction runs between the passes.
+ *
+ * Hand-over: per (light l, camera work item i) one slot of four words at
+ * slots + 4 (l * nitems + i): the state word and the entry point (x, y, z).
+ * All four hold kSlotFree before the launch. The camera lane that finishes
+ * item i writes the entry point, then exchanges the state word for the
+ * pixel index (kSlotNone when the pixel is not shaded); the shadow lane
+ * that claims the slot exchanges it for kSlotDefer. Whichever comes second
+ * sees the other's value: a shadow lane that finds a pixel index traces the
+ * ray (waiting, within a bound, until the entry point's words are visible:
+ * they are written before the exchange), and a camera lane that finds
+ * kSlotDefer traces that light's ray itself after its camera ray. No lane
+ * waits for another lane's ray. Slots are claimed in light-major item order,
+ * the order of k_shadow's lists. Slot words are read and written with
+ * device-scope atomics, so lanes on different XCDs see each other's values. */
+constexpr uint32_t kSlotFree  = 0xFFFFFFFFu; /* also the entry-point words not written yet (a NaN) */
+constexpr uint32_t kSlotDefer = 0xFFFFFFFEu;
+constexpr uint32_t kSlotNone  = 0xFFFFFFFDu;
+constexpr int      kSlotSpin  = 1 << 16; /* bound on the wait for an entry point; past it the ray counts as aborted */
+
+__device__ __forceinline__ uint32_t slot_load(uint32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void slot_store(uint32_t* p, uint32_t v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t slot_xchg(uint32_t* p, uint32_t v)
+{
+    return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* lane state of k_frame, packed to keep the kernel at the separate
+ * launches' register count: gp = the ray's batch pixel; k = the camera ray's
+ * work item, or the shadow ray's slot; tag = kind bits, the sprite slot of a
+ * sprite trace, and the lights still deferred to this lane (bits 8..15) */
+constexpr uint32_t kTagShadow = 0x80000000u; /* the lane's ray is a shadow ray */
+constexpr uint32_t kTagSprite = 0x40000000u; /* camera-side sprite trace; low bits: sprite slot */
+constexpr uint32_t kTagFresh  = 0x20000000u; /* k still holds the finished camera item */
+constexpr uint32_t kTagPend   = 0x0000ff00u; /* deferred lights, bit 8 + l */
+
+/* light and camera item of slot k = l * nitems + item */
+__device__ __forceinline__ int slot_light(uint32_t k, uint32_t nitems, int nl, uint32_t& item)
+{
+    uint32_t it = k;
+    int      l  = 0;
+    for (int j = 1; j < nl; j++) {
+        const bool past = it >= nitems;
+        it -= past ? nitems : 0u;
+        l += past ? 1 : 0;
+    }
+    item = it;
+    return l;
+}
+
+template <bool CMP, bool DUAL, bool GRID, bool COUNT, bool GSTK>
+__global__ void __launch_bounds__(256)
+    k_frame(SceneDev s, BatchDev b, PrimOut o, LightsDev L, uint32_t nitems, uint32_t nspr, int strip,
+            int* __restrict__ sprite, uint32_t* __restrict__ slots, uint8_t* __restrict__ lstat, size_t npix,
+            unsigned* __restrict__ disc, unsigned* __restrict__ q, unsigned refill_p, unsigned refill_s,
+            uint32_t* gstack, int* __restrict__ lexp, unsigned long long* __restrict__ counters)
+{
+    extern __shared__ uint32_t lds[];
+    const Stack st   = lane_stack<DUAL, GSTK>(lds, gstack, s.maxlevel);
+    const int   lane = threadIdx.x & 63;
+    const int   nl   = L.n;
+    Refill      rp(nspr + nitems), rs(nitems * (uint32_t) nl);
+    bool        active = false;
+    uint32_t    gp = 0, k = 0, tag = 0;
+    Ray         r = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    TraceState  t;
+    unsigned    n_exp = 0;
+    int         fin   = ST_CONT; /* status of a ray that ended and awaits its outputs */
+    /* shaded camera hits and shadow rays traced, per wave: ballots taken in
+     * wave-uniform code, so the sums sit in scalar registers */
+    uint32_t    w_hit = 0, w_ray = 0, w_def = 0;
+
+    auto abort_count = [&]() { atomicAdd(&counters[CNT_ABORT], 1ull); }; /* rare: no register kept for it */
+
+    /* the shadow ray's result: lit when the light's first leaf starts within
+     * sqrt(15) of the entry point it aims at (octree_fsh.c:431-441) */
+    auto finish_shadow = [&](int stt) {
+        uint32_t  item;
+        const int l = slot_light(k, nitems, nl, item);
+        uint8_t   ls;
+        if (stt == ST_DISCARD) {
+            ls = LS_DISCARD;
+            atomicMin(&disc[gp], (unsigned) l);
+        } else {
+            if (stt == ST_ABORT) abort_count();
+            uint32_t*   w   = slots + 4 * (size_t) k;
+            const float px  = __uint_as_float(slot_load(w + 1));
+            const float py  = __uint_as_float(slot_load(w + 2));
+            const float pz  = __uint_as_float(slot_load(w + 3));
+            float4      li  = stt == ST_HIT ? t.entry : make_float4(0.f, 0.f, 0.f, 0.f);
+            float       ix  = li.x - px, iy = li.y - py, iz = li.z - pz;
+            float       sqr = ix * ix + iy * iy + iz * iz;
+            ls              = (15.0f < sqr) ? LS_OCCLUDED : LS_LIT; /* step(sqr, 15.0) */
+        }
+        lstat[(size_t) l * npix + gp] = ls;
+        if (COUNT) lexp[(size_t) l * npix + gp] = (int) t.exps;
+    };
+
+    /* the ray of slot k (light l) to the entry point (px, py, pz) of pixel gp */
+    auto start_shadow = [&](int l, float px, float py, float pz) -> bool {
+        if (disc[gp] < (unsigned) l) { /* a lower light discarded the fragment */
+            lstat[(size_t) l * npix + gp] = LS_SKIPPED;
+            if (COUNT) lexp[(size_t) l * npix + gp] = 0;
+            return false;
+        }
+        /* the light's position by selects (a per-lane index into the lights
+         * would go through scratch memory) */
+        float lx = L.p[0][0], ly = L.p[0][1], lz = L.p[0][2];
+        for (int j = 1; j < nl; j++) {
+            lx = l == j ? L.p[j][0] : lx;
+            ly = l == j ? L.p[j][1] : ly;
+            lz = l == j ? L.p[j][2] : lz;
+        }
+        r.px          = lx;
+        r.py          = ly;
+        r.pz          = lz;
+        r.dx          = px - lx;
+        r.dy          = py - ly;
+        r.dz          = pz - lz;
+        const int stt = trace_begin<false>(s, r, t);
+        if (stt == ST_CONT) active = true;
+        else fin = stt; /* outputs at the next refill point */
+        return true;
+    };
+
+    /* the camera ray's outputs (k_primary's), then its slots */
+    auto finish_camera = [&](int stt) -> bool {
+        if (tag & kTagSprite) { /* white if the light is in front of the first surface (resv / lghtv.x > 1) */
+            const float ix = stt == ST_HIT ? t.entry.x : 0.0f;
+            int         f  = 0;
+            if (stt == ST_DISCARD) f |= 1;
+            if ((ix - r.px) / r.dx > 1.0f) f |= 2;
+            if (stt == ST_ABORT) f |= 4;
+            sprite[tag & ~kTagSprite] = f;
+            tag                       = 0;
+            return false;
+        }
+        float4  isp = make_float4(0.f, 0.f, 0.f, 0.f);
+        int     ms = -1, md = -1, leaf = -1;
+        uint8_t f  = 0;
+        bool    sh = false;
+        if (stt == ST_DISCARD) f = QBT_PX_DISCARD;
+        if (stt == ST_ABORT) abort_count();
+        if (stt == ST_HIT) {
+            const int L_ = s.maxlevel;
+            ms           = (t.ns == 0 && L_ > 0) ? 0 : s.stat.model[t.ns];
+            md           = 0;
+            if (DUAL) md = (t.nd == 0 && L_ > 0) ? 0 : s.dyn.model[t.nd];
+            leaf = CMP ? s.stat.ref[t.ns] : t.ns; /* reference node index */
+            isp  = t.entry;
+            f    = QBT_PX_LEAF;
+            if (isp.w > 0.0f) {
+                f |= QBT_PX_SHADED;
+                sh = true;
+            }
+        }
+        o.isp[gp]   = isp;
+        o.ms[gp]    = ms;
+        o.md[gp]    = md;
+        o.leaf[gp]  = leaf;
+        o.flags[gp] = f;
+        if (COUNT) {
+            o.exp[gp] = (int) t.exps;
+            n_exp += t.exps;
+        }
+        uint32_t pend = 0;
+        for (int l = 0; l < nl; l++) {
+            uint32_t* w = slots + 4 * ((size_t) l * nitems + k);
+            if (sh) {
+                slot_store(w + 1, __float_as_uint(isp.x));
+                slot_store(w + 2, __float_as_uint(isp.y));
+                slot_store(w + 3, __float_as_uint(isp.z));
+            }
+            const uint32_t old = slot_xchg(w, sh ? gp : kSlotNone);
+            if (sh && old == kSlotDefer) pend |= 1u << l;
+        }
+        tag = pend ? (kTagShadow | kTagFresh | pend << 8) : 0u;
+        return sh;
+    };
+
+    for (;;) {
+        bool hit_now = false, ray_now = false, def_now = false; /* this lane's events of this refill point */
+        if (fin != ST_CONT) { /* rays that ended since the last refill point, together */
+            if (tag & kTagShadow) finish_shadow(fin);
+            else hit_now = finish_camera(fin);
+            fin = ST_CONT;
+        }
+        if (!active && (tag & kTagPend)) { /* the next deferred light of this lane's pixel */
+            uint32_t item = k;
+            if (!(tag & kTagFresh)) (void) slot_light(k, nitems, nl, item);
+            const uint32_t pend = (tag & kTagPend) >> 8;
+            const int      l    = __ffs(pend) - 1;
+            tag                 = kTagShadow | (pend & (pend - 1u)) << 8;
+            k                   = (uint32_t) l * nitems + item;
+            uint32_t* w         = slots + 4 * (size_t) k; /* written by this lane */
+            ray_now = start_shadow(l, __uint_as_float(slot_load(w + 1)), __uint_as_float(slot_load(w + 2)),
+                                   __uint_as_float(slot_load(w + 3)));
+            def_now = ray_now;
+        }
+        /* free lanes: no ray, no deferred light, no outputs still to write */
+        const uint64_t idle = __ballot(!active && !(tag & kTagPend) && fin == ST_CONT);
+        if (rp.left > 0) {
+            if (__popcll(idle) >= (int) refill_p || idle == ~0ull) {
+                const uint32_t fetched = rp.fetch(&q[Q_PRIM], idle, lane);
+                if (fetched != UINT32_MAX && fetched < nspr) {
+                    const int      v = (int) (fetched / (uint32_t) nl), l = (int) (fetched % (uint32_t) nl);
+                    const ViewDev& V = b.views[v];
+                    tag              = kTagSprite | (uint32_t) (v * kNL + l);
+                    r.px             = V.cam[0];
+                    r.py             = V.cam[1];
+                    r.pz             = V.cam[2];
+                    r.dx             = V.light[l][0] - V.cam[0];
+                    r.dy             = V.light[l][1] - V.cam[1];
+                    r.dz             = V.light[l][2] - V.cam[2];
+                    const int stt    = trace_begin<false>(s, r, t);
+                    if (stt == ST_CONT) active = true;
+                    else fin = stt;
+                } else if (fetched != UINT32_MAX) {
+                    const uint32_t it = fetched - nspr;
+                    const int      v  = batch_view(b.item_off, b.n, it);
+                    const ViewDev& V  = b.views[v];
+                    int            px, py;
+                    tag = 0;
+                    k   = it;
+                    if (item_pixel(it - b.item_off[v], V.w, V.h, strip, px, py)) {
+                        gp = b.pix_off[v] + (uint32_t) py * (uint32_t) V.w + (uint32_t) px;
+                        float csv[3];
+                        pixel_dir(V, px, py, csv);
+                        r.px          = V.cam[0];
+                        r.py          = V.cam[1];
+                        r.pz          = V.cam[2];
+                        r.dx          = csv[0];
+                        r.dy          = csv[1];
+                        r.dz          = csv[2];
+                        const int stt = trace_begin<false>(s, r, t);
+                        if (stt == ST_CONT) active = true;
+                        else fin = stt;
+                    } else { /* padding item of an edge tile: no shadow rays */
+                        for (int l = 0; l < nl; l++) (void) slot_xchg(slots + 4 * ((size_t) l * nitems + it), kSlotNone);
+                    }
+                }
+            }
+        } else if (rs.left > 0) {
+            if (__popcll(idle) >= (int) refill_s || idle == ~0ull) {
+                const uint32_t ks = rs.fetch(&q[Q_SHADOW], idle, lane);
+                if (ks != UINT32_MAX) {
+                    uint32_t*      w   = slots + 4 * (size_t) ks;
+                    const uint32_t old = slot_xchg(w, kSlotDefer);
+                    if (old < kSlotNone) { /* the camera lane has published its pixel */
+                        uint32_t x = slot_load(w + 1), y = slot_load(w + 2), z = slot_load(w + 3);
+                        for (int i = 0; (x == kSlotFree || y == kSlotFree || z == kSlotFree) && i < kSlotSpin; i++) {
+                            __builtin_amdgcn_s_sleep(1);
+                            x = slot_load(w + 1);
+                            y = slot_load(w + 2);
+                            z = slot_load(w + 3);
+                        }
+                        uint32_t  item;
+                        const int l = slot_light(ks, nitems, nl, item);
+                        gp          = old;
+                        k           = ks;
+                        tag         = kTagShadow;
+                        if (x == kSlotFree || y == kSlotFree || z == kSlotFree) {
+                            abort_count(); /* reported to the caller as an aborted render */
+                            lstat[(size_t) l * npix + gp] = LS_OCCLUDED;
+                        } else {
+                            ray_now = start_shadow(l, __uint_as_float(x), __uint_as_float(y), __uint_as_float(z));
+                        }
+                    }
+                }
+            }
+        }
+        w_hit += (uint32_t) __popcll(__ballot(hit_now));
+        w_ray += (uint32_t) __popcll(__ballot(ray_now));
+        w_def += (uint32_t) __popcll(__ballot(def_now));
+        if (__ballot(active) == 0ull) {
+            if (rp.left <= 0 && rs.left <= 0 && __ballot((tag & kTagPend) != 0u || fin != ST_CONT) == 0ull) break;
+            continue;
+        }
+        /* step until `keep` lanes are left: refill thresholds as in the two
+         * separate launches; a wave with deferred lights returns as soon as
+         * one ray ends, so that lane starts its next light */
+        const int keep = __ballot((tag & kTagPend) != 0u) ? 64
+                         : rp.left > 0                    ? 65 - (int) refill_p
+                         : rs.left > 0                    ? 65 - (int) refill_s
+                                                          : 1;
+        if (active) step_run<CMP, DUAL, GRID>(s, r, st, t, active, keep, fin);
+    }
+    if (COUNT) n_exp = wave_sum(n_exp);
+    if (lane == 0) {
+        if (w_hit) atomicAdd(&counters[CNT_HITS], (unsigned long long) w_hit);
+        if (w_ray) atomicAdd(&counters[CNT_SHADOW], (unsigned long long) w_ray);
+        if (w_def) atomicAdd(&counters[CNT_DEFER], (unsigned long long) w_def);
+        if (COUNT && n_exp) atomicAdd(&counters[CNT_EXP], (unsigned long long) n_exp);
+        if (COUNT && n_exp) atomicAdd(&counters[CNT_EXP_PRIM], (unsigned long long) n_exp);
+    }
+}
+
 /* per batch pixel: attributes, shading, sprite, outputs (octree_fsh.c:421-463);
  * counting renders add the shadow expansions of the lights up to the first
  * discarding one (lexp) to the pixel's count and to the counters */
@@ -710,10 +1026,10 @@ int persistent_grid(qbt_ctx* c, K kernel, int bs, size_t lds, size_t items)
 /* the launch configuration a captured graph depends on (kernel arguments) */
 struct GraphKey {
     SceneDev s;
-    void*    bufs[17];
+    void*    bufs[18];
     uint32_t nitems, npix, nviews, nlights;
     unsigned mask, flags;
-    int      bs, refill, gp, gs, stack, reorder, strip;
+    int      bs, refill, gp, gs, gf, stack, reorder, strip, merged;
     float    band[2];
     float    lights[kNL][3];
     uint32_t tmpl;
@@ -752,6 +1068,8 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     const bool     surrogate = (flags & QBT_RENDER_SURROGATE) && nl > 0;
     const bool     collect   = (flags & QBT_RENDER_COLLECT) && nl > 0 && !surrogate;
     const bool     reorder   = c->tune_reorder && nl > 0;
+    /* exact renders: camera and shadow rays in one launch (k_frame) */
+    const bool     merged    = c->tune_merge && nl > 0 && !surrogate && !collect && !reorder;
     const uint32_t nspr      = (uint32_t) b.n * (uint32_t) (nl > 0 ? nl : 0); /* sprite traces */
     int            rc;
     /* every buffer exists before a capture starts */
@@ -763,6 +1081,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     if ((rc = c->dev_alloc(c->wb_bcnt, (size_t) nb_max * 4 * (nl > 0 ? nl : 1)))) return rc;
     if (surrogate && (rc = c->dev_alloc(c->wb_fb, (size_t) npix * 4 * nl))) return rc;
     if (COUNT && nl > 0 && (rc = c->dev_alloc(c->wb_lexp, (size_t) npix * 4 * nl))) return rc;
+    if (merged && (rc = c->dev_alloc(c->wb_slots, (size_t) nitems * 16 * nl))) return rc;
 
     const int      bs     = c->block_threads(s.maxlevel, DUAL);
     const size_t   lds    = GSTK ? 0 : qbt_stack_bytes(s.maxlevel, DUAL, bs);
@@ -773,14 +1092,16 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     const unsigned refill_sh = (unsigned) (tuned ? c->tune_refill : 16);
     auto           kp     = k_primary<CMP, DUAL, GRID, COUNT, GSTK>;
     auto           ks     = k_shadow<CMP, DUAL, GRID, COUNT, GSTK>;
+    auto           kf     = k_frame<CMP, DUAL, GRID, COUNT, GSTK>;
     /* resident blocks, but no more than one work item per lane: small
      * batches then start fewer waves (less launch and refill-counter
      * contention); shadow items are at most one per batch pixel and light */
     const size_t   np = (size_t) nitems + nspr, ns = (size_t) npix * (nl > 0 ? nl : 1);
     const int      gp = std::min(persistent_grid(c, kp, bs, lds, np), (int) nblk(np, (unsigned) bs));
     const int      gs = std::min(persistent_grid(c, ks, bs, lds, ns), (int) nblk(ns, (unsigned) bs));
-    if (GSTK && (rc = c->dev_alloc(c->wb_gstack, qbt_stack_bytes(s.maxlevel, DUAL, (gp > gs ? gp : gs) * bs))))
-        return rc;
+    const int      gf = merged ? std::min(persistent_grid(c, kf, bs, lds, np + ns), (int) nblk(np + ns, (unsigned) bs)) : 0;
+    const int gmax = std::max(std::max(gp, gs), gf);
+    if (GSTK && (rc = c->dev_alloc(c->wb_gstack, qbt_stack_bytes(s.maxlevel, DUAL, gmax * bs)))) return rc;
 
     PrimOut po;
     po.isp   = (float4*) c->wb_isp.p;
@@ -806,6 +1127,18 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
         k_fill_u32n<<<nblk(npix), 256, 0, st>>>((unsigned*) c->wb_disc.p, npix, ~0u);
         if (COUNT && nl > 0) /* lights the surrogate decided add no shadow expansions */
             k_fill_u32n<<<nblk((size_t) npix * nl), 256, 0, st>>>((unsigned*) c->wb_lexp.p, (size_t) npix * nl, 0u);
+        if (merged) {
+            const size_t nw = (size_t) nitems * 4 * nl;
+            k_fill_u32n<<<nblk(nw), 256, 0, st>>>((unsigned*) c->wb_slots.p, nw, kSlotFree);
+            c->prof_mark_begin(QBT_PROF_FRAME);
+            kf<<<gf, bs, lds, st>>>(s, b, po, L, nitems, nspr, c->tune_strip, (int*) c->sprite.p,
+                                    (uint32_t*) c->wb_slots.p, (uint8_t*) c->wb_lstat.p, npix,
+                                    (unsigned*) c->wb_disc.p, q, refill, refill_sh, gstk, (int*) c->wb_lexp.p, cnt);
+            c->prof_mark_end();
+            k_shade<DUAL, COUNT><<<nblk(npix), 256, 0, st>>>(s, b, po, (const uint8_t*) c->wb_lstat.p,
+                                                             (const int*) c->wb_lexp.p, npix, mask, cnt);
+            return QBT_OK;
+        }
         c->prof_mark_begin(QBT_PROF_PRIMARY);
         kp<<<gp, bs, lds, st>>>(s, b, po, nitems, nspr, nl > 0 ? nl : 1, c->tune_strip, (int*) c->sprite.p, q,
                                 refill, gstk, cnt);
@@ -843,7 +1176,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     if (!use_graph) {
         if ((rc = enqueue())) return rc;
         HIPCHK(hipGetLastError());
-        c->last.launches += 6 + (nl > 0 ? 1 : 0) + (reorder ? 3 : 0) + (surrogate ? 4 : 0);
+        c->last.launches += merged ? 5 + (COUNT ? 1 : 0) : 6 + (nl > 0 ? 1 : 0) + (reorder ? 3 : 0) + (surrogate ? 4 : 0);
         return QBT_OK;
     }
 
@@ -853,12 +1186,12 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     key.s      = s;
     void* bp[] = {c->wb_isp.p,   c->wb_ms.p,    c->wb_md.p,   c->wb_leaf.p,  c->wb_flags.p, c->wb_disc.p,
                   c->wb_lstat.p, c->wb_exp.p,   c->wb_cand.p, c->wb_items.p, c->wb_q.p,     c->wb_lists.p,
-                  c->wb_fb.p,    c->wb_gstack.p, c->bt_tab.p, c->sprite.p, c->wb_bcnt.p};
+                  c->wb_fb.p,    c->wb_gstack.p, c->bt_tab.p, c->sprite.p, c->wb_bcnt.p, c->wb_slots.p};
     static_assert(sizeof(bp) == sizeof(key.bufs), "graph key buffers");
     memcpy(key.bufs, bp, sizeof(bp));
     key.nitems = nitems, key.npix = npix, key.nviews = (uint32_t) b.n, key.nlights = (uint32_t) nl;
     key.mask = mask, key.flags = flags, key.bs = bs, key.refill = (int) (refill | refill_sh << 8), key.gp = gp, key.gs = gs;
-    key.stack = GSTK, key.reorder = reorder, key.strip = c->tune_strip;
+    key.stack = GSTK, key.reorder = reorder, key.strip = c->tune_strip, key.gf = gf, key.merged = merged;
     memcpy(key.lights, L.p, sizeof(key.lights));
     key.tmpl = (CMP ? 1u : 0u) | (DUAL ? 2u : 0u) | (GRID ? 4u : 0u);
     if (surrogate) qbt_surrogate_band(c, &key.band[0], &key.band[1]);

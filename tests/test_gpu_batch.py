@@ -26,7 +26,7 @@ def scene_ctx(gpu_ctx):
     gpu_ctx.upload_model(col, nrm)
     gpu_ctx.upload_octree(np.zeros((1, 12), np.int32), dynamic=True)
     yield gpu_ctx, sc, po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth)
-    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=0, graph=1, strip=0)
+    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=0, graph=1, strip=0, merge=1)
 
 
 def _views(sc, seed, sizes=SIZES, maxlevel=None):
@@ -44,15 +44,18 @@ def _same(a, b, what):
         assert np.array_equal(a[k], b[k]), f"{what}: {k} differs ({np.count_nonzero(a[k] != b[k])} elements)"
 
 
-@pytest.mark.parametrize("stack,reorder,strip", [(0, 1, 0), (0, 0, 0), (1, 1, 0), (1, 0, 0), (0, 0, 2), (0, 0, 5)])
-def test_batch_equals_oracle(scene_ctx, stack, reorder, strip):
+@pytest.mark.parametrize("stack,reorder,strip,merge", [(0, 1, 0, 1), (0, 0, 0, 1), (0, 0, 0, 0), (1, 1, 0, 1),
+                                                      (1, 0, 0, 1), (1, 0, 0, 0), (0, 0, 2, 1), (0, 0, 5, 1)])
+def test_batch_equals_oracle(scene_ctx, stack, reorder, strip, merge):
     ctx, sc, osc = scene_ctx
-    ctx.set_tuning(path=1, stack=stack, reorder=reorder, graph=0, strip=strip)
+    ctx.set_tuning(path=1, stack=stack, reorder=reorder, graph=0, strip=strip, merge=merge)
     views, pos, ang = _views(sc, 21)
-    print(f"[case] batch of {len(views)} views stack={stack} reorder={reorder}", flush=True)
+    print(f"[case] batch of {len(views)} views stack={stack} reorder={reorder} merge={merge}", flush=True)
     ctx.render(views, LIGHTS4, MASK)
     st = ctx.stats()
     assert st["primary_rays"] == sum(w * h for w, h in SIZES)
+    if not merge or reorder:  # the merged launch runs only for exact renders without reordering
+        assert st["shadow_deferred"] == 0
     for i, (w, h) in enumerate(SIZES):
         o = osc.render(pos[i], ang[i], w, h, LIGHTS4, shoot=i % 2)
         _same(ctx.read_frame(i), o, f"view {i} {w}x{h}")
@@ -116,7 +119,7 @@ def test_autotune_picks_a_valid_tuning(scene_ctx):
     best, table = ctx.autotune(views, LIGHTS4[:2], MASK, reps=2)
     assert len(table) >= 8
     assert best["path"] == 1 and best["block_threads"] in (64, 128, 256)
-    assert best["stack"] in (0, 1) and best["reorder"] in (0, 1)
+    assert best["stack"] in (0, 1) and best["reorder"] in (0, 1) and best["merge"] in (0, 1)
     assert min(ms for _, ms in table) > 0
     ctx.render(views, LIGHTS4[:2], MASK)
     for i, (w, h) in enumerate(SIZES[:3]):

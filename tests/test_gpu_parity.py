@@ -20,15 +20,17 @@ ALL = q.OUT_ALL | q.RENDER_COUNT
 
 
 @pytest.fixture(params=[dict(path=1, layout=0), dict(path=1, layout=1), dict(path=0, layout=1),
-                        dict(path=1, layout=0, stack=1, reorder=0, strip=3)],
-                ids=["wavefront-compact", "wavefront-direct", "fused", "wavefront-gstack"])
+                        dict(path=1, layout=0, stack=1, reorder=0, strip=3), dict(path=1, layout=0, merge=0)],
+                ids=["wavefront-compact", "wavefront-direct", "fused", "wavefront-gstack", "wavefront-two-launch"])
 def ctx(gpu_ctx, request):
     """Every render path: wavefront stages on the compact node layout
-    (default), wavefront on the direct layout, the fused kernel, and the
-    wavefront stages with global-memory stacks and tiles in 3-tile strips."""
+    (default: camera and shadow rays in one merged launch), wavefront on the
+    direct layout, the fused kernel, the wavefront stages with global-memory
+    stacks and tiles in 3-tile strips, and the wavefront stages with separate
+    camera and shadow launches."""
     gpu_ctx.set_tuning(**request.param)
     yield gpu_ctx
-    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=0, graph=1, strip=0)
+    gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=0, graph=1, strip=0, merge=1)
 
 
 def test_shared_child_falls_back_to_direct(ctx):
@@ -481,3 +483,40 @@ def test_render_and_trace_limits(ctx):
     ctx.render(v, np.full((8, 3), 900.0, np.float32))  # 8 lights is allowed
     p, s = ctx.particle_update(np.zeros((0, 3)), np.zeros((0, 3)), sc.depth, float(sc.basecube[3]))
     assert p.shape == (0, 3) and s.shape == (0, 3)
+
+
+def test_merged_launch_matches_two_launches(gpu_ctx):
+    """The merged camera-and-shadow launch (k_frame) against the separate
+    camera and shadow launches on the same frames: every output, the
+    expansion counts and the ray statistics identical, and the oracle on the
+    largest frame. Small frames put every camera ray in flight at once, so
+    shadow lanes reach most slots before their camera ray ends and hand the
+    ray back to the camera lane (stats shadow_deferred): both hand-over
+    directions are exercised."""
+    ctx = gpu_ctx
+    sc, pnt, nrm, col, nodes = H.scene(1, npoints=200000)
+    _upload_scene(ctx, nodes, col, nrm)
+    lights = np.array([[900.0, 1700.0, 900.0], [300.0, 1500.0, 1200.0]], np.float32)
+    pos, ang = sc.poses(3, seed=17)
+    deferred = 0
+    try:
+        for i, (W, H_) in enumerate([(8, 8), (64, 48), (320, 240)]):
+            v = q.view(pos[i], ang[i], W, H_, sc.basecube, sc.depth, i % 2)
+            out = {}
+            for merge in (0, 1):
+                ctx.set_tuning(path=1, layout=0, merge=merge)
+                print(f"[case] merge={merge} {W}x{H_}", flush=True)
+                ctx.render(v, lights, ALL)
+                out[merge] = (ctx.read_frame(0), ctx.stats())
+            (f0, s0), (f1, s1) = out[0], out[1]
+            for k in f0:
+                assert np.array_equal(f0[k], f1[k]), f"{W}x{H_}: {k} differs"
+            for k in ("primary_rays", "shadow_rays", "hits", "expansions", "expansions_primary"):
+                assert s0[k] == s1[k], (k, s0[k], s1[k])
+            assert s0["shadow_deferred"] == 0 and s1["shadow_deferred"] <= s1["shadow_rays"]
+            deferred += s1["shadow_deferred"]
+        o = po.OracleScene(nodes, col, nrm, sc.basecube, sc.depth).render(pos[2], ang[2], 320, 240, lights, shoot=0)
+        _assert_same(f1, o, "merged 320x240")
+        assert deferred > 0
+    finally:
+        ctx.set_tuning(path=1, layout=0, merge=1)
