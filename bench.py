@@ -130,7 +130,7 @@ def parse():
     return ap.parse_args()
 
 
-TRAVERSAL_KERNELS = ("k_render", "k_primary", "k_shadow")
+TRAVERSAL_KERNELS = ("k_render", "k_primary", "k_shadow", "k_frame")
 
 
 def summarize_pmc(dirs, out_path):
@@ -146,7 +146,7 @@ def summarize_pmc(dirs, out_path):
                     kern = None
                     if "k_copy16" in name:
                         kern = "copy"
-                    m = re.search(r"\b(k_render|k_primary|k_shadow)<([^>]*)>", name)
+                    m = re.search(r"\b(k_render|k_primary|k_shadow|k_frame)<([^>]*)>", name)
                     if m:  # not the counting variant: COUNT is k_render's 3rd, the others' 4th argument
                         targs = [a.strip() for a in m.group(2).split(",")]
                         ci = 2 if m.group(1) == "k_render" else 3

@@ -243,9 +243,11 @@ int qbt_profile_end(qbt_ctx* ctx, qbt_profile* out);
  * launch configuration is unchanged. strip: order of the 8x8 pixel tiles,
  * 0 = row-major, S (1..1024) = vertical strips S tiles wide, each row-major.
  * merge: 1 = an exact render (no surrogate, no sample collection, reorder 0)
+ * whose camera rays number fewer than 16 per lane of the full persistent grid
  * traces its camera and shadow rays in one persistent launch, the shadow
- * rays of finished pixels taken by waves whose camera work has run dry; 0 =
- * a camera launch, then a shadow launch. Results never depend on these. */
+ * rays of finished pixels taken by waves whose camera work has run dry; 0,
+ * and longer camera passes: a camera launch, then a shadow launch. Results
+ * never depend on these. */
 typedef struct qbt_tuning {
     int path;
     int block_threads;

@@ -1010,15 +1010,30 @@ constexpr int kSmallGridWavesPerSimd = 4;
 constexpr int kFullGridRays          = 16;
 
 template <class K>
-int persistent_grid(qbt_ctx* c, K kernel, int bs, size_t lds, size_t items)
+int resident_blocks(K kernel, int bs, size_t lds)
 {
     int per = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, bs, lds) != hipSuccess || per <= 0) {
         (void) hipGetLastError();
         per = 1;
     }
-    const size_t lanes = (size_t) per * (size_t) c->num_cus * (size_t) bs;
-    if (items >= (size_t) kFullGridRays * lanes) return per * c->num_cus;
+    return per;
+}
+
+/* whether `items` rays fill the full resident grid of `kernel` (at least
+ * kFullGridRays per lane) */
+template <class K>
+bool fills_grid(qbt_ctx* c, K kernel, int bs, size_t lds, size_t items)
+{
+    const size_t lanes = (size_t) resident_blocks(kernel, bs, lds) * (size_t) c->num_cus * (size_t) bs;
+    return items >= (size_t) kFullGridRays * lanes;
+}
+
+template <class K>
+int persistent_grid(qbt_ctx* c, K kernel, int bs, size_t lds, size_t items)
+{
+    const int per = resident_blocks(kernel, bs, lds);
+    if (fills_grid(c, kernel, bs, lds, items)) return per * c->num_cus;
     const int cap = std::max(1, 4 * kSmallGridWavesPerSimd / std::max(1, bs / 64)); /* blocks per CU */
     return std::min(per, cap) * c->num_cus;
 }
@@ -1068,8 +1083,6 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     const bool     surrogate = (flags & QBT_RENDER_SURROGATE) && nl > 0;
     const bool     collect   = (flags & QBT_RENDER_COLLECT) && nl > 0 && !surrogate;
     const bool     reorder   = c->tune_reorder && nl > 0;
-    /* exact renders: camera and shadow rays in one launch (k_frame) */
-    const bool     merged    = c->tune_merge && nl > 0 && !surrogate && !collect && !reorder;
     const uint32_t nspr      = (uint32_t) b.n * (uint32_t) (nl > 0 ? nl : 0); /* sprite traces */
     int            rc;
     /* every buffer exists before a capture starts */
@@ -1081,7 +1094,6 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     if ((rc = c->dev_alloc(c->wb_bcnt, (size_t) nb_max * 4 * (nl > 0 ? nl : 1)))) return rc;
     if (surrogate && (rc = c->dev_alloc(c->wb_fb, (size_t) npix * 4 * nl))) return rc;
     if (COUNT && nl > 0 && (rc = c->dev_alloc(c->wb_lexp, (size_t) npix * 4 * nl))) return rc;
-    if (merged && (rc = c->dev_alloc(c->wb_slots, (size_t) nitems * 16 * nl))) return rc;
 
     const int      bs     = c->block_threads(s.maxlevel, DUAL);
     const size_t   lds    = GSTK ? 0 : qbt_stack_bytes(s.maxlevel, DUAL, bs);
@@ -1093,6 +1105,13 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     auto           kp     = k_primary<CMP, DUAL, GRID, COUNT, GSTK>;
     auto           ks     = k_shadow<CMP, DUAL, GRID, COUNT, GSTK>;
     auto           kf     = k_frame<CMP, DUAL, GRID, COUNT, GSTK>;
+    /* exact renders: camera and shadow rays in one launch (k_frame) when the
+     * camera pass alone would not fill the full grid (then the end of that
+     * pass, a few long rays on mostly idle SIMDs, is a large share of it);
+     * longer camera passes keep the separate launches at the full grid */
+    const bool merged = c->tune_merge && nl > 0 && !surrogate && !collect && !reorder &&
+                        !fills_grid(c, kp, bs, lds, (size_t) nitems + nspr);
+    if (merged && (rc = c->dev_alloc(c->wb_slots, (size_t) nitems * 16 * nl))) return rc;
     /* resident blocks, but no more than one work item per lane: small
      * batches then start fewer waves (less launch and refill-counter
      * contention); shadow items are at most one per batch pixel and light */
