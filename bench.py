@@ -126,7 +126,8 @@ def parse():
     ap.add_argument("--graph", type=int, default=None, help="1 replay captured hipGraphs")
     ap.add_argument("--strip", type=int, default=None, help="tile order: 0 row-major, S strips of S tiles")
     ap.add_argument("--merge", type=int, default=None,
-                    help="1 camera and shadow rays of exact renders in one launch (k_frame), 0 two launches")
+                    help="camera and shadow rays of exact renders in one launch (k_frame): 0 never, "
+                         "1 where it pays (library default), 2 always")
     return ap.parse_args()
 
 

@@ -242,12 +242,12 @@ int qbt_profile_end(qbt_ctx* ctx, qbt_profile* out);
  * capture the batch's launches into a hipGraph and replay it while the
  * launch configuration is unchanged. strip: order of the 8x8 pixel tiles,
  * 0 = row-major, S (1..1024) = vertical strips S tiles wide, each row-major.
- * merge: 1 = an exact render (no surrogate, no sample collection, reorder 0)
- * whose camera rays number fewer than 16 per lane of the full persistent grid
- * traces its camera and shadow rays in one persistent launch, the shadow
- * rays of finished pixels taken by waves whose camera work has run dry; 0,
- * and longer camera passes: a camera launch, then a shadow launch. Results
- * never depend on these. */
+ * merge: whether an exact render (no surrogate, no sample collection,
+ * reorder 0) traces its camera and shadow rays in one persistent launch, the
+ * shadow rays of finished pixels taken by waves whose camera work has run
+ * dry, instead of a camera launch, then a shadow launch: 0 = never, 1 =
+ * where it pays (default: two or more lights and 1 to 16 camera rays per lane
+ * of the full persistent grid), 2 = always. Results never depend on these. */
 typedef struct qbt_tuning {
     int path;
     int block_threads;
@@ -267,7 +267,7 @@ int qbt_set_tuning(qbt_ctx* ctx, const qbt_tuning* in);
  * counting/collection flags are dropped) under candidate tunings of the
  * wavefront path - coordinate descent over block_threads {64, 128, 256},
  * refill {defaults, 8, 16, 24, 32, 48}, stack {LDS, global}, reorder {0, 1}
- * and merge {0, 1} (merge takes effect with reorder 0), two
+ * and merge {0, 1, 2} (merge takes effect with reorder 0), two
  * sweeps, a candidate kept only if it is at least 1% faster - and leaves the
  * fastest set on the context. results (up to max_results) receive every
  * timed candidate, *nresults how many were timed. reps: timed renders per

@@ -1428,7 +1428,7 @@ extern "C" int qbt_set_tuning(qbt_ctx* c, const qbt_tuning* in)
     if (in->refill < 0 || in->refill > 64) return QBT_EINVAL;
     if (in->layout < 0 || in->layout > 1) return QBT_EINVAL;
     if (in->stack < 0 || in->stack > 1 || in->reorder < 0 || in->reorder > 1 || in->graph < 0 || in->graph > 1 ||
-        in->strip < 0 || in->strip > 1024 || in->merge < 0 || in->merge > 1)
+        in->strip < 0 || in->strip > 1024 || in->merge < 0 || in->merge > 2)
         return QBT_EINVAL;
     c->tune_strip   = in->strip;
     c->tune_merge   = in->merge;
@@ -1500,7 +1500,7 @@ extern "C" int qbt_autotune(qbt_ctx* c, const qbt_view* views, int nviews, const
         bool improved = false;
         for (int knob = 0; knob < 5; knob++) {
             const int* vals = knob == 0 ? blocks : knob == 1 ? refills : nullptr;
-            const int  nv   = knob == 0 ? 3 : knob == 1 ? 6 : 2;
+            const int  nv   = knob == 0 ? 3 : knob == 1 ? 6 : knob == 4 ? 3 : 2;
             for (int k = 0; k < nv; k++) {
                 qbt_tuning t = best;
                 int*       f = knob == 0   ? &t.block_threads

@@ -206,7 +206,7 @@ struct qbt_ctx {
     int          tune_reorder = 0; /* shadow rays in direction-octant order (measured: no gain) */
     int          tune_graph   = 1; /* replay batched renders from a captured hipGraph */
     int          tune_strip   = 0; /* tile order: 0 row-major, S > 0 vertical strips of S tiles */
-    int          tune_merge   = 1; /* exact renders: camera and shadow rays in one launch (k_frame) */
+    int          tune_merge   = 1; /* exact renders: camera and shadow rays in one launch (k_frame): 0 never, 1 where it pays, 2 always */
     /* captured render graph and the launch configuration it was captured for */
     hipGraph_t           graph      = nullptr;
     hipGraphExec_t       graph_exec = nullptr;

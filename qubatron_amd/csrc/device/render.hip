@@ -1020,13 +1020,19 @@ int resident_blocks(K kernel, int bs, size_t lds)
     return per;
 }
 
+/* lanes of the full resident grid of `kernel` */
+template <class K>
+size_t grid_lanes(qbt_ctx* c, K kernel, int bs, size_t lds)
+{
+    return (size_t) resident_blocks(kernel, bs, lds) * (size_t) c->num_cus * (size_t) bs;
+}
+
 /* whether `items` rays fill the full resident grid of `kernel` (at least
  * kFullGridRays per lane) */
 template <class K>
 bool fills_grid(qbt_ctx* c, K kernel, int bs, size_t lds, size_t items)
 {
-    const size_t lanes = (size_t) resident_blocks(kernel, bs, lds) * (size_t) c->num_cus * (size_t) bs;
-    return items >= (size_t) kFullGridRays * lanes;
+    return items >= (size_t) kFullGridRays * grid_lanes(c, kernel, bs, lds);
 }
 
 template <class K>
@@ -1105,12 +1111,16 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     auto           kp     = k_primary<CMP, DUAL, GRID, COUNT, GSTK>;
     auto           ks     = k_shadow<CMP, DUAL, GRID, COUNT, GSTK>;
     auto           kf     = k_frame<CMP, DUAL, GRID, COUNT, GSTK>;
-    /* exact renders: camera and shadow rays in one launch (k_frame) when the
-     * camera pass alone would not fill the full grid (then the end of that
-     * pass, a few long rays on mostly idle SIMDs, is a large share of it);
-     * longer camera passes keep the separate launches at the full grid */
-    const bool merged = c->tune_merge && nl > 0 && !surrogate && !collect && !reorder &&
-                        !fills_grid(c, kp, bs, lds, (size_t) nitems + nspr);
+    /* exact renders: camera and shadow rays in one launch (k_frame). Tuning
+     * merge 2 always; merge 1 where it was measured to pay: at least two
+     * lights (shadow work enough to fill the end of the camera pass, a few
+     * long rays on mostly idle SIMDs) and a camera pass of 1 to 16 rays per
+     * lane of the full grid (longer passes keep the separate launches at the
+     * full grid; shorter ones are latency-bound either way) */
+    const size_t cam  = (size_t) nitems + nspr;
+    const bool   auto_merge = nl >= 2 && cam >= grid_lanes(c, kp, bs, lds) && !fills_grid(c, kp, bs, lds, cam);
+    const bool merged = (c->tune_merge == 2 || (c->tune_merge == 1 && auto_merge)) && nl > 0 && !surrogate &&
+                        !collect && !reorder;
     if (merged && (rc = c->dev_alloc(c->wb_slots, (size_t) nitems * 16 * nl))) return rc;
     /* resident blocks, but no more than one work item per lane: small
      * batches then start fewer waves (less launch and refill-counter

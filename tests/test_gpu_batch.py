@@ -44,8 +44,8 @@ def _same(a, b, what):
         assert np.array_equal(a[k], b[k]), f"{what}: {k} differs ({np.count_nonzero(a[k] != b[k])} elements)"
 
 
-@pytest.mark.parametrize("stack,reorder,strip,merge", [(0, 1, 0, 1), (0, 0, 0, 1), (0, 0, 0, 0), (1, 1, 0, 1),
-                                                      (1, 0, 0, 1), (1, 0, 0, 0), (0, 0, 2, 1), (0, 0, 5, 1)])
+@pytest.mark.parametrize("stack,reorder,strip,merge", [(0, 1, 0, 2), (0, 0, 0, 2), (0, 0, 0, 0), (1, 1, 0, 2),
+                                                      (1, 0, 0, 2), (1, 0, 0, 0), (0, 0, 2, 2), (0, 0, 5, 1)])
 def test_batch_equals_oracle(scene_ctx, stack, reorder, strip, merge):
     ctx, sc, osc = scene_ctx
     ctx.set_tuning(path=1, stack=stack, reorder=reorder, graph=0, strip=strip, merge=merge)
@@ -119,7 +119,7 @@ def test_autotune_picks_a_valid_tuning(scene_ctx):
     best, table = ctx.autotune(views, LIGHTS4[:2], MASK, reps=2)
     assert len(table) >= 8
     assert best["path"] == 1 and best["block_threads"] in (64, 128, 256)
-    assert best["stack"] in (0, 1) and best["reorder"] in (0, 1) and best["merge"] in (0, 1)
+    assert best["stack"] in (0, 1) and best["reorder"] in (0, 1) and best["merge"] in (0, 1, 2)
     assert min(ms for _, ms in table) > 0
     ctx.render(views, LIGHTS4[:2], MASK)
     for i, (w, h) in enumerate(SIZES[:3]):

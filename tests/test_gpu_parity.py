@@ -20,14 +20,14 @@ ALL = q.OUT_ALL | q.RENDER_COUNT
 
 
 @pytest.fixture(params=[dict(path=1, layout=0), dict(path=1, layout=1), dict(path=0, layout=1),
-                        dict(path=1, layout=0, stack=1, reorder=0, strip=3), dict(path=1, layout=0, merge=0)],
-                ids=["wavefront-compact", "wavefront-direct", "fused", "wavefront-gstack", "wavefront-two-launch"])
+                        dict(path=1, layout=0, stack=1, reorder=0, strip=3, merge=2), dict(path=1, layout=0, merge=2)],
+                ids=["wavefront-compact", "wavefront-direct", "fused", "wavefront-gstack-merged", "wavefront-merged"])
 def ctx(gpu_ctx, request):
     """Every render path: wavefront stages on the compact node layout
-    (default: camera and shadow rays in one merged launch), wavefront on the
-    direct layout, the fused kernel, the wavefront stages with global-memory
-    stacks and tiles in 3-tile strips, and the wavefront stages with separate
-    camera and shadow launches."""
+    (default tuning: these small frames take separate camera and shadow
+    launches), wavefront on the direct layout, the fused kernel, and the
+    merged camera-and-shadow launch, on LDS stacks and on global-memory
+    stacks with tiles in 3-tile strips."""
     gpu_ctx.set_tuning(**request.param)
     yield gpu_ctx
     gpu_ctx.set_tuning(path=1, layout=0, stack=0, reorder=0, graph=1, strip=0, merge=1)
@@ -503,12 +503,12 @@ def test_merged_launch_matches_two_launches(gpu_ctx):
         for i, (W, H_) in enumerate([(8, 8), (64, 48), (320, 240)]):
             v = q.view(pos[i], ang[i], W, H_, sc.basecube, sc.depth, i % 2)
             out = {}
-            for merge in (0, 1):
+            for merge in (0, 2):
                 ctx.set_tuning(path=1, layout=0, merge=merge)
                 print(f"[case] merge={merge} {W}x{H_}", flush=True)
                 ctx.render(v, lights, ALL)
                 out[merge] = (ctx.read_frame(0), ctx.stats())
-            (f0, s0), (f1, s1) = out[0], out[1]
+            (f0, s0), (f1, s1) = out[0], out[2]
             for k in f0:
                 assert np.array_equal(f0[k], f1[k]), f"{W}x{H_}: {k} differs"
             for k in ("primary_rays", "shadow_rays", "hits", "expansions", "expansions_primary"):
