@@ -13,8 +13,9 @@
  *              pixel index when the pixel is shaded
  *   k_sel_*    list of the shaded pixels in item order: per-block counts, a
  *              scan of the counts, scatter (no same-address atomics)
- *   k_oct_*    per light, the list reordered by the direction octant of the
- *              shadow ray (tuning reorder = 1)
+ *   k_oct_*    per light, the list partitioned by the direction octant of
+ *              the shadow ray, list order kept within an octant (tuning
+ *              reorder = 1)
  *   (k_mlp_shadow, surrogate.hip: QBT_RENDER_SURROGATE decisions)
  *   k_shadow   one launch for all lights: the ray from each light to each
  *              shaded pixel's entry point (:431-434), light after light in
@@ -413,35 +414,75 @@ __device__ __forceinline__ int shadow_octant(const float4& ip, const float* Lp)
     return (ip.x - Lp[0] < 0.0f ? 1 : 0) | (ip.y - Lp[1] < 0.0f ? 2 : 0) | (ip.z - Lp[2] < 0.0f ? 4 : 0);
 }
 
-/* Octant reordering, pass 1 (SCATTER = false): per (light, octant) counts;
- * pass 2 (SCATTER = true): each pixel appended to its bin of that light's
- * list. One atomic per (wave, light, octant present). */
-template <bool SCATTER>
-__global__ void k_oct_bin(const uint32_t* __restrict__ list, const unsigned* __restrict__ nsel,
-                          const float4* __restrict__ isp, LightsDev L, unsigned* __restrict__ cnt,
-                          uint32_t* __restrict__ lists, size_t stride)
+/* Octant reordering: per light, the shaded-pixel list partitioned by the
+ * direction octant of the shadow ray, each octant's pixels in list order (a
+ * stable partition: neighbouring pixels of one octant stay neighbours).
+ * Pass 1 counts, per block of kSelItems list entries, the entries of each
+ * (light, octant) row into bcnt[row * nb + block]; k_sel_scan turns those
+ * counts into block offsets and row totals, k_oct_scan the totals into each
+ * octant's start in its light's list; pass 2 scatters. */
+__global__ void __launch_bounds__(256) k_oct_count(const uint32_t* __restrict__ list, const unsigned* __restrict__ nsel,
+                                                   const float4* __restrict__ isp, LightsDev L,
+                                                   uint32_t* __restrict__ bcnt, uint32_t nb)
 {
-    const uint32_t i     = blockIdx.x * blockDim.x + threadIdx.x;
-    const int      lane  = threadIdx.x & 63;
-    const bool     valid = i < *nsel;
-    const uint32_t gp    = valid ? list[i] : 0u;
-    const float4   ip    = valid ? isp[gp] : make_float4(0.f, 0.f, 0.f, 0.f);
-    if (__ballot(valid) == 0ull) return; /* uniform over the wave */
-    for (int l = 0; l < L.n; l++) {
-        const int oct     = shadow_octant(ip, L.p[l]);
-        uint64_t  pending = __ballot(valid);
-        while (pending) {
-            const int      leader = __ffsll((unsigned long long) pending) - 1;
-            const int      k      = __shfl(oct, leader, 64);
-            const uint64_t mine   = __ballot(valid && oct == k);
-            uint32_t       base   = 0;
-            if (lane == leader) base = atomicAdd(&cnt[8 * l + k], (unsigned) __popcll(mine));
-            if (SCATTER) {
-                base = __shfl(base, leader, 64);
-                if (valid && oct == k)
-                    lists[(size_t) l * stride + base + (uint32_t) __popcll(mine & ((1ull << lane) - 1ull))] = gp;
+    __shared__ uint32_t h[8 * kNL];
+    const int lane = threadIdx.x & 63;
+    if (threadIdx.x < 8 * kNL) h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t n = *nsel;
+    for (int c = 0; c < kSelItems / 256; c++) {
+        const uint32_t i     = blockIdx.x * kSelItems + c * 256 + threadIdx.x;
+        const bool     valid = i < n;
+        const float4   ip    = valid ? isp[list[i]] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int l = 0; l < L.n; l++) {
+            const int oct = shadow_octant(ip, L.p[l]);
+            for (int k = 0; k < 8; k++) {
+                const uint64_t m = __ballot(valid && oct == k);
+                if (lane == 0 && m) atomicAdd(&h[8 * l + k], (uint32_t) __popcll(m));
             }
-            pending &= ~mine;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 * L.n) bcnt[(size_t) threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+/* pass 2: entry i to lists[l * stride + start of its octant + block offset
+ * + its rank among the block's entries of that octant before it] */
+__global__ void __launch_bounds__(256) k_oct_scatter(const uint32_t* __restrict__ list,
+                                                     const unsigned* __restrict__ nsel, const float4* __restrict__ isp,
+                                                     LightsDev L, const uint32_t* __restrict__ boff, uint32_t nb,
+                                                     const unsigned* __restrict__ cur, uint32_t* __restrict__ lists,
+                                                     size_t stride)
+{
+    __shared__ uint32_t base[8 * kNL];
+    __shared__ uint32_t ws[4][8];
+    const int           lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x < 8 * L.n) base[threadIdx.x] = cur[threadIdx.x] + boff[(size_t) threadIdx.x * nb + blockIdx.x];
+    __syncthreads();
+    const uint32_t n = *nsel;
+    for (int c = 0; c < kSelItems / 256; c++) {
+        const uint32_t i     = blockIdx.x * kSelItems + c * 256 + threadIdx.x;
+        const bool     valid = i < n;
+        const uint32_t gp    = valid ? list[i] : 0u;
+        const float4   ip    = valid ? isp[gp] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int l = 0; l < L.n; l++) {
+            const int oct  = shadow_octant(ip, L.p[l]);
+            uint32_t  rank = 0;
+            for (int k = 0; k < 8; k++) {
+                const uint64_t m = __ballot(valid && oct == k);
+                if (lane == 0) ws[w][k] = (uint32_t) __popcll(m);
+                if (valid && oct == k) rank = (uint32_t) __popcll(m & ((1ull << lane) - 1ull));
+            }
+            __syncthreads();
+            if (valid) {
+                uint32_t pre = base[8 * l + oct];
+                for (int v = 0; v < w; v++) pre += ws[v][oct];
+                lists[(size_t) l * stride + pre + rank] = gp;
+            }
+            __syncthreads();
+            if (threadIdx.x < 8) base[8 * l + threadIdx.x] += ws[0][threadIdx.x] + ws[1][threadIdx.x] +
+                                                              ws[2][threadIdx.x] + ws[3][threadIdx.x];
+            __syncthreads();
         }
     }
 }
@@ -1097,7 +1138,8 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
         return rc;
     if (reorder && (rc = c->dev_alloc(c->wb_lists, (size_t) npix * 4 * nl))) return rc;
     const uint32_t nb_max = ((nitems > npix ? nitems : npix) + kSelItems - 1) / kSelItems + 1;
-    if ((rc = c->dev_alloc(c->wb_bcnt, (size_t) nb_max * 4 * (nl > 0 ? nl : 1)))) return rc;
+    /* rows of the compactions: one per light, or one per (light, octant) */
+    if ((rc = c->dev_alloc(c->wb_bcnt, (size_t) nb_max * 4 * (reorder ? 8 * nl : (nl > 0 ? nl : 1))))) return rc;
     if (surrogate && (rc = c->dev_alloc(c->wb_fb, (size_t) npix * 4 * nl))) return rc;
     if (COUNT && nl > 0 && (rc = c->dev_alloc(c->wb_lexp, (size_t) npix * 4 * nl))) return rc;
 
@@ -1174,10 +1216,12 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
         c->prof_mark_end();
         compact(st, PredCand{po.cand, nitems}, nitems, 1, (uint32_t*) c->wb_bcnt.p, q + Q_NSEL, 1, items, 0);
         if (reorder) {
-            k_oct_bin<false><<<nblk(npix), 256, 0, st>>>(items, q + Q_NSEL, po.isp, L, q + Q_OCT, nullptr, 0);
+            const uint32_t nbo = (npix + kSelItems - 1) / kSelItems + 1;
+            k_oct_count<<<nbo, 256, 0, st>>>(items, q + Q_NSEL, po.isp, L, (uint32_t*) c->wb_bcnt.p, nbo);
+            k_sel_scan<<<8 * nl, 1024, 0, st>>>((uint32_t*) c->wb_bcnt.p, nbo, q + Q_OCT, 1);
             k_oct_scan<<<1, 64, 0, st>>>(q + Q_OCT, q + Q_CUR, nl);
-            k_oct_bin<true><<<nblk(npix), 256, 0, st>>>(items, q + Q_NSEL, po.isp, L, q + Q_CUR,
-                                                        (uint32_t*) c->wb_lists.p, npix);
+            k_oct_scatter<<<nbo, 256, 0, st>>>(items, q + Q_NSEL, po.isp, L, (const uint32_t*) c->wb_bcnt.p, nbo,
+                                                q + Q_CUR, (uint32_t*) c->wb_lists.p, npix);
         }
         if (surrogate) {
             if ((err = qbt_surrogate_decide(c, s, DUAL, L, lists, lstride, q + Q_NSEL, npix))) return err;
@@ -1205,7 +1249,7 @@ static int render_batch_t(qbt_ctx* c, const SceneDev& s, const BatchHost& bh, co
     if (!use_graph) {
         if ((rc = enqueue())) return rc;
         HIPCHK(hipGetLastError());
-        c->last.launches += merged ? 5 + (COUNT ? 1 : 0) : 6 + (nl > 0 ? 1 : 0) + (reorder ? 3 : 0) + (surrogate ? 4 : 0);
+        c->last.launches += merged ? 5 + (COUNT ? 1 : 0) : 6 + (nl > 0 ? 1 : 0) + (reorder ? 4 : 0) + (surrogate ? 4 : 0);
         return QBT_OK;
     }
 
