@@ -61,18 +61,22 @@ def test_batch_equals_oracle(scene_ctx, stack, reorder, strip, merge):
         _same(ctx.read_frame(i), o, f"view {i} {w}x{h}")
 
 
-def test_graph_replays_match_direct(scene_ctx):
+@pytest.mark.parametrize("merge", [1, 2])
+def test_graph_replays_match_direct(scene_ctx, merge):
     """Capture once, replay for other poses of the same shape, with direct
-    (uncaptured) renders in between; every replay equals the direct render."""
+    (uncaptured) renders in between; every replay equals the direct render.
+    merge 2: the captured sequence is the merged camera-and-shadow launch,
+    whose hand-over slots are reset by a kernel of the graph on every replay."""
     ctx, sc, _ = scene_ctx
+    ctx.set_tuning(merge=merge)
     for rep in range(4):
         views, _, _ = _views(sc, 100 + rep)
         ctx.set_tuning(graph=0)
-        print(f"[case] direct render {rep}", flush=True)
+        print(f"[case] direct render {rep} merge={merge}", flush=True)
         ctx.render(views, LIGHTS4, MASK)
         ref = _frames(ctx, len(views))
         ctx.set_tuning(graph=1)
-        print(f"[case] graph render {rep}", flush=True)
+        print(f"[case] graph render {rep} merge={merge}", flush=True)
         ctx.render(views, LIGHTS4, MASK)
         got = _frames(ctx, len(views))
         for i in range(len(views)):
