@@ -105,7 +105,11 @@ def parse():
     ap.add_argument("--sur-final-lr", type=float, default=0.0, help="rate of the final steps (0 = --sur-lr)")
     ap.add_argument("--sur-decay", type=float, default=0.0,
                     help="final steps: cosine decay of the rate to this fraction (0 = constant)")
-    ap.add_argument("--sur-band", type=float, nargs=2, default=(0.1, 0.9))
+    ap.add_argument("--sur-band", type=float, nargs=2, default=None,
+                    help="fixed uncertainty band (default: calibrated on held-out frames to --sur-target)")
+    ap.add_argument("--sur-target", type=float, default=0.01,
+                    help="shadow-bit mismatch the calibrated band allows on the held-out frames")
+    ap.add_argument("--sur-cal-frames", type=int, default=8, help="held-out exact frames of the band calibration")
     ap.add_argument("--uhd-frames", type=int, default=5, help="frames of the 3840x2160 leg (0 = skip)")
     ap.add_argument("--dyn-frames", type=int, default=5, help="frames of the dynamic-tree leg (0 = skip)")
     ap.add_argument("--batch-views", type=int, default=64, help="views of the batched leg (0 = skip)")
@@ -310,7 +314,18 @@ def surrogate_leg(args, q, ctx, sc, views, Wm, K, mask, exact_wall, exact_rays, 
     train_s = time.time() - t0
     log(f"surrogate: trained on {info['samples']} samples from {args.sur_frames} frames (stride {stride}), "
         f"{info['steps']} Adam steps in {train_s:.2f}s, last loss {info['loss_last']:.4f}")
-    ctx.surrogate_set_band(*args.sur_band)
+    cal = None
+    if args.sur_band is None:  # the widest band within the mismatch target on frames not trained on
+        cpos, cang = sc.poses(args.sur_cal_frames, seed=args.pose_seed + 4243)
+        cviews = [q.view(cpos[i], cang[i], W, H, sc.basecube, sc.depth) for i in range(args.sur_cal_frames)]
+        cal = S.calibrate_band(ctx, cviews, lights, sc.basecube, target=args.sur_target)
+        cal["pose_seed"] = args.pose_seed + 4243
+        band = cal["band"]
+        log(f"surrogate: band [{band[0]:.3f}, {band[1]:.3f}] calibrated on {args.sur_cal_frames} held-out frames "
+            f"({cal['samples']} queries): decided {cal['held_out_decided']:.3f}, mismatch {cal['held_out_mismatch']:.4f}")
+    else:
+        band = list(args.sur_band)
+    ctx.surrogate_set_band(*band)
     smask = mask | q.RENDER_SURROGATE
     for i in range(Wm):
         ctx.render(views[i], lights, smask)
@@ -357,7 +372,8 @@ def surrogate_leg(args, q, ctx, sc, views, Wm, K, mask, exact_wall, exact_rays, 
            "ms_per_step": round(wall * 1e3 / K, 4), "mismatch": round(bad / max(tot, 1), 6),
            "mismatched_queries": bad, "shadow_queries": tot,
            "predicted_fraction": round(pred / max(pred + traced, 1), 4),
-           "rays_resolved_per_frame": round(resolved / K), "band": list(args.sur_band),
+           "rays_resolved_per_frame": round(resolved / K), "band": [round(b, 4) for b in band],
+           "band_calibration": cal,
            "kernel_ms_per_frame": {"primary": round(km[KIND_PRIMARY] / K, 4),
                                    "shadow": round(km[KIND_SHADOW] / K, 4),
                                    "mlp_decision": round(km[KIND_SURROGATE] / K, 4)},

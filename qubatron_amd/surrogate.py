@@ -54,3 +54,52 @@ def shadow_mismatch(exact, surrogate, nlights):
     diff = (exact["shadow"] ^ surrogate["shadow"])[shaded]
     bad = sum(int(np.count_nonzero((diff >> l) & 1)) for l in range(nlights))
     return bad, int(np.count_nonzero(shaded)) * nlights
+
+
+def choose_band(p, y, target=0.01, steps=500):
+    """The uncertainty band [lo, hi] that decides the most queries while the
+    decided ones that disagree with their label stay within target of all
+    queries: a query is decided 0 when p < lo and 1 when p > hi (the render
+    path's rule). lo runs over a grid of steps values in (0, 0.5], hi over
+    [0.5, 1). Returns (lo, hi, decided fraction, mismatch fraction)."""
+    p = np.asarray(p, dtype=np.float64).ravel()
+    y = np.asarray(y).ravel().astype(bool)
+    n = p.size
+    if n == 0:
+        raise ValueError("choose_band: no samples")
+    lo = np.linspace(0.0, 0.5, steps + 1)[1:]
+    hi = np.linspace(0.5, 1.0, steps + 1)[:-1]
+    ps, pp, pn = np.sort(p), np.sort(p[y]), np.sort(p[~y])
+    n0 = np.searchsorted(ps, lo, side="left")                 # p < lo: decided 0
+    e0 = np.searchsorted(pp, lo, side="left")                 # ... labelled 1
+    n1 = n - np.searchsorted(ps, hi, side="right")            # p > hi: decided 1
+    e1 = pn.size - np.searchsorted(pn, hi, side="right")      # ... labelled 0
+    err = e0[:, None] + e1[None, :]
+    dec = np.where(err <= target * n, n0[:, None] + n1[None, :], -1)
+    i, j = np.unravel_index(int(np.argmax(dec)), dec.shape)
+    if dec[i, j] < 0:  # no band of the grid meets the target: decide (almost) nothing
+        a, b = 1e-6, 1.0 - 1e-6
+        d0, d1 = p < a, p > b
+        return a, b, float(np.count_nonzero(d0) + np.count_nonzero(d1)) / n, \
+            float(np.count_nonzero(d0 & y) + np.count_nonzero(d1 & ~y)) / n
+    return float(lo[i]), float(hi[j]), float(n0[i] + n1[j]) / n, float(e0[i] + e1[j]) / n
+
+
+def calibrate_band(ctx, views, lights, basecube, target=0.01, nsamples=1 << 23):
+    """Collect every stride-th shadow query of exact renders of views (frames
+    the network was not trained on), predict them, and set the band that
+    choose_band picks for them. Empties the training store. Returns a dict of
+    the band and the held-out figures."""
+    ctx.render(views[0], lights, _ffi.OUT_FLAGS)
+    per_frame = max(1, ctx.stats()["shadow_rays"])
+    stride = max(1, int(round(len(views) * per_frame / nsamples)))
+    ctx.surrogate_reserve(nsamples, stride)
+    for v in views:
+        ctx.render(v, lights, _ffi.OUT_FLAGS | _ffi.RENDER_COLLECT)
+    s = ctx.surrogate_read_samples()
+    p = ctx.surrogate_predict(s[:, 0:3], s[:, 3:6], s[:, 6:9], basecube)
+    lo, hi, dec, mis = choose_band(p, s[:, 9] > 0.5, target)
+    ctx.surrogate_set_band(lo, hi)
+    return {"band": [lo, hi], "frames": len(views), "samples": int(s.shape[0]), "stride": stride,
+            "target_mismatch": target, "held_out_decided": round(dec, 4), "held_out_mismatch": round(mis, 5)}
+
