@@ -645,9 +645,10 @@ __device__ __forceinline__ uint32_t slot_xchg(uint32_t* p, uint32_t v)
     return __hip_atomic_exchange(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-/* lane state of k_frame, packed to keep the kernel at the separate
- * launches' register count: gp = the ray's batch pixel; k = the camera ray's
- * work item, or the shadow ray's slot; tag = kind bits, the sprite slot of a
+/* lane state of k_frame, packed into three words so that the kernel needs
+ * few more registers than the separate launches (86-87 against 78-79, 5
+ * waves per SIMD): gp = the ray's batch pixel; k = the camera ray's work
+ * item, or the shadow ray's slot; tag = kind bits, the sprite slot of a
  * sprite trace, and the lights still deferred to this lane (bits 8..15) */
 constexpr uint32_t kTagShadow = 0x80000000u; /* the lane's ray is a shadow ray */
 constexpr uint32_t kTagSprite = 0x40000000u; /* camera-side sprite trace; low bits: sprite slot */
@@ -686,8 +687,9 @@ __global__ void __launch_bounds__(256)
     TraceState  t;
     unsigned    n_exp = 0;
     int         fin   = ST_CONT; /* status of a ray that ended and awaits its outputs */
-    /* shaded camera hits and shadow rays traced, per wave: ballots taken in
-     * wave-uniform code, so the sums sit in scalar registers */
+    /* shaded camera hits, shadow rays traced and of those the deferred ones,
+     * per wave: ballots taken in wave-uniform code, so the sums sit in scalar
+     * registers */
     uint32_t    w_hit = 0, w_ray = 0, w_def = 0;
 
     auto abort_count = [&]() { atomicAdd(&counters[CNT_ABORT], 1ull); }; /* rare: no register kept for it */
