@@ -132,3 +132,42 @@ def test_config5_surrogate_frames(trained):
     assert saved > 0.2 * traced_all
     assert tot > 1_000_000
     assert mismatch < 0.02 and frac > 0.2, (mismatch, frac)
+
+
+def test_config5_calibrated_band(trained):
+    """The band calibrated on 4 held-out exact frames to 1% mismatch
+    (qubatron_amd.surrogate.calibrate_band), used on 3 other frames: decisions
+    follow the calibrated band, the mismatch stays near the target (the
+    frames differ from the calibration frames, so 2% is allowed) and the
+    network decides a part of the queries. How large a part depends on the
+    view: two thirds of the calibration frames' queries, 15% of these three
+    frames' (0.8% mismatch), 39% of the bench's ten."""
+    sc, ctx, pnt, nrm, col, n, info = trained
+    lights = np.asarray(sc.lights, np.float32)
+    cpos, cang = sc.poses(4, seed=3)
+    cal = S.calibrate_band(ctx, [q.view(cpos[i], cang[i], W, H_, sc.basecube, sc.depth) for i in range(4)],
+                           lights, sc.basecube, target=0.01)
+    lo, hi = cal["band"]
+    print(f"[surrogate] calibrated band [{lo:.3f}, {hi:.3f}]: {cal}", flush=True)
+    assert 0.0 < lo <= 0.5 <= hi < 1.0 and cal["held_out_mismatch"] <= 0.01
+    pos, ang = sc.poses(3, seed=11)
+    bad = tot = pred = 0
+    for i in range(3):
+        v = q.view(pos[i], ang[i], W, H_, sc.basecube, sc.depth)
+        print(f"[case] calibrated band frame {i}", flush=True)
+        ctx.render(v, lights, q.OUT_ALL)
+        ex = ctx.read_frame(0)
+        ctx.render(v, lights, q.OUT_ALL | q.RENDER_SURROGATE)
+        su = ctx.read_frame(0)
+        pred += ctx.stats()["predicted"]
+        sh = ((ex["flags"] & q.PX_SHADED) != 0) & ((ex["flags"] & q.PX_DISCARD) == 0)
+        ii = np.nonzero(sh.ravel())[0]
+        pr = ctx.surrogate_predict(ex["isp"].reshape(-1, 4)[ii, :3], nrm[ex["hit"].ravel()[ii]],
+                                   np.repeat(lights[:1], ii.size, 0), sc.basecube)
+        decided = (su["flags"].ravel()[ii] & q.PX_PREDICTED) != 0
+        assert not np.any(decided & (pr > lo + 1e-3) & (pr < hi - 1e-3))  # inside the band: traced
+        assert np.all(decided[(pr < lo - 1e-3) | (pr > hi + 1e-3)])       # outside: decided
+        b, t = S.shadow_mismatch(ex, su, 1)
+        bad, tot = bad + b, tot + t
+    print(f"[surrogate] calibrated band on new frames: mismatch {bad / tot:.4f}, decided {pred / tot:.3f}", flush=True)
+    assert bad / tot < 0.02 and pred / tot > 0.05
