@@ -646,7 +646,7 @@ __device__ __forceinline__ uint32_t slot_xchg(uint32_t* p, uint32_t v)
 }
 
 /* lane state of k_frame, packed into three words so that the kernel needs
- * few more registers than the separate launches (86-87 against 78-79, 5
+ * few more registers than the separate launches (88-89 against 78-79, 5
  * waves per SIMD): gp = the ray's batch pixel; k = the camera ray's work
  * item, or the shadow ray's slot; tag = kind bits, the sprite slot of a
  * sprite trace, and the lights still deferred to this lane (bits 8..15) */
